@@ -1,0 +1,431 @@
+// Coarse levels of the MGCG hierarchy: level-1 rediscretisation from child
+// densities, element-Galerkin operators, the coarsest-level penalty,
+// dense assembly, blocked FP64 Cholesky, the explicit inverse of the
+// factor and the two triangular products that apply it in every V-cycle.
+#include "gputo_internal.h"
+
+namespace gputo {
+
+__host__ __device__ constexpr int lidx2(int x, int y, int z) { return 4 * z + (y ? (x ? 2 : 3) : (x ? 1 : 0)); }
+__host__ __device__ constexpr int pidx2(int m, int n) { return n <= m ? m * (m + 1) / 2 + n : n * (n + 1) / 2 + m; }
+__device__ constexpr int kIE[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__device__ constexpr int kJE[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+__device__ constexpr int kKE[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+// ---------------------------------------------------------------------------
+// Level-1 rediscretisation (coarseAe2c, mgcg1.cu:2042-2070): the packed
+// coarse element matrix is sum_na CX[child(na)] * Aelm[:, na] where Aelm
+// holds the h=2 Gauss-point stiffness matrices and child na = 4 iP + 2 kP
+// + jP.  One thread per coarse element; Aelm is read through the scalar
+// cache (uniform addresses).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_coarse_l1(Grid gc, const double *__restrict__ Aelm,
+                                                  const double *__restrict__ CXf, double *__restrict__ A)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= gc.nel) return;
+    const int j = (int)(e % gc.ny), k = (int)((e / gc.ny) % gc.nz), i = (int)(e / ((long)gc.ny * gc.nz));
+    const int fny = 2 * gc.ny, fnz = 2 * gc.nz;
+    double cx[8];
+#pragma unroll
+    for (int iP = 0; iP < 2; ++iP)
+#pragma unroll
+        for (int kP = 0; kP < 2; ++kP)
+#pragma unroll
+            for (int jP = 0; jP < 2; ++jP)
+                cx[4 * iP + 2 * kP + jP] = CXf[(long)fny * fnz * (2 * i + iP) + (long)fny * (2 * k + kP) + 2 * j + jP];
+    for (int m = 0; m < 300; ++m) {
+        double s = 0.0;
+#pragma unroll
+        for (int na = 0; na < 8; ++na) s += cx[na] * Aelm[8 * m + na];
+        A[(long)m * gc.nel + e] = s;
+    }
+}
+
+void launch_coarse_l1(hipStream_t s, const Grid &gc, const double *Aelm, const double *CXf, double *A)
+{
+    k_coarse_l1<<<(unsigned)((gc.nel + 255) / 256), 256, 0, s>>>(gc, Aelm, CXf, A);
+}
+
+// ---------------------------------------------------------------------------
+// Element Galerkin (coarseAe3c, mgcg1.cu:1678-1743): A_c = P (sum of the 8
+// children on the 27-node patch) P^T.  P is the trilinear interpolation of
+// the coarse element's 8 nodes onto its 27 fine nodes, so the (a,b) node
+// block of A_c is
+//   sum_ch sum_{p,q} w(a, ch+p) w(b, ch+q) A_ch[p q block],
+// with w(a, x) = prod_axis (1 - |2 a_axis - x_axis| / 2).  One 64-thread
+// block per coarse element stages its 8 children (full 24x24 each) in LDS;
+// thread (a >= b) forms one 3x3 block.  Children come from the packed
+// matrices of the finer level, or are CX*Ke on level 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double trilin_w(int a, int x0, int x1, int x2)
+{
+    // coarse local node a (offsets kIE/kJE/kKE), fine patch node (x0,x1,x2) in {0,1,2}^3
+    const double w0 = 1.0 - 0.5 * abs(2 * kIE[a] - x0);
+    const double w1 = 1.0 - 0.5 * abs(2 * kJE[a] - x1);
+    const double w2 = 1.0 - 0.5 * abs(2 * kKE[a] - x2);
+    return w0 * w1 * w2;
+}
+
+__global__ __launch_bounds__(64) void k_galerkin(Grid gc, const double *__restrict__ Af, const double *__restrict__ Ke,
+                                                const double *__restrict__ CXf, double *__restrict__ Ac)
+{
+    __shared__ double S[8][576];
+    const long e = blockIdx.x;
+    const int j = (int)(e % gc.ny), k = (int)((e / gc.ny) % gc.nz), i = (int)(e / ((long)gc.ny * gc.nz));
+    const int fny = 2 * gc.ny, fnz = 2 * gc.nz;
+    const long nelf = 8 * gc.nel;
+    for (int ch = 0; ch < 8; ++ch) {
+        const int ci = kIE[ch], cj = kJE[ch], ck = kKE[ch];
+        const long eP = (long)fny * fnz * (2 * i + ci) + (long)fny * (2 * k + ck) + 2 * j + cj;
+        const double cx = Af ? 0.0 : CXf[eP];
+        for (int m = threadIdx.x; m < 576; m += 64) {
+            const int r = m / 24, c = m - 24 * r;
+            S[ch][m] = Af ? Af[(long)pidx2(r, c) * nelf + eP] : cx * Ke[m];
+        }
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q >= 36) return;
+    int a = 0;
+    while ((a + 1) * (a + 2) / 2 <= q) ++a;
+    const int b = q - a * (a + 1) / 2;
+    double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int ch = 0; ch < 8; ++ch) {
+        const int ci = kIE[ch], cj = kJE[ch], ck = kKE[ch];
+        for (int p = 0; p < 8; ++p) {
+            const double wa = trilin_w(a, ci + kIE[p], cj + kJE[p], ck + kKE[p]);
+            if (wa == 0.0) continue;
+            for (int r = 0; r < 8; ++r) {
+                const double wb = trilin_w(b, ci + kIE[r], cj + kJE[r], ck + kKE[r]);
+                if (wb == 0.0) continue;
+                const double w = wa * wb;
+                const double *s = &S[ch][24 * (3 * p) + 3 * r];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) blk[c][d] += w * s[24 * c + d];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int r = 3 * a + c, s = 3 * b + d;
+            if (r >= s) Ac[(long)pidx2(r, s) * gc.nel + e] = blk[c][d];
+        }
+}
+
+void launch_coarse_galerkin(hipStream_t s, const Grid &gc, const double *Af, const double *Ke, const double *CXf,
+                            double *Ac)
+{
+    k_galerkin<<<(unsigned)gc.nel, 64, 0, s>>>(gc, Af, Ke, CXf, Ac);
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest-level penalty (CorrCsAc, mgcg1.cu:1368-1402): the diagonal of
+// every incident element matrix is set to 1e10 for each fixed DOF.  The
+// reference's table {5,6,1,2,4,7,0,3} is the y-mirror of the node's own
+// local index {6,5,2,1,7,4,3,0}; `mirrored` selects it.  Threads that hit
+// the same entry store the same value.
+// ---------------------------------------------------------------------------
+__global__ void k_corrcsa(Grid g, const int *__restrict__ fixed, int nfd, double *__restrict__ A, int mirrored)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nfd) return;
+    const int tmir[8] = {5, 6, 1, 2, 4, 7, 0, 3};
+    const int town[8] = {6, 5, 2, 1, 7, 4, 3, 0};
+    const int dof = fixed[n];
+    const int idof = dof % 3, nod = dof / 3;
+    const int i = nod / (g.nnz * g.nny);
+    const int k = (nod - g.nnz * g.nny * i) / g.nny;
+    const int j = nod - g.nnz * g.nny * i - g.nny * k;
+    for (int di = -1; di <= 0; di++)
+        for (int dk = -1; dk <= 0; dk++)
+            for (int dj = -1; dj <= 0; dj++) {
+                const int iP = i + di, jP = j + dj, kP = k + dk;
+                if (iP < 0 || iP > g.nx - 1 || jP < 0 || jP > g.ny - 1 || kP < 0 || kP > g.nz - 1) continue;
+                const long e = (long)g.ny * g.nz * iP + (long)g.ny * kP + jP;
+                const int t = 4 * (di + 1) + 2 * (dk + 1) + dj + 1;
+                const int n1 = mirrored ? tmir[t] : town[t];
+                const int m = 3 * n1 + idof;
+                A[(long)pidx2(m, m) * g.nel + e] = 1e10;
+            }
+}
+
+void launch_corrcsa(hipStream_t s, const Grid &g, const int *fixed_d, int nfd, double *A, int mirrored)
+{
+    if (nfd > 0) k_corrcsa<<<(nfd + 255) / 256, 256, 0, s>>>(g, fixed_d, nfd, A, mirrored);
+}
+
+// ---------------------------------------------------------------------------
+// Dense coarsest assembly (assembAc, mgcg1.cu:1747-1806): one thread per
+// matrix row gathers the 8 incident element matrices.  The matrix is padded
+// to N = roundup(n, 32) with an identity block so the blocked kernels below
+// work on whole 32x32 tiles.
+// ---------------------------------------------------------------------------
+constexpr int NB = 32;
+
+__global__ void k_assemble(Grid g, const double *__restrict__ A, double *__restrict__ AQ, int N)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    if (r >= g.ndof) {
+        AQ[r * N + r] = 1.0;
+        return;
+    }
+    const long nod = r / 3;
+    const int c = (int)(r % 3);
+    const int j = (int)(nod % g.nny), k = (int)((nod / g.nny) % g.nnz), i = (int)(nod / ((long)g.nny * g.nnz));
+    for (int ei = 0; ei < 2; ++ei)
+        for (int ek = 0; ek < 2; ++ek)
+            for (int ej = 0; ej < 2; ++ej) {
+                const int ex = i - 1 + ei, ey = j - 1 + ej, ez = k - 1 + ek;
+                if (ex < 0 || ex >= g.nx || ey < 0 || ey >= g.ny || ez < 0 || ez >= g.nz) continue;
+                const long e = (long)g.ny * g.nz * ex + (long)g.ny * ez + ey;
+                const int la = lidx2(1 - ei, 1 - ej, 1 - ek);
+                for (int lb = 0; lb < 8; ++lb) {
+                    const long nb = (long)g.nny * g.nnz * (ex + kIE[lb]) + (long)g.nny * (ez + kKE[lb]) + (ey + kJE[lb]);
+                    for (int d = 0; d < 3; ++d) AQ[r * N + 3 * nb + d] += A[(long)pidx2(3 * la + c, 3 * lb + d) * g.nel + e];
+                }
+            }
+}
+
+void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ)
+{
+    const int N = (int)((g.ndof + NB - 1) / NB * NB);
+    launch_fill(s, AQ, (long)N * N, 0.0);
+    k_assemble<<<(N + 255) / 256, 256, 0, s>>>(g, A, AQ, N);
+}
+
+// ---------------------------------------------------------------------------
+// Blocked right-looking Cholesky A = L L^T on the N x N row-major matrix
+// (lower triangle used), NB = 32: factor the diagonal tile in LDS, solve
+// the panel below it, update the trailing lower triangle.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_potrf_diag(double *A, int N, int k0, int *flag)
+{
+    __shared__ double L[NB][NB + 1];
+    for (int q = threadIdx.x; q < NB * NB; q += 256) {
+        const int r = q / NB, c = q % NB;
+        L[r][c] = c <= r ? A[(long)(k0 + r) * N + k0 + c] : 0.0;
+    }
+    __syncthreads();
+    for (int jj = 0; jj < NB; ++jj) {
+        if (threadIdx.x == 0) {
+            double d = L[jj][jj];
+            if (!(d > 0.0)) {  // not positive definite (or NaN)
+                flag[0] = 1;
+                d = 1.0;
+            }
+            L[jj][jj] = sqrt(d);
+        }
+        __syncthreads();
+        if (threadIdx.x > jj && threadIdx.x < NB) L[threadIdx.x][jj] /= L[jj][jj];
+        __syncthreads();
+        for (int q = threadIdx.x; q < NB * NB; q += 256) {
+            const int r = q / NB, c = q % NB;
+            if (r > jj && c > jj && c <= r) L[r][c] -= L[r][jj] * L[c][jj];
+        }
+        __syncthreads();
+    }
+    for (int q = threadIdx.x; q < NB * NB; q += 256) {
+        const int r = q / NB, c = q % NB;
+        if (c <= r) A[(long)(k0 + r) * N + k0 + c] = L[r][c];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_trsm_panel(double *A, int N, int k0)
+{
+    __shared__ double L[NB][NB + 1];
+    for (int q = threadIdx.x; q < NB * NB; q += 256) {
+        const int r = q / NB, c = q % NB;
+        L[r][c] = c <= r ? A[(long)(k0 + r) * N + k0 + c] : 0.0;
+    }
+    __syncthreads();
+    const long row = (long)k0 + NB + (long)blockIdx.x * 256 + threadIdx.x;
+    if (row >= N) return;
+    double x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = A[row * N + k0 + c];
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+        double s = x[jj];
+#pragma unroll
+        for (int m = 0; m < jj; ++m) s -= x[m] * L[jj][m];
+        x[jj] = s / L[jj][jj];
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) A[row * N + k0 + c] = x[c];
+}
+
+__device__ __forceinline__ void tri_index(int b, int &ti, int &tj)
+{
+    // b-th tile of a lower-triangular tile set: (ti, tj) with tj <= ti
+    int t = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while ((t + 1) * (t + 2) / 2 <= b) ++t;
+    while (t * (t + 1) / 2 > b) --t;
+    ti = t;
+    tj = b - t * (t + 1) / 2;
+}
+
+__global__ __launch_bounds__(256) void k_syrk_update(double *A, int N, int k0)
+{
+    __shared__ double Pi[NB][NB + 1], Pj[NB][NB + 1];
+    int ti, tj;
+    tri_index(blockIdx.x, ti, tj);
+    const int base = k0 + NB;
+    const int r0 = base + ti * NB, c0 = base + tj * NB;
+    for (int q = threadIdx.x; q < NB * NB; q += 256) {
+        const int r = q / NB, c = q % NB;
+        Pi[r][c] = A[(long)(r0 + r) * N + k0 + c];
+        Pj[r][c] = A[(long)(c0 + r) * N + k0 + c];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < NB * NB; q += 256) {
+        const int r = q / NB, c = q % NB;
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < NB; ++m) s += Pi[r][m] * Pj[c][m];
+        A[(long)(r0 + r) * N + c0 + c] -= s;
+    }
+}
+
+// Inverse of each 32x32 diagonal factor tile, written into Linv.
+__global__ __launch_bounds__(64) void k_trtri_diag(const double *A, double *Linv, int N)
+{
+    __shared__ double L[NB][NB + 1];
+    const int k0 = blockIdx.x * NB;
+    for (int q = threadIdx.x; q < NB * NB; q += 64) {
+        const int r = q / NB, c = q % NB;
+        L[r][c] = c <= r ? A[(long)(k0 + r) * N + k0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int col = threadIdx.x;
+    if (col >= NB) return;
+    double x[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        double s = (r == col) ? 1.0 : 0.0;
+#pragma unroll
+        for (int m = 0; m < r; ++m) s -= L[r][m] * x[m];
+        x[r] = (r < col) ? 0.0 : s / L[r][r];
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) Linv[(long)(k0 + r) * N + k0 + col] = x[r];
+}
+
+// Block row ib of Linv: Linv[ib][jb] = -inv(L[ib][ib]) sum_{kb=jb}^{ib-1}
+// L[ib][kb] Linv[kb][jb], one block per jb < ib.
+__global__ __launch_bounds__(256) void k_linv_row(const double *A, double *Linv, int N, int ib)
+{
+    __shared__ double Ta[NB][NB + 1], Tb[NB][NB + 1];
+    const int jb = blockIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    for (int kb = jb; kb < ib; ++kb) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < NB * NB; q += 256) {
+            const int r = q / NB, c = q % NB;
+            Ta[r][c] = A[(long)(ib * NB + r) * N + kb * NB + c];
+            Tb[r][c] = Linv[(long)(kb * NB + r) * N + jb * NB + c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            const int r = q / NB, c = q % NB;
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < NB; ++m) s += Ta[r][m] * Tb[m][c];
+            acc[u] += s;
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < NB * NB; q += 256) {
+        const int r = q / NB, c = q % NB;
+        Ta[r][c] = Linv[(long)(ib * NB + r) * N + ib * NB + c];  // inv(L[ib][ib])
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        Tb[q / NB][q % NB] = acc[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        const int r = q / NB, c = q % NB;
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < NB; ++m) s += Ta[r][m] * Tb[m][c];
+        Linv[(long)(ib * NB + r) * N + jb * NB + c] = -s;
+    }
+}
+
+__global__ void k_transpose(const double *__restrict__ In, double *__restrict__ Out, int N)
+{
+    __shared__ double T[NB][NB + 1];
+    const int bi = blockIdx.y * NB, bj = blockIdx.x * NB;
+    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) T[q / NB][q % NB] = In[(long)(bi + q / NB) * N + bj + q % NB];
+    __syncthreads();
+    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) Out[(long)(bj + q / NB) * N + bi + q % NB] = T[q % NB][q / NB];
+}
+
+int coarse_factor(hipStream_t s, Coarse &C)
+{
+    const int N = (C.n + NB - 1) / NB * NB;
+    const int nb = N / NB;
+    hipMemsetAsync(C.flag, 0, sizeof(int), s);
+    for (int kb = 0; kb < nb; ++kb) {
+        const int k0 = kb * NB;
+        k_potrf_diag<<<1, 256, 0, s>>>(C.AQ, N, k0, C.flag);
+        const int rows = N - k0 - NB;
+        if (rows > 0) {
+            k_trsm_panel<<<(rows + 255) / 256, 256, 0, s>>>(C.AQ, N, k0);
+            const int nt = rows / NB;
+            k_syrk_update<<<nt * (nt + 1) / 2, 256, 0, s>>>(C.AQ, N, k0);
+        }
+    }
+    launch_fill(s, C.Linv, (long)N * N, 0.0);
+    k_trtri_diag<<<nb, 64, 0, s>>>(C.AQ, C.Linv, N);
+    for (int ib = 1; ib < nb; ++ib) k_linv_row<<<ib, 256, 0, s>>>(C.AQ, C.Linv, N, ib);
+    k_transpose<<<dim3(nb, nb), 256, 0, s>>>(C.Linv, C.LinvT, N);
+    int flag = 0;
+    hipMemcpyAsync(&flag, C.flag, sizeof(int), hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    return flag ? GPUTO_ERR_NOT_SPD : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest solve x = L^-T L^-1 b as two row-parallel triangular products
+// (one 64-lane wave per row, fixed reduction order), then x[fixed] = 0
+// (mgcg1.c:431-432).
+// ---------------------------------------------------------------------------
+template <bool UPPER>
+__global__ __launch_bounds__(64) void k_trmv(const double *__restrict__ M, const double *__restrict__ b,
+                                            double *__restrict__ x, int N, int n, const uint8_t *__restrict__ mask,
+                                            const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const int row = blockIdx.x;
+    const int lo = UPPER ? row : 0, hi = UPPER ? n - 1 : (row < n ? row : n - 1);
+    double s = 0.0;
+    for (int c = lo + threadIdx.x; c <= hi; c += 64) s += M[(long)row * N + c] * b[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x == 0 && row < n) {
+        if (UPPER && mask && ((mask[row / 3] >> (row % 3)) & 1u)) s = 0.0;
+        x[row] = s;
+    }
+}
+
+void launch_coarse_solve(hipStream_t s, const Coarse &C, const double *b, double *x, const uint8_t *mask,
+                         const int *ctl)
+{
+    const int N = (C.n + NB - 1) / NB * NB;
+    k_trmv<false><<<C.n, 64, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);
+    k_trmv<true><<<C.n, 64, 0, s>>>(C.LinvT, C.y, x, N, C.n, mask, ctl);
+}
+
+}  // namespace gputo

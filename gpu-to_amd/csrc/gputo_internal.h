@@ -1,0 +1,154 @@
+// Internal declarations of libgputo: grid geometry, tiling of the node-gather
+// kernels, device state of one multigrid level and of a solver handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/gputo.h"
+
+namespace gputo {
+
+// Largest halo / element box of one node-gather tile (LDS arrays are sized
+// from these; the tile chooser keeps every tile inside them).
+constexpr int kMaxHalo = 864;
+constexpr int kMaxElemBox = 640;
+constexpr int kBlock = 256;          // threads per block of the gather kernels
+constexpr int kRedBlocks = 1024;     // blocks of the two-stage reductions
+constexpr int kMaxLevels = 12;
+constexpr int kMaxCoarseDofs = 16384;
+
+// One structured hex grid: elements nx*ny*nz, nodes (nx+1)(ny+1)(nz+1);
+// node id = nny*nnz*i + nny*k + j, element id = ny*nz*i + ny*k + j.
+struct Grid {
+    int nx, ny, nz;
+    int nnx, nny, nnz;
+    long nel, nnod, ndof;
+};
+
+Grid make_grid(int nx, int ny, int nz);
+
+// Node tile of a gather kernel: TJ x TK x TI nodes per block.
+struct Tile {
+    int TJ, TK, TI;
+    int ntj, ntk, nti;
+    long blocks() const { return (long)ntj * ntk * nti; }
+};
+
+Tile choose_tile(const Grid &g);
+
+// Arguments of the node-gather operator kernels.
+struct OpArgs {
+    Grid g;
+    Tile t;
+    const double *Ke;      // 24x24 row-major (level 0)
+    const double *CX;      // element scale (level 0)
+    const double *A;       // packed 300 x nel entry-major element matrices (levels >= 1)
+    const double *U;       // input vector
+    const double *F;       // right-hand side (defect / Jacobi)
+    const double *AD;      // diagonal (Jacobi)
+    const uint8_t *mask;   // per node: bit d set when DOF 3n+d is fixed
+    double *out;           // result vector
+    double *out2;          // second result (prerelax+defect writes U and R)
+    double *partials;      // per-block partial sums (fused dot), may be null
+    const int *ctl;        // ctl[0] != 0: skip (converged)
+    double omega, prerelax;
+};
+
+enum OpMode {
+    OP_MATVEC = 0,      // out = K U, masked; partials = sum U.out
+    OP_DEFECT = 1,      // out = F - K U, masked
+    OP_JACOBI = 2,      // out = (1-w) U + w (r + AD U)/AD with r = F - K U masked; partials = sum F.out
+    OP_PRE_JACOBI = 3,  // as OP_JACOBI with U = prerelax*F/AD formed on the fly
+    OP_PRE_DEFECT = 4   // U = prerelax*F/AD written to out2, out = F - K U masked
+};
+
+// Scalars of one device-resident PCG (all on the device, one struct).
+struct CgScalars {
+    double rho, rho_p, alpha, beta, res0, relres, pq, rr;
+    int done, nit, it, bad;
+};
+
+struct Level {
+    Grid g;
+    Tile t;
+    double *U = nullptr, *F = nullptr, *R = nullptr, *AD = nullptr, *T = nullptr;
+    double *A = nullptr;          // packed element matrices (levels >= 1)
+    uint8_t *mask = nullptr;      // fixed-DOF node mask
+    int *fixed_d = nullptr;       // fixed list (0-based DOF) on the device
+    std::vector<int> fixed;       // fixed list (0-based DOF), reference order
+};
+
+struct Coarse {
+    int n = 0;
+    double *AQ = nullptr;    // dense matrix, then its Cholesky factor (lower)
+    double *Linv = nullptr;  // inverse of the factor (lower, row-major)
+    double *LinvT = nullptr; // its transpose (row-major)
+    double *y = nullptr;     // work vector
+    int *flag = nullptr;     // SPD failure flag
+};
+
+}  // namespace gputo
+
+struct gputo_handle {
+    gputo_params prm{};
+    int nl = 0;
+    gputo::Level L[gputo::kMaxLevels];
+    gputo::Coarse C;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    double *Ke = nullptr;         // fine element matrix (row-major, symmetric)
+    double *Aelm = nullptr;       // level-1 Gauss-point matrices, packed [300][8]
+    double *CX = nullptr;         // element scales of level 0
+    double *Fv = nullptr, *Uv = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr, *Z = nullptr;
+    double *partials = nullptr;   // reduction partials
+    gputo::CgScalars *sc = nullptr;
+    gputo::CgScalars *sc_host = nullptr;  // pinned mirror
+    int *ctl_zero = nullptr;      // an always-zero control word
+    long long bytes = 0;
+    bool hier_ready = false;
+    std::vector<void *> allocs;
+    struct ToState *to = nullptr;
+    struct MixedState *mx = nullptr;
+};
+
+namespace gputo {
+
+// ---- launch helpers (kernels_f64.hip) --------------------------------
+void launch_op0(hipStream_t s, int mode, const OpArgs &a);
+void launch_opA(hipStream_t s, int mode, const OpArgs &a);
+void launch_diag0(hipStream_t s, const Grid &g, const double *Ke, const double *CX, double *AD);
+void launch_diagA(hipStream_t s, const Grid &g, const double *A, double *AD);
+void launch_restrict(hipStream_t s, const Grid &gc, const double *Rf, double *Fc, const uint8_t *maskc, const int *ctl);
+void launch_interp(hipStream_t s, const Grid &gf, const double *Uc, double *Uf, const uint8_t *maskf, int zero_fixed,
+                   const int *ctl);
+void launch_fill(hipStream_t s, double *v, long n, double val);
+void launch_dot_partial(hipStream_t s, const double *a, const double *b, long n, double *partials, const int *ctl);
+void launch_copy(hipStream_t s, const double *src, double *dst, long n, const int *ctl);
+void launch_coarse_l1(hipStream_t s, const Grid &gc, const double *Aelm, const double *CXf, double *A);
+void launch_coarse_galerkin(hipStream_t s, const Grid &gc, const double *Af, const double *Ke, const double *CXf,
+                            double *Ac);
+void launch_corrcsa(hipStream_t s, const Grid &g, const int *fixed_d, int nfd, double *A, int mirrored);
+void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ);
+int coarse_factor(hipStream_t s, Coarse &C);
+void launch_coarse_solve(hipStream_t s, const Coarse &C, const double *b, double *x, const uint8_t *mask,
+                         const int *ctl);
+
+// PCG scalar steps (single-block kernels reading the reduction partials).
+void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
+void launch_cg_rho0(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
+void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
+void launch_cg_update_ur(hipStream_t s, double *U, double *R, const double *P, const double *Q, long n, CgScalars *sc,
+                         double *partials);
+void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit);
+void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit);
+void launch_cg_update_p(hipStream_t s, double *P, const double *Z, long n, CgScalars *sc);
+
+void check_hip(hipError_t e, const char *what);
+void to_release(gputo_handle *h);  // frees the design-loop state (driver.hip)
+
+}  // namespace gputo

@@ -1,0 +1,572 @@
+// FP64 kernels of the MGCG hierarchy: node-gather operators with LDS-staged
+// nodal bricks, operator diagonals, grid transfers, coarse operators,
+// dense coarsest factorisation and the device-side PCG scalar steps.
+//
+// Layout: DOF vector [3*nnod], node id = nny*nnz*i + nny*k + j (j fastest);
+// element scales [nel], element id = ny*nz*i + ny*k + j; coarse element
+// matrices packed lower-triangular and entry-major: A[p*nel + e] with
+// p = m(m+1)/2 + n for n <= m (the packing of Cantilever/TOP3D125MGCGCUDA/
+// mgcg1.cu:1313-1316, stored entry-major so that neighbouring elements are
+// neighbouring in memory).
+#include "gputo_internal.h"
+
+namespace gputo {
+
+__host__ __device__ constexpr int lidx(int x, int y, int z)
+{
+    // local node number of offset (x,y,z) in {0,1}^3 in the element order
+    // ie={0,1,1,0,0,1,1,0}, je={0,0,1,1,0,0,1,1}, ke={0,0,0,0,1,1,1,1}
+    // (Cantilever/omp/mgcg1.c:621-624)
+    return 4 * z + (y ? (x ? 2 : 3) : (x ? 1 : 0));
+}
+
+__host__ __device__ constexpr int pidx(int m, int n) { return n <= m ? m * (m + 1) / 2 + n : n * (n + 1) / 2 + m; }
+
+__device__ __forceinline__ double block_sum(double v)
+{
+    // deterministic sum over a 256-thread block: fixed butterfly per wave,
+    // then the 4 wave sums in order
+    __shared__ double ws[kBlock / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) ws[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) s += ws[i];
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Node-gather operator.  One block owns a TJ x TK x TI brick of nodes; it
+// stages the nodal values of the brick plus a one-node halo in LDS
+// (structure of arrays, one plane per component) and, for level 0, the
+// element scales of the (TJ+1)(TK+1)(TI+1) element box.  Each thread then
+// forms the 3 rows of K U at its node from the 27 neighbouring nodes:
+//   y_c = sum_e E_e sum_b Ke[3a_e + c][3b + d] u_{b,d}
+// with the element products accumulated per incident element (z_e) so that
+// every neighbour value is read from LDS once and scaled once per element.
+// Ke indices are compile-time constants, so Ke is read through the scalar
+// cache.  No atomics: each output DOF is written by exactly one thread.
+// ---------------------------------------------------------------------------
+template <int MODE, bool STORED>
+__global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
+{
+    if (a.ctl && a.ctl[0]) return;
+    __shared__ double su[3][kMaxHalo];
+    __shared__ double sE[STORED ? 1 : kMaxElemBox];
+    const Grid g = a.g;
+    const Tile t = a.t;
+    int b = blockIdx.x;
+    const int bj = b % t.ntj;
+    b /= t.ntj;
+    const int bk = b % t.ntk;
+    const int bi = b / t.ntk;
+    const int j0 = bj * t.TJ, k0 = bk * t.TK, i0 = bi * t.TI;
+    const int HJ = t.TJ + 2, HK = t.TK + 2, HI = t.TI + 2;
+
+    const int rowlen = 3 * HJ;
+    for (int q = threadIdx.x; q < rowlen * HK * HI; q += kBlock) {
+        const int row = q / rowlen;
+        const int col = q - row * rowlen;
+        const int hk = row % HK, hi = row / HK;
+        const int hj = col / 3, c = col - 3 * hj;
+        const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
+        double v = 0.0;
+        if (gi >= 0 && gi < g.nnx && gj >= 0 && gj < g.nny && gk >= 0 && gk < g.nnz) {
+            const long dof = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + gj) + c;
+            if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
+                v = a.prerelax * a.F[dof] / a.AD[dof];
+            else
+                v = a.U[dof];
+        }
+        su[c][(hi * HK + hk) * HJ + hj] = v;
+    }
+    const int EJ = t.TJ + 1, EK = t.TK + 1;
+    if (!STORED) {
+        const int EI = t.TI + 1;
+        for (int q = threadIdx.x; q < EJ * EK * EI; q += kBlock) {
+            const int ej = q % EJ;
+            const int r = q / EJ;
+            const int ek = r % EK, ei = r / EK;
+            const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = k0 - 1 + ek;
+            double v = 0.0;
+            if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz)
+                v = a.CX[(long)g.ny * g.nz * gi + (long)g.ny * gk + gj];
+            sE[q] = v;
+        }
+    }
+    __syncthreads();
+
+    double part = 0.0;
+    const int tid = threadIdx.x;
+    if (tid < t.TJ * t.TK * t.TI) {
+        const int tj = tid % t.TJ;
+        const int r = tid / t.TJ;
+        const int tk = r % t.TK, ti = r / t.TK;
+        const int gi = i0 + ti, gj = j0 + tj, gk = k0 + tk;
+        if (gi < g.nnx && gj < g.nny && gk < g.nnz) {
+            const int hb = (ti * HK + tk) * HJ + tj;  // halo index of neighbour offset (0,0,0)
+            double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+            if (!STORED) {
+                const double *__restrict__ Ke = a.Ke;
+                double z[8][3];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) z[e][0] = z[e][1] = z[e][2] = 0.0;
+#pragma unroll
+                for (int oi = 0; oi < 3; ++oi)
+#pragma unroll
+                    for (int ok = 0; ok < 3; ++ok)
+#pragma unroll
+                        for (int oj = 0; oj < 3; ++oj) {
+                            const int h = hb + (oi * HK + ok) * HJ + oj;
+                            const double u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
+#pragma unroll
+                            for (int ei = 0; ei < 2; ++ei)
+#pragma unroll
+                                for (int ek = 0; ek < 2; ++ek)
+#pragma unroll
+                                    for (int ej = 0; ej < 2; ++ej) {
+                                        const int bx = oi - ei, by = oj - ej, bz = ok - ek;
+                                        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+                                        const int la = lidx(1 - ei, 1 - ej, 1 - ek);
+                                        const int lb = lidx(bx, by, bz);
+                                        const int e = 4 * ei + 2 * ek + ej;
+#pragma unroll
+                                        for (int c = 0; c < 3; ++c) {
+                                            const double *kr = Ke + 24 * (3 * la + c) + 3 * lb;
+                                            z[e][c] += kr[0] * u0 + kr[1] * u1 + kr[2] * u2;
+                                        }
+                                    }
+                        }
+                const int eb = (ti * EK + tk) * EJ + tj;
+#pragma unroll
+                for (int ei = 0; ei < 2; ++ei)
+#pragma unroll
+                    for (int ek = 0; ek < 2; ++ek)
+#pragma unroll
+                        for (int ej = 0; ej < 2; ++ej) {
+                            const int e = 4 * ei + 2 * ek + ej;
+                            const double E = sE[eb + (ei * EK + ek) * EJ + ej];
+                            y0 += E * z[e][0];
+                            y1 += E * z[e][1];
+                            y2 += E * z[e][2];
+                        }
+            } else {
+                const long nel = g.nel;
+                const double *__restrict__ A = a.A;
+#pragma unroll
+                for (int ei = 0; ei < 2; ++ei)
+#pragma unroll
+                    for (int ek = 0; ek < 2; ++ek)
+#pragma unroll
+                        for (int ej = 0; ej < 2; ++ej) {
+                            const int ex = gi - 1 + ei, ey = gj - 1 + ej, ez = gk - 1 + ek;
+                            if (ex < 0 || ex >= g.nx || ey < 0 || ey >= g.ny || ez < 0 || ez >= g.nz) continue;
+                            const double *__restrict__ Ae = A + ((long)g.ny * g.nz * ex + (long)g.ny * ez + ey);
+                            const int la = lidx(1 - ei, 1 - ej, 1 - ek);
+#pragma unroll
+                            for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+                                for (int bz = 0; bz < 2; ++bz)
+#pragma unroll
+                                    for (int by = 0; by < 2; ++by) {
+                                        const int h = hb + ((ei + bx) * HK + (ek + bz)) * HJ + (ej + by);
+                                        const double u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
+                                        const int lb = lidx(bx, by, bz);
+                                        y0 += Ae[pidx(3 * la + 0, 3 * lb + 0) * nel] * u0 +
+                                              Ae[pidx(3 * la + 0, 3 * lb + 1) * nel] * u1 +
+                                              Ae[pidx(3 * la + 0, 3 * lb + 2) * nel] * u2;
+                                        y1 += Ae[pidx(3 * la + 1, 3 * lb + 0) * nel] * u0 +
+                                              Ae[pidx(3 * la + 1, 3 * lb + 1) * nel] * u1 +
+                                              Ae[pidx(3 * la + 1, 3 * lb + 2) * nel] * u2;
+                                        y2 += Ae[pidx(3 * la + 2, 3 * lb + 0) * nel] * u0 +
+                                              Ae[pidx(3 * la + 2, 3 * lb + 1) * nel] * u1 +
+                                              Ae[pidx(3 * la + 2, 3 * lb + 2) * nel] * u2;
+                                    }
+                        }
+            }
+            const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
+            const long dof = 3 * nod;
+            const unsigned m = a.mask[nod];
+            const double y[3] = {y0, y1, y2};
+            const int hc = hb + (HK + 1) * HJ + 1;  // the node itself
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const bool fixed = (m >> c) & 1u;
+                if (MODE == OP_MATVEC) {
+                    const double v = fixed ? 0.0 : y[c];
+                    a.out[dof + c] = v;
+                    part += su[c][hc] * v;
+                } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
+                    const double rv = fixed ? 0.0 : a.F[dof + c] - y[c];
+                    a.out[dof + c] = rv;
+                    if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = su[c][hc];
+                } else {
+                    const double f = a.F[dof + c];
+                    const double rv = fixed ? 0.0 : f - y[c];
+                    const double u = su[c][hc];
+                    const double ad = a.AD[dof + c];
+                    // U = (1-w) U + w (R + AD U)/AD  (mgcg1.cu:1199-1206)
+                    const double un = (1.0 - a.omega) * u + a.omega * (rv + ad * u) / ad;
+                    a.out[dof + c] = un;
+                    part += f * un;
+                }
+            }
+        }
+    }
+    if (a.partials) {
+        const double s = block_sum(part);
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+    }
+}
+
+template <bool STORED>
+static void launch_gather(hipStream_t s, int mode, const OpArgs &a)
+{
+    const dim3 grid((unsigned)a.t.blocks()), block(kBlock);
+    switch (mode) {
+    case OP_MATVEC: k_gather<OP_MATVEC, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_DEFECT: k_gather<OP_DEFECT, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_JACOBI: k_gather<OP_JACOBI, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_JACOBI: k_gather<OP_PRE_JACOBI, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_DEFECT: k_gather<OP_PRE_DEFECT, STORED><<<grid, block, 0, s>>>(a); break;
+    default: break;
+    }
+}
+
+void launch_op0(hipStream_t s, int mode, const OpArgs &a) { launch_gather<false>(s, mode, a); }
+void launch_opA(hipStream_t s, int mode, const OpArgs &a) { launch_gather<true>(s, mode, a); }
+
+// ---------------------------------------------------------------------------
+// Operator diagonals (prepAD0c / prepADc, mgcg1.cu:1404-1487): one thread
+// per node, sum over its incident elements.
+// ---------------------------------------------------------------------------
+template <bool STORED>
+__global__ __launch_bounds__(kBlock) void k_diag(Grid g, const double *__restrict__ Ke, const double *__restrict__ CX,
+                                                const double *__restrict__ A, double *__restrict__ AD)
+{
+    const long nod = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (nod >= g.nnod) return;
+    const int j = (int)(nod % g.nny);
+    const int k = (int)((nod / g.nny) % g.nnz);
+    const int i = (int)(nod / ((long)g.nny * g.nnz));
+    double d[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ei = 0; ei < 2; ++ei)
+#pragma unroll
+        for (int ek = 0; ek < 2; ++ek)
+#pragma unroll
+            for (int ej = 0; ej < 2; ++ej) {
+                const int ex = i - 1 + ei, ey = j - 1 + ej, ez = k - 1 + ek;
+                if (ex < 0 || ex >= g.nx || ey < 0 || ey >= g.ny || ez < 0 || ez >= g.nz) continue;
+                const long e = (long)g.ny * g.nz * ex + (long)g.ny * ez + ey;
+                const int la = lidx(1 - ei, 1 - ej, 1 - ek);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int m = 3 * la + c;
+                    d[c] += STORED ? A[pidx(m, m) * g.nel + e] : CX[e] * Ke[25 * m];
+                }
+            }
+    AD[3 * nod + 0] = d[0];
+    AD[3 * nod + 1] = d[1];
+    AD[3 * nod + 2] = d[2];
+}
+
+void launch_diag0(hipStream_t s, const Grid &g, const double *Ke, const double *CX, double *AD)
+{
+    k_diag<false><<<(unsigned)((g.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(g, Ke, CX, nullptr, AD);
+}
+
+void launch_diagA(hipStream_t s, const Grid &g, const double *A, double *AD)
+{
+    k_diag<true><<<(unsigned)((g.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(g, nullptr, nullptr, A, AD);
+}
+
+// ---------------------------------------------------------------------------
+// Restriction F_c = P^T R_f, F_c[fixed] = 0 (rstrFc + ZeroBCc,
+// mgcg1.cu:792-831): one thread per coarse node gathers the 27 fine nodes
+// around (2I,2J,2K) with weight 2^-(|di|+|dj|+|dk|).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const double *__restrict__ Rf, double *__restrict__ Fc,
+                                                    const uint8_t *__restrict__ maskc, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long nod = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (nod >= gc.nnod) return;
+    const int J = (int)(nod % gc.nny);
+    const int K = (int)((nod / gc.nny) % gc.nnz);
+    const int I = (int)(nod / ((long)gc.nny * gc.nnz));
+    const int fny = 2 * gc.ny + 1, fnz = 2 * gc.nz + 1, fnx = 2 * gc.nx + 1;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int di = -1; di <= 1; ++di) {
+        const int iP = 2 * I + di;
+        if (iP < 0 || iP >= fnx) continue;
+        for (int dk = -1; dk <= 1; ++dk) {
+            const int kP = 2 * K + dk;
+            if (kP < 0 || kP >= fnz) continue;
+            for (int dj = -1; dj <= 1; ++dj) {
+                const int jP = 2 * J + dj;
+                if (jP < 0 || jP >= fny) continue;
+                const double w = 1.0 / (double)(1 << (abs(di) + abs(dj) + abs(dk)));
+                const long f = 3 * ((long)fny * fnz * iP + (long)fny * kP + jP);
+                s0 += w * Rf[f];
+                s1 += w * Rf[f + 1];
+                s2 += w * Rf[f + 2];
+            }
+        }
+    }
+    const unsigned m = maskc[nod];
+    Fc[3 * nod + 0] = (m & 1u) ? 0.0 : s0;
+    Fc[3 * nod + 1] = (m & 2u) ? 0.0 : s1;
+    Fc[3 * nod + 2] = (m & 4u) ? 0.0 : s2;
+}
+
+void launch_restrict(hipStream_t s, const Grid &gc, const double *Rf, double *Fc, const uint8_t *maskc, const int *ctl)
+{
+    k_restrict<<<(unsigned)((gc.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gc, Rf, Fc, maskc, ctl);
+}
+
+// ---------------------------------------------------------------------------
+// Prolongation U_f += P U_c (interpc, mgcg1.cu:1587-1641) followed by
+// U_f[fixed] = 0 (mgcg1.cu:631): one thread per fine node averages the 1,
+// 2, 4 or 8 coarse nodes around it according to the parity of (i,j,k).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const double *__restrict__ Uc, double *__restrict__ Uf,
+                                                  const uint8_t *__restrict__ maskf, int zero_fixed, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long nod = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (nod >= gf.nnod) return;
+    const int j = (int)(nod % gf.nny);
+    const int k = (int)((nod / gf.nny) % gf.nnz);
+    const int i = (int)(nod / ((long)gf.nny * gf.nnz));
+    const int cny = gf.ny / 2 + 1, cnz = gf.nz / 2 + 1;
+    const int ia = i >> 1, ib = (i + 1) >> 1, ja = j >> 1, jb = (j + 1) >> 1, ka = k >> 1, kb = (k + 1) >> 1;
+    const double w = 1.0 / (double)(1 << ((i & 1) + (j & 1) + (k & 1)));
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    // corners (ia|ib, ja|jb, ka|kb); for even indices a == b and the
+    // duplicate corners are skipped so each coarse node counts once
+    for (int ci = 0; ci < 2; ++ci) {
+        const int I = ci ? ib : ia;
+        if (ci && ib == ia) continue;
+        for (int ck = 0; ck < 2; ++ck) {
+            const int K = ck ? kb : ka;
+            if (ck && kb == ka) continue;
+            for (int cj = 0; cj < 2; ++cj) {
+                const int J = cj ? jb : ja;
+                if (cj && jb == ja) continue;
+                const long c = 3 * ((long)cny * cnz * I + (long)cny * K + J);
+                s0 += Uc[c];
+                s1 += Uc[c + 1];
+                s2 += Uc[c + 2];
+            }
+        }
+    }
+    const unsigned m = zero_fixed ? maskf[nod] : 0u;
+    const long d = 3 * nod;
+    Uf[d + 0] = (m & 1u) ? 0.0 : Uf[d + 0] + w * s0;
+    Uf[d + 1] = (m & 2u) ? 0.0 : Uf[d + 1] + w * s1;
+    Uf[d + 2] = (m & 4u) ? 0.0 : Uf[d + 2] + w * s2;
+}
+
+void launch_interp(hipStream_t s, const Grid &gf, const double *Uc, double *Uf, const uint8_t *maskf, int zero_fixed,
+                   const int *ctl)
+{
+    k_interp<<<(unsigned)((gf.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gf, Uc, Uf, maskf, zero_fixed, ctl);
+}
+
+// ---------------------------------------------------------------------------
+// BLAS-1 and reductions.  Dot products are two-stage with a fixed grid of
+// kRedBlocks blocks and a fixed per-thread stride, so the summation order
+// (and the result) does not depend on timing.
+// ---------------------------------------------------------------------------
+__global__ void k_fill(double *v, long n, double val)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[i] = val;
+}
+
+void launch_fill(hipStream_t s, double *v, long n, double val)
+{
+    long nb = (n + kBlock - 1) / kBlock;
+    if (nb > 4096) nb = 4096;
+    if (nb < 1) nb = 1;
+    k_fill<<<(unsigned)nb, kBlock, 0, s>>>(v, n, val);
+}
+
+__global__ void k_copy(const double *__restrict__ src, double *__restrict__ dst, long n, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+void launch_copy(hipStream_t s, const double *src, double *dst, long n, const int *ctl)
+{
+    long nb = (n + kBlock - 1) / kBlock;
+    if (nb > 4096) nb = 4096;
+    if (nb < 1) nb = 1;
+    k_copy<<<(unsigned)nb, kBlock, 0, s>>>(src, dst, n, ctl);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_partial(const double *__restrict__ a, const double *__restrict__ b,
+                                                       long n, double *partials, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) s += a[i] * b[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+void launch_dot_partial(hipStream_t s, const double *a, const double *b, long n, double *partials, const int *ctl)
+{
+    k_dot_partial<<<kRedBlocks, kBlock, 0, s>>>(a, b, n, partials, ctl);
+}
+
+__device__ double sum_partials(const double *p, int n)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) s += p[i];
+    return block_sum(s);
+}
+
+// PCG scalar steps (the scalar arithmetic of mgcg1.cu:235-275, kept on the
+// device so the loop never waits on the host).
+__global__ __launch_bounds__(kBlock) void k_cg_init(CgScalars *sc, const double *p, int n)
+{
+    const double ff = sum_partials(p, n);
+    if (threadIdx.x == 0) {
+        sc->res0 = sqrt(ff);
+        sc->done = 0;
+        sc->nit = 0;
+        sc->it = 0;
+        sc->bad = 0;
+        sc->relres = 0.0;
+        if (!(sc->res0 < INFINITY)) {  // NaN or Inf in F
+            sc->bad = 1;
+            sc->done = 1;
+        } else if (sc->res0 == 0.0) {  // F = 0: U = 0 is exact
+            sc->done = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_rho0(CgScalars *sc, const double *p, int n)
+{
+    if (sc->done) return;
+    const double rz = sum_partials(p, n);
+    if (threadIdx.x == 0) {
+        sc->rho = rz;
+        sc->rho_p = rz;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars *sc, const double *p, int n)
+{
+    if (sc->done) return;
+    const double pq = sum_partials(p, n);
+    if (threadIdx.x == 0) {
+        sc->pq = pq;
+        sc->alpha = sc->rho_p / pq;
+        if (!(fabs(sc->alpha) < INFINITY)) {
+            sc->bad = 1;
+            sc->done = 1;
+            sc->nit = sc->it;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_update_ur(double *__restrict__ U, double *__restrict__ R,
+                                                        const double *__restrict__ P, const double *__restrict__ Q,
+                                                        long n, const CgScalars *sc, double *partials)
+{
+    if (sc->done) return;
+    const double al = sc->alpha;
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) {
+        U[i] += al * P[i];
+        const double r = R[i] - al * Q[i];
+        R[i] = r;
+        s += r * r;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_check(CgScalars *sc, const double *p, int n, double tol, int maxit)
+{
+    if (sc->done) return;
+    const double rr = sum_partials(p, n);
+    if (threadIdx.x == 0) {
+        sc->rr = rr;
+        sc->relres = sqrt(rr) / sc->res0;
+        if (!(sc->relres < INFINITY)) {
+            sc->bad = 1;
+            sc->done = 1;
+            sc->nit = sc->it;
+        } else if (sc->relres < tol) {
+            sc->done = 1;
+            sc->nit = sc->it;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_beta(CgScalars *sc, const double *p, int n, int maxit)
+{
+    if (sc->done) return;
+    const double rz = sum_partials(p, n);
+    if (threadIdx.x == 0) {
+        sc->rho = rz;
+        sc->beta = rz / sc->rho_p;
+        sc->rho_p = rz;
+        sc->it += 1;
+        if (sc->it >= maxit) {
+            sc->done = 1;
+            sc->nit = maxit;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(double *__restrict__ P, const double *__restrict__ Z, long n,
+                                                       const CgScalars *sc)
+{
+    if (sc->done) return;
+    const double be = sc->beta;
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) P[i] = Z[i] + be * P[i];
+}
+
+void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts)
+{
+    k_cg_init<<<1, kBlock, 0, s>>>(sc, partials, nparts);
+}
+void launch_cg_rho0(hipStream_t s, CgScalars *sc, const double *partials, int nparts)
+{
+    k_cg_rho0<<<1, kBlock, 0, s>>>(sc, partials, nparts);
+}
+void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int nparts)
+{
+    k_cg_alpha<<<1, kBlock, 0, s>>>(sc, partials, nparts);
+}
+void launch_cg_update_ur(hipStream_t s, double *U, double *R, const double *P, const double *Q, long n, CgScalars *sc,
+                         double *partials)
+{
+    k_cg_update_ur<<<kRedBlocks, kBlock, 0, s>>>(U, R, P, Q, n, sc, partials);
+}
+void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit)
+{
+    k_cg_check<<<1, kBlock, 0, s>>>(sc, partials, nparts, tol, maxit);
+}
+void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit)
+{
+    k_cg_beta<<<1, kBlock, 0, s>>>(sc, partials, nparts, maxit);
+}
+void launch_cg_update_p(hipStream_t s, double *P, const double *Z, long n, CgScalars *sc)
+{
+    long nb = (n + kBlock - 1) / kBlock;
+    if (nb > 4096) nb = 4096;
+    k_cg_update_p<<<(unsigned)nb, kBlock, 0, s>>>(P, Z, n, sc);
+}
+
+}  // namespace gputo

@@ -1,0 +1,817 @@
+// Host side of libgputo: handle and workspace, multigrid hierarchy set-up,
+// the V-cycle and the device-resident FP64 PCG, and the C-ABI entry points
+// that replace the mgcg1 MEX solve (Cantilever/TOP3D125MGCGCUDA/mgcg1.cu).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "gputo_internal.h"
+#include "mixed.h"
+
+namespace gputo {
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string &s) : std::runtime_error(s) {}
+};
+
+void check_hip(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(x) check_hip((x), #x)
+
+Grid make_grid(int nx, int ny, int nz)
+{
+    Grid g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.nnx = nx + 1;
+    g.nny = ny + 1;
+    g.nnz = nz + 1;
+    g.nel = (long)nx * ny * nz;
+    g.nnod = (long)g.nnx * g.nny * g.nnz;
+    g.ndof = 3 * g.nnod;
+    return g;
+}
+
+static int balance(int n, int tmax)
+{
+    // smallest tile >= n/ceil(n/tmax) so that ceil(n/T) tiles cover n evenly
+    if (tmax < 1) tmax = 1;
+    if (tmax >= n) return n;
+    const int nt = (n + tmax - 1) / tmax;
+    return (n + nt - 1) / nt;
+}
+
+Tile choose_tile(const Grid &g)
+{
+    Tile t;
+    t.TJ = balance(g.nny, 32);
+    t.TK = balance(g.nnz, std::max(1, kBlock / (t.TJ * 2)));
+    t.TI = balance(g.nnx, std::max(1, kBlock / (t.TJ * t.TK)));
+    while ((t.TJ + 2) * (t.TK + 2) * (t.TI + 2) > kMaxHalo || (t.TJ + 1) * (t.TK + 1) * (t.TI + 1) > kMaxElemBox) {
+        if (t.TI > 1)
+            t.TI--;
+        else if (t.TK > 1)
+            t.TK--;
+        else
+            t.TJ--;
+    }
+    t.ntj = (g.nny + t.TJ - 1) / t.TJ;
+    t.ntk = (g.nnz + t.TK - 1) / t.TK;
+    t.nti = (g.nnx + t.TI - 1) / t.TI;
+    return t;
+}
+
+// ---- Gauss-point element stiffness (h = 2, level-1 rediscretisation) ----
+// B^T D B |J| at one 2x2x2 Gauss point of a cube of side h in the element
+// order ie/je/ke; rows of B: exx, eyy, ezz, gxy, gyz, gzx (the formulation
+// of Kem3/defD/Bem3, Cantilever/omp/mgcg1.c:1356-1537).
+static void gauss_point_matrix(double nu, double h, double s, double e, double t, double *K)
+{
+    static const int ie[8] = {0, 1, 1, 0, 0, 1, 1, 0}, je[8] = {0, 0, 1, 1, 0, 0, 1, 1}, ke[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    double B[6][24] = {};
+    for (int a = 0; a < 8; ++a) {
+        const double sx = ie[a] ? 1 : -1, sy = je[a] ? 1 : -1, sz = ke[a] ? 1 : -1;
+        const double dx = 0.25 * sx * (1 + sy * e) * (1 + sz * t) / h;
+        const double dy = 0.25 * sy * (1 + sx * s) * (1 + sz * t) / h;
+        const double dz = 0.25 * sz * (1 + sx * s) * (1 + sy * e) / h;
+        B[0][3 * a] = dx;
+        B[1][3 * a + 1] = dy;
+        B[2][3 * a + 2] = dz;
+        B[3][3 * a] = dy;
+        B[3][3 * a + 1] = dx;
+        B[4][3 * a + 1] = dz;
+        B[4][3 * a + 2] = dy;
+        B[5][3 * a] = dz;
+        B[5][3 * a + 2] = dx;
+    }
+    const double c1 = 1.0 / ((1 + nu) * (1 - 2 * nu));
+    double D[6][6] = {};
+    for (int a = 0; a < 3; ++a) {
+        for (int b = 0; b < 3; ++b) D[a][b] = c1 * (a == b ? 1 - nu : nu);
+        D[3 + a][3 + a] = c1 * (1 - 2 * nu) / 2;
+    }
+    const double w = h * h * h / 8.0;
+    for (int m = 0; m < 24; ++m)
+        for (int n = 0; n < 24; ++n) {
+            double acc = 0.0;
+            for (int p = 0; p < 6; ++p)
+                for (int q = 0; q < 6; ++q) acc += B[p][m] * D[p][q] * B[q][n];
+            K[24 * m + n] = w * acc;
+        }
+}
+
+// Packed [300][8] table: Aelm[8 p + na], na = 4 iP + 2 kP + jP with the
+// Gauss point (g[iP], g[jP], g[kP]) (prepAelm, mgcg1.cu:1998-2034).
+static std::vector<double> level1_gauss_table(double nu, double h)
+{
+    const double g[2] = {-1.0 / std::sqrt(3.0), 1.0 / std::sqrt(3.0)};
+    std::vector<double> tab(300 * 8);
+    double K[576];
+    for (int iP = 0; iP < 2; ++iP)
+        for (int kP = 0; kP < 2; ++kP)
+            for (int jP = 0; jP < 2; ++jP) {
+                gauss_point_matrix(nu, h, g[iP], g[jP], g[kP], K);
+                const int na = 4 * iP + 2 * kP + jP;
+                for (int m = 0; m < 24; ++m)
+                    for (int n = 0; n <= m; ++n) tab[8 * (m * (m + 1) / 2 + n) + na] = K[24 * m + n];
+            }
+    return tab;
+}
+
+// rstrFD (Cantilever/omp/mgcg1.c:532-562): a fine fixed DOF survives on the
+// next level iff its node has even (i, j, k); order follows the fine list.
+static std::vector<int> restrict_fixed(const std::vector<int> &fine, const Grid &gc)
+{
+    const int fny = 2 * gc.ny + 1, fnz = 2 * gc.nz + 1;
+    std::vector<int> out;
+    out.reserve(fine.size() / 2 + 1);
+    for (int dof : fine) {
+        const int idof = dof % 3, nodP = dof / 3;
+        const int iP = nodP / (fnz * fny);
+        if (iP % 2) continue;
+        const int kP = (nodP - fnz * fny * iP) / fny;
+        if (kP % 2) continue;
+        const int jP = nodP - fnz * fny * iP - fny * kP;
+        if (jP % 2) continue;
+        const long nod = (long)gc.nny * gc.nnz * (iP / 2) + (long)gc.nny * (kP / 2) + jP / 2;
+        out.push_back((int)(3 * nod + idof));
+    }
+    return out;
+}
+
+static std::vector<uint8_t> node_mask(const std::vector<int> &fixed, const Grid &g)
+{
+    std::vector<uint8_t> m(g.nnod, 0);
+    for (int dof : fixed) m[dof / 3] |= (uint8_t)(1u << (dof % 3));
+    return m;
+}
+
+}  // namespace gputo
+
+using namespace gputo;
+
+namespace {
+
+template <class T>
+T *dalloc(gputo_handle *h, size_t n)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    check_hip(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    h->allocs.push_back(p);
+    h->bytes += (long long)(n * sizeof(T));
+    return static_cast<T *>(p);
+}
+
+struct Preset {
+    int npre, npost;
+    double omega, prerelax;
+    int l1_galerkin, zero_after_interp, corr_mirrored;
+};
+
+Preset preset_of(int p)
+{
+    if (p == GPUTO_PRESET_MATLAB_CFG0) return {0, 1, 0.6, 0.6, 1, 1, 0};
+    return {3, 3, 0.48, 0.7, 0, 1, 1};
+}
+
+int map_exception()
+{
+    try {
+        throw;
+    } catch (const HipError &e) {
+        std::fprintf(stderr, "gputo: %s\n", e.what());
+        return GPUTO_ERR_DEVICE;
+    } catch (const std::bad_alloc &) {
+        return GPUTO_ERR_WORKSPACE;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "gputo: %s\n", e.what());
+        return GPUTO_ERR_DEVICE;
+    }
+}
+
+OpArgs op_args(const gputo_handle *h, int l)
+{
+    OpArgs a{};
+    a.g = h->L[l].g;
+    a.t = h->L[l].t;
+    a.Ke = h->Ke;
+    a.CX = h->CX;
+    a.A = h->L[l].A;
+    a.mask = h->L[l].mask;
+    a.AD = h->L[l].AD;
+    a.ctl = &h->sc->done;
+    const Preset p = preset_of(h->prm.preset);
+    a.omega = p.omega;
+    a.prerelax = p.prerelax;
+    return a;
+}
+
+void launch_level_op(const gputo_handle *h, int l, int mode, const OpArgs &a)
+{
+    if (l == 0)
+        launch_op0(h->stream, mode, a);
+    else
+        launch_opA(h->stream, mode, a);
+}
+
+// n damped-Jacobi sweeps on level l ending in U; `pre` forms the first
+// sweep's input as prerelax*F/AD (preRelc, mgcg1.cu:1190-1197).
+void sweeps(gputo_handle *h, int l, const double *F, double *U, int n, bool pre)
+{
+    Level &v = h->L[l];
+    const double *src = U;
+    for (int s = 0; s < n; ++s) {
+        double *dst = ((n - 1 - s) % 2 == 0) ? U : v.T;
+        if (!(pre && s == 0) && dst == src) {
+            launch_copy(h->stream, U, v.T, v.g.ndof, &h->sc->done);
+            src = v.T;
+        }
+        OpArgs a = op_args(h, l);
+        a.U = src;
+        a.F = F;
+        a.out = dst;
+        launch_level_op(h, l, (pre && s == 0) ? OP_PRE_JACOBI : OP_JACOBI, a);
+        src = dst;
+    }
+}
+
+// vcycle (mgcg1.cu:468-651; mgcg1.c:283-447): U0 = V(F0).
+void vcycle(gputo_handle *h, const double *F0, double *U0)
+{
+    const Preset p = preset_of(h->prm.preset);
+    const int nl = h->nl;
+    const int *ctl = &h->sc->done;
+    for (int l = 0; l < nl; ++l) {
+        Level &v = h->L[l];
+        const double *F = l == 0 ? F0 : v.F;
+        double *U = l == 0 ? U0 : v.U;
+        if (l > 0) launch_restrict(h->stream, v.g, h->L[l - 1].R, v.F, v.mask, ctl);
+        if (l < nl - 1) {
+            if (p.npre == 0) {
+                OpArgs a = op_args(h, l);
+                a.F = F;
+                a.out = v.R;
+                a.out2 = U;
+                launch_level_op(h, l, OP_PRE_DEFECT, a);
+            } else {
+                sweeps(h, l, F, U, p.npre, true);
+                OpArgs a = op_args(h, l);
+                a.U = U;
+                a.F = F;
+                a.out = v.R;
+                launch_level_op(h, l, OP_DEFECT, a);
+            }
+        } else {
+            launch_coarse_solve(h->stream, h->C, F, U, v.mask, ctl);
+        }
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        Level &v = h->L[l];
+        const double *F = l == 0 ? F0 : v.F;
+        double *U = l == 0 ? U0 : v.U;
+        launch_interp(h->stream, v.g, h->L[l + 1].U, U, v.mask, p.zero_after_interp, ctl);
+        sweeps(h, l, F, U, p.npost, false);
+    }
+}
+
+int validate_fixed(const std::vector<int> &f, long ndof)
+{
+    for (int d : f)
+        if (d < 0 || d >= ndof) return GPUTO_ERR_ARGUMENT;
+    return 0;
+}
+
+bool all_finite(const double *v, long n)
+{
+    for (long i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+}  // namespace
+
+namespace gputo {
+
+// Build the hierarchy for the current Ke, CX (device) and fixed list
+// (initVcycle, mgcg1.cu:326-466).
+int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
+{
+    const Preset p = preset_of(h->prm.preset);
+    hipStream_t s = h->stream;
+    Level &L0 = h->L[0];
+    if (int rc = validate_fixed(fixed0, L0.g.ndof)) return rc;
+    L0.fixed = fixed0;
+    for (int l = 1; l < h->nl; ++l) h->L[l].fixed = restrict_fixed(h->L[l - 1].fixed, h->L[l].g);
+    for (int l = 0; l < h->nl; ++l) {
+        Level &v = h->L[l];
+        std::vector<uint8_t> m = node_mask(v.fixed, v.g);
+        HIPCHK(hipMemcpyAsync(v.mask, m.data(), m.size(), hipMemcpyHostToDevice, s));
+        if (!v.fixed.empty())
+            HIPCHK(hipMemcpyAsync(v.fixed_d, v.fixed.data(), v.fixed.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));  // host vectors above are temporaries
+    }
+    launch_diag0(s, L0.g, h->Ke, h->CX, L0.AD);
+    for (int l = 1; l < h->nl; ++l) {
+        Level &v = h->L[l];
+        if (l == 1 && !p.l1_galerkin)
+            launch_coarse_l1(s, v.g, h->Aelm, h->CX, v.A);
+        else
+            launch_coarse_galerkin(s, v.g, l == 1 ? nullptr : h->L[l - 1].A, h->Ke, h->CX, v.A);
+        if (l == h->nl - 1) {
+            launch_corrcsa(s, v.g, v.fixed_d, (int)v.fixed.size(), v.A, p.corr_mirrored);
+            launch_assemble(s, v.g, v.A, h->C.AQ);
+        }
+        launch_diagA(s, v.g, v.A, v.AD);
+    }
+    HIPCHK(hipGetLastError());
+    int rc = coarse_factor(s, h->C);
+    HIPCHK(hipGetLastError());
+    h->hier_ready = (rc == 0);
+    return rc;
+}
+
+// Device-resident PCG (mgcg1.cu:235-280); U, F on the device.
+int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit)
+{
+    hipStream_t s = h->stream;
+    Level &L0 = h->L[0];
+    const long n = L0.g.ndof;
+    const int *ctl = &h->sc->done;
+    const int nmv = (int)L0.t.blocks();
+    launch_fill(s, U, n, 0.0);
+    launch_copy(s, F, h->R, n, nullptr);
+    launch_dot_partial(s, F, F, n, h->partials, nullptr);
+    launch_cg_init(s, h->sc, h->partials, kRedBlocks);
+    vcycle(h, h->R, h->P);
+    launch_copy(s, h->P, h->Z, n, ctl);
+    launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
+    launch_cg_rho0(s, h->sc, h->partials, kRedBlocks);
+    const int chunk = 4;
+    int enq = 0;
+    for (;;) {
+        for (int c = 0; c < chunk && enq < maxit; ++c, ++enq) {
+            OpArgs a = op_args(h, 0);
+            a.U = h->P;
+            a.out = h->Q;
+            a.partials = h->partials;
+            launch_op0(s, OP_MATVEC, a);
+            launch_cg_alpha(s, h->sc, h->partials, nmv);
+            launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials);
+            launch_cg_check(s, h->sc, h->partials, kRedBlocks, tol, maxit);
+            vcycle(h, h->R, h->Z);
+            launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
+            launch_cg_beta(s, h->sc, h->partials, kRedBlocks, maxit);
+            launch_cg_update_p(s, h->P, h->Z, n, h->sc);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h->sc_host, h->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (h->sc_host->done || enq >= maxit) break;
+    }
+    const CgScalars &r = *h->sc_host;
+    *relres = r.relres;
+    *nit = r.done ? r.nit : maxit;
+    if (maxit <= 0) *nit = 0;
+    if (h->prm.print_level >= 1)
+        std::fprintf(stderr, "gputo: MGCG nit=%d relres=%.6g res0=%.6g\n", *nit, *relres, r.res0);
+    if (r.bad) {
+        *nit = GPUTO_ERR_NONFINITE;
+        return GPUTO_ERR_NONFINITE;
+    }
+    return 0;
+}
+
+}  // namespace gputo
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+void gputo_default_params(gputo_params *p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->preset = GPUTO_PRESET_REF_C;
+    p->precision = GPUTO_PREC_F64;
+    p->learned = 0;
+    p->inner_tol = 1e-4;
+    p->print_level = 0;
+}
+
+const char *gputo_version(void) { return "gputo-mi355x 0.1 (gfx950)"; }
+
+long long gputo_est_workspace(int nfd, int nl, int nelx, int nely, int nelz)
+{
+    // estWorkSpace (mgcg1.cu:293-322), in doubles
+    long long nnel0 = (long long)nelx * nely * nelz;
+    long long ndof0 = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1);
+    long long lW = ndof0 * 6 + nnel0 + nfd;
+    for (int lev = 0; lev < nl; ++lev) {
+        long long ex = nelx >> lev, ey = nely >> lev, ez = nelz >> lev;
+        long long ndof = 3 * (ex + 1) * (ey + 1) * (ez + 1);
+        lW += lev == 0 ? 2 * ndof : 4 * ndof + 300 * ex * ey * ez + nfd;
+        if (lev == nl - 1) lW += 3 * ndof + ndof * ndof;
+    }
+    return lW;
+}
+
+long long gputo_device_bytes(const gputo_handle *h) { return h ? h->bytes : 0; }
+
+int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *params, gputo_handle **out)
+{
+    if (!out) return GPUTO_ERR_ARGUMENT;
+    *out = nullptr;
+    if (nelx < 1 || nely < 1 || nelz < 1 || nl < 2 || nl > kMaxLevels) return GPUTO_ERR_ARGUMENT;
+    const int f = 1 << (nl - 1);
+    if (nelx % f || nely % f || nelz % f) return GPUTO_ERR_ARGUMENT;
+    const Grid gc = make_grid(nelx / f, nely / f, nelz / f);
+    if (gc.ndof > kMaxCoarseDofs) return GPUTO_ERR_WORKSPACE;
+    gputo_handle *h = new gputo_handle();
+    if (params)
+        h->prm = *params;
+    else
+        gputo_default_params(&h->prm);
+    h->nl = nl;
+    try {
+        HIPCHK(hipGetDevice(&h->device));
+        HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        int maxblocks = kRedBlocks;
+        for (int l = 0; l < nl; ++l) {
+            Level &v = h->L[l];
+            v.g = make_grid(nelx >> l, nely >> l, nelz >> l);
+            v.t = choose_tile(v.g);
+            maxblocks = std::max<long>(maxblocks, v.t.blocks());
+            v.R = dalloc<double>(h, v.g.ndof);
+            v.AD = dalloc<double>(h, v.g.ndof);
+            v.T = dalloc<double>(h, v.g.ndof);
+            v.mask = dalloc<uint8_t>(h, v.g.nnod);
+            v.fixed_d = dalloc<int>(h, v.g.ndof);
+            if (l > 0) {
+                v.U = dalloc<double>(h, v.g.ndof);
+                v.F = dalloc<double>(h, v.g.ndof);
+                v.A = dalloc<double>(h, 300 * v.g.nel);
+            }
+        }
+        const Grid &g0 = h->L[0].g;
+        h->Ke = dalloc<double>(h, 576);
+        h->Aelm = dalloc<double>(h, 2400);
+        h->CX = dalloc<double>(h, g0.nel);
+        h->Fv = dalloc<double>(h, g0.ndof);
+        h->Uv = dalloc<double>(h, g0.ndof);
+        h->R = dalloc<double>(h, g0.ndof);
+        h->P = dalloc<double>(h, g0.ndof);
+        h->Q = dalloc<double>(h, g0.ndof);
+        h->Z = dalloc<double>(h, g0.ndof);
+        h->partials = dalloc<double>(h, maxblocks);
+        h->sc = dalloc<CgScalars>(h, 1);
+        HIPCHK(hipHostMalloc((void **)&h->sc_host, sizeof(CgScalars), hipHostMallocDefault));
+        std::memset(h->sc_host, 0, sizeof(CgScalars));
+        HIPCHK(hipMemcpy(h->sc, h->sc_host, sizeof(CgScalars), hipMemcpyHostToDevice));
+        Coarse &C = h->C;
+        C.n = (int)h->L[nl - 1].g.ndof;
+        const long N = (C.n + 31) / 32 * 32;
+        C.AQ = dalloc<double>(h, N * N);
+        C.Linv = dalloc<double>(h, N * N);
+        C.LinvT = dalloc<double>(h, N * N);
+        C.y = dalloc<double>(h, N);
+        C.flag = dalloc<int>(h, 1);
+        std::vector<double> tab = level1_gauss_table(0.3, 2.0);
+        HIPCHK(hipMemcpy(h->Aelm, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+        mixed_create(h);
+    } catch (...) {
+        int rc = map_exception();
+        gputo_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+int gputo_destroy(gputo_handle *h)
+{
+    if (!h) return 0;
+    to_release(h);
+    mixed_destroy(h);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) hipFree(p);
+    if (h->sc_host) hipHostFree(h->sc_host);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+int gputo_set_params(gputo_handle *h, const gputo_params *params)
+{
+    if (!h || !params) return GPUTO_ERR_ARGUMENT;
+    if (params->preset != h->prm.preset) h->hier_ready = false;
+    h->prm = *params;
+    return 0;
+}
+
+static int upload_problem(gputo_handle *h, const double *Ke0, const double *CX, const double *fixed, int nfd,
+                          std::vector<int> &fx)
+{
+    const Grid &g0 = h->L[0].g;
+    if (!Ke0 || !CX || (nfd > 0 && !fixed) || nfd < 0 || nfd > g0.ndof) return GPUTO_ERR_ARGUMENT;
+    if (!all_finite(Ke0, 576) || !all_finite(CX, g0.nel)) return GPUTO_ERR_NONFINITE;
+    fx.resize(nfd);
+    for (int n = 0; n < nfd; ++n) fx[n] = (int)fixed[n] - 1;  // (int)FDofs[n]-1, mgcg1.c:663
+    // Ke is read row-major from the MATLAB column-major array (a transpose;
+    // Ke0 is symmetric), as mgcg1.cu:131-133 does.
+    HIPCHK(hipMemcpyAsync(h->Ke, Ke0, 576 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->CX, CX, g0.nel * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    return 0;
+}
+
+int gputo_setup(gputo_handle *h, const double *Ke0, const double *CX, const double *fixed, int nfd)
+{
+    if (!h) return GPUTO_ERR_ARGUMENT;
+    try {
+        std::vector<int> fx;
+        if (int rc = upload_problem(h, Ke0, CX, fixed, nfd, fx)) return rc;
+        return setup_hierarchy(h, fx);
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int gputo_mgcg1(gputo_handle *h, const double *Ke0, const double *F, double *U, const double *CX, const double *fixed,
+                int nfd, double tol, int maxit, double *relres, int *nit)
+{
+    if (!h || !F || !U || !relres || !nit) return GPUTO_ERR_ARGUMENT;
+    *relres = 0.0;
+    *nit = GPUTO_ERR_ARGUMENT;
+    try {
+        const Grid &g0 = h->L[0].g;
+        std::vector<int> fx;
+        int rc = upload_problem(h, Ke0, CX, fixed, nfd, fx);
+        if (rc == 0 && !all_finite(F, g0.ndof)) rc = GPUTO_ERR_NONFINITE;
+        if (rc) {
+            *nit = rc;
+            return rc;
+        }
+        HIPCHK(hipMemcpyAsync(h->Fv, F, g0.ndof * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        rc = setup_hierarchy(h, fx);
+        if (rc == 0) {
+            if (h->prm.precision == GPUTO_PREC_F64)
+                rc = run_pcg(h, h->Fv, h->Uv, tol, maxit, relres, nit);
+            else
+                rc = mixed_solve(h, h->Fv, h->Uv, tol, maxit, relres, nit);
+        }
+        if (rc) {
+            *nit = rc;
+            return rc;
+        }
+        HIPCHK(hipMemcpyAsync(U, h->Uv, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (long n = 1; n < g0.ndof; n += 3) U[n] = -U[n];  // MATLAB frame (mgcg1.cu:282-283)
+        return 0;
+    } catch (...) {
+        int rc = map_exception();
+        *nit = rc;
+        return rc;
+    }
+}
+
+int gputo_solve_d(gputo_handle *h, const double *Ke0, const double *d_F, double *d_U, const double *d_CX,
+                  const int *d_fixed, int nfd, double tol, int maxit, double *relres, int *nit)
+{
+    if (!h || !Ke0 || !d_F || !d_U || !d_CX || !relres || !nit || nfd < 0) return GPUTO_ERR_ARGUMENT;
+    try {
+        const Grid &g0 = h->L[0].g;
+        std::vector<int> fx(nfd);
+        if (nfd) HIPCHK(hipMemcpyAsync(fx.data(), d_fixed, nfd * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(h->Ke, Ke0, 576 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->CX, d_CX, g0.nel * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        int rc = setup_hierarchy(h, fx);
+        if (rc == 0) {
+            if (h->prm.precision == GPUTO_PREC_F64)
+                rc = run_pcg(h, d_F, d_U, tol, maxit, relres, nit);
+            else
+                rc = mixed_solve(h, d_F, d_U, tol, maxit, relres, nit);
+        }
+        if (rc) *nit = rc;
+        return rc;
+    } catch (...) {
+        int rc = map_exception();
+        *nit = rc;
+        return rc;
+    }
+}
+
+// ---- kernel-level entry points ----------------------------------------------
+
+static int host_op(gputo_handle *h, int mode, const double *Ke0, const double *CX, const double *F, const double *u,
+                   const double *fixed, int nfd, double *y)
+{
+    if (!h || !y || (mode != 2 && !u)) return GPUTO_ERR_ARGUMENT;
+    try {
+        const Grid &g0 = h->L[0].g;
+        std::vector<int> fx;
+        if (int rc = upload_problem(h, Ke0, CX, fixed, nfd, fx)) return rc;
+        if (int rc = validate_fixed(fx, g0.ndof)) return rc;
+        std::vector<uint8_t> m = node_mask(fx, g0);
+        hipStream_t s = h->stream;
+        HIPCHK(hipMemcpyAsync(h->L[0].mask, m.data(), m.size(), hipMemcpyHostToDevice, s));
+        if (mode == 2) {
+            launch_diag0(s, g0, h->Ke, h->CX, h->Q);
+        } else {
+            HIPCHK(hipMemcpyAsync(h->P, u, g0.ndof * sizeof(double), hipMemcpyHostToDevice, s));
+            if (F) HIPCHK(hipMemcpyAsync(h->Fv, F, g0.ndof * sizeof(double), hipMemcpyHostToDevice, s));
+            OpArgs a = op_args(h, 0);
+            a.ctl = nullptr;
+            a.U = h->P;
+            a.F = h->Fv;
+            a.out = h->Q;
+            launch_op0(s, mode == 0 ? OP_MATVEC : OP_DEFECT, a);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(y, h->Q, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        h->hier_ready = false;  // the level-0 mask was replaced
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int gputo_matvec_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *u, const double *fixed, int nfd,
+                     double *y)
+{
+    return host_op(h, 0, Ke0, CX, nullptr, u, fixed, nfd, y);
+}
+
+int gputo_defect_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *F, const double *u,
+                     const double *fixed, int nfd, double *r)
+{
+    if (!F) return GPUTO_ERR_ARGUMENT;
+    return host_op(h, 1, Ke0, CX, F, u, fixed, nfd, r);
+}
+
+int gputo_diag_f64(gputo_handle *h, const double *Ke0, const double *CX, double *ad)
+{
+    return host_op(h, 2, Ke0, CX, nullptr, nullptr, nullptr, 0, ad);
+}
+
+int gputo_vcycle_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *fixed, int nfd, const double *r,
+                     double *z)
+{
+    if (!h || !r || !z) return GPUTO_ERR_ARGUMENT;
+    try {
+        int rc = gputo_setup(h, Ke0, CX, fixed, nfd);
+        if (rc) return rc;
+        const Grid &g0 = h->L[0].g;
+        hipStream_t s = h->stream;
+        std::memset(h->sc_host, 0, sizeof(CgScalars));
+        HIPCHK(hipMemcpyAsync(h->sc, h->sc_host, sizeof(CgScalars), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h->R, r, g0.ndof * sizeof(double), hipMemcpyHostToDevice, s));
+        vcycle(h, h->R, h->Z);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(z, h->Z, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int gputo_level_info(const gputo_handle *h, int level, int *nelx, int *nely, int *nelz, int *nfd)
+{
+    if (!h || level < 0 || level >= h->nl) return GPUTO_ERR_ARGUMENT;
+    const Level &v = h->L[level];
+    if (nelx) *nelx = v.g.nx;
+    if (nely) *nely = v.g.ny;
+    if (nelz) *nelz = v.g.nz;
+    if (nfd) *nfd = (int)v.fixed.size();
+    return 0;
+}
+
+int gputo_level_matrices(gputo_handle *h, int level, double *A, double *ad, double *fixed)
+{
+    if (!h || level < 0 || level >= h->nl || !h->hier_ready) return GPUTO_ERR_ARGUMENT;
+    try {
+        const Level &v = h->L[level];
+        hipStream_t s = h->stream;
+        if (A && level > 0) {
+            std::vector<double> packed(300 * v.g.nel);
+            HIPCHK(hipMemcpyAsync(packed.data(), v.A, packed.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (long e = 0; e < v.g.nel; ++e)
+                for (int m = 0; m < 24; ++m)
+                    for (int n = 0; n < 24; ++n) {
+                        const int p = n <= m ? m * (m + 1) / 2 + n : n * (n + 1) / 2 + m;
+                        A[576 * e + 24 * m + n] = packed[(long)p * v.g.nel + e];
+                    }
+        }
+        if (ad) HIPCHK(hipMemcpyAsync(ad, v.AD, v.g.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (fixed)
+            for (size_t n = 0; n < v.fixed.size(); ++n) fixed[n] = v.fixed[n] + 1.0;
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int gputo_fixed_list(const gputo_handle *h, int level, double *fixed, int *nfd)
+{
+    if (!h || level < 0 || level >= h->nl) return GPUTO_ERR_ARGUMENT;
+    const Level &v = h->L[level];
+    if (nfd) *nfd = (int)v.fixed.size();
+    if (fixed)
+        for (size_t n = 0; n < v.fixed.size(); ++n) fixed[n] = v.fixed[n] + 1.0;
+    return 0;
+}
+
+int gputo_restrict_f64(gputo_handle *h, int level, const double *r_fine, double *f_coarse)
+{
+    if (!h || level < 0 || level + 1 >= h->nl || !h->hier_ready || !r_fine || !f_coarse) return GPUTO_ERR_ARGUMENT;
+    try {
+        Level &vf = h->L[level], &vc = h->L[level + 1];
+        hipStream_t s = h->stream;
+        HIPCHK(hipMemcpyAsync(vf.T, r_fine, vf.g.ndof * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_restrict(s, vc.g, vf.T, vc.T, vc.mask, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(f_coarse, vc.T, vc.g.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int gputo_interp_f64(gputo_handle *h, int level, const double *u_coarse, double *u_fine)
+{
+    if (!h || level < 0 || level + 1 >= h->nl || !h->hier_ready || !u_coarse || !u_fine) return GPUTO_ERR_ARGUMENT;
+    try {
+        Level &vf = h->L[level], &vc = h->L[level + 1];
+        hipStream_t s = h->stream;
+        HIPCHK(hipMemcpyAsync(vc.T, u_coarse, vc.g.ndof * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(vf.T, u_fine, vf.g.ndof * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_interp(s, vf.g, vc.T, vf.T, vf.mask, 0, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(u_fine, vf.T, vf.g.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int gputo_bench_operator(gputo_handle *h, int precision, int reps, double *avg_ms)
+{
+    if (!h || !h->hier_ready || reps < 1 || !avg_ms) return GPUTO_ERR_ARGUMENT;
+    try {
+        hipStream_t s = h->stream;
+        const Grid &g0 = h->L[0].g;
+        launch_fill(s, h->P, g0.ndof, 1.0);
+        if (precision != GPUTO_PREC_F64) return mixed_bench_operator(h, precision, reps, avg_ms);
+        OpArgs a = op_args(h, 0);
+        a.ctl = nullptr;
+        a.U = h->P;
+        a.out = h->Q;
+        launch_op0(s, OP_MATVEC, a);  // warm-up
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) launch_op0(s, OP_MATVEC, a);
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        HIPCHK(hipGetLastError());
+        *avg_ms = ms / reps;
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int gputo_coarse_solve_f64(gputo_handle *h, const double *b, double *x)
+{
+    if (!h || !h->hier_ready || !b || !x) return GPUTO_ERR_ARGUMENT;
+    try {
+        Level &v = h->L[h->nl - 1];
+        hipStream_t s = h->stream;
+        HIPCHK(hipMemcpyAsync(v.F, b, v.g.ndof * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_coarse_solve(s, h->C, v.F, v.T, v.mask, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(x, v.T, v.g.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+}  // extern "C"

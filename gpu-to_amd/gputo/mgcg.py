@@ -1,0 +1,220 @@
+"""MATLAB-facing solve entry points.
+
+`mgcg1` keeps the argument list and semantics of the reference's MEX solve
+so driver code ports line for line:
+
+    [cgres, cgiters] = mgcg1(Ke0, F, U, CX, fixed', nfd, nelx, nely, nelz, nl, cgtol, cgmax[, printLev])
+        (Cantilever/TOP3D125MGCGCUDA/mgcg1.cu:120-182)
+    [cgres, cgiters] = mgcg1(Ke0, F1, U, CX, fixed', WRK, nfd, nelx, nely, nelz, nl, cgtol, cgmax)
+        (Cantilever/omp/mgcg1.c:51-105; WRK is accepted and checked against the
+         reference's workspace estimate, then ignored)
+
+U is overwritten in place with the solution in the MATLAB frame (y negated),
+cgiters is the 0-based exit index, and a failure returns cgiters < 0 instead
+of terminating the process.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, f64, ptr
+
+_HANDLES = {}
+
+
+class Solver:
+    """One libgputo handle for an nelx x nely x nelz mesh with nl levels."""
+
+    def __init__(self, nelx, nely, nelz, nl, preset=_lib.PRESET_REF_C, precision=_lib.PREC_F64, learned=False,
+                 inner_tol=1e-4, print_level=0):
+        self.lib = _lib.load()
+        self.nelx, self.nely, self.nelz, self.nl = int(nelx), int(nely), int(nelz), int(nl)
+        self.nel = self.nelx * self.nely * self.nelz
+        self.ndof = 3 * (self.nelx + 1) * (self.nely + 1) * (self.nelz + 1)
+        self.params = _lib.GputoParams()
+        self.lib.gputo_default_params(ctypes.byref(self.params))
+        self.params.preset = preset
+        self.params.precision = precision
+        self.params.learned = int(bool(learned))
+        self.params.inner_tol = inner_tol
+        self.params.print_level = print_level
+        h = ctypes.c_void_p()
+        check(self.lib.gputo_create(self.nelx, self.nely, self.nelz, self.nl, ctypes.byref(self.params),
+                                    ctypes.byref(h)), "gputo_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gputo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        check(self.lib.gputo_set_params(self.h, ctypes.byref(self.params)), "gputo_set_params")
+
+    # ---- the mgcg1 solve ------------------------------------------------
+    def solve(self, Ke0, F, CX, fixed, tol, maxit, U=None):
+        """Returns (U_matlab_frame, relres, nit, rc)."""
+        Ke0, F, CX = f64(Ke0), f64(F), f64(CX)
+        fixed = f64(fixed).ravel()
+        if U is None:
+            U = np.zeros(self.ndof)
+        relres = ctypes.c_double(0.0)
+        nit = ctypes.c_int(0)
+        rc = self.lib.gputo_mgcg1(self.h, ptr(Ke0), ptr(F), ptr(U), ptr(CX), ptr(fixed), len(fixed), float(tol),
+                                  int(maxit), ctypes.byref(relres), ctypes.byref(nit))
+        return U, relres.value, nit.value, rc
+
+    # ---- kernel-level entry points --------------------------------------
+    def matvec(self, Ke0, CX, u, fixed):
+        y = np.zeros(self.ndof)
+        fixed = f64(fixed).ravel()
+        check(self.lib.gputo_matvec_f64(self.h, ptr(f64(Ke0)), ptr(f64(CX)), ptr(f64(u)), ptr(fixed), len(fixed),
+                                        ptr(y)), "gputo_matvec_f64")
+        return y
+
+    def defect(self, Ke0, CX, F, u, fixed):
+        r = np.zeros(self.ndof)
+        fixed = f64(fixed).ravel()
+        check(self.lib.gputo_defect_f64(self.h, ptr(f64(Ke0)), ptr(f64(CX)), ptr(f64(F)), ptr(f64(u)), ptr(fixed),
+                                        len(fixed), ptr(r)), "gputo_defect_f64")
+        return r
+
+    def diag(self, Ke0, CX):
+        ad = np.zeros(self.ndof)
+        check(self.lib.gputo_diag_f64(self.h, ptr(f64(Ke0)), ptr(f64(CX)), ptr(ad)), "gputo_diag_f64")
+        return ad
+
+    def setup(self, Ke0, CX, fixed):
+        fixed = f64(fixed).ravel()
+        check(self.lib.gputo_setup(self.h, ptr(f64(Ke0)), ptr(f64(CX)), ptr(fixed), len(fixed)), "gputo_setup")
+
+    def vcycle(self, Ke0, CX, fixed, r):
+        z = np.zeros(self.ndof)
+        fixed = f64(fixed).ravel()
+        check(self.lib.gputo_vcycle_f64(self.h, ptr(f64(Ke0)), ptr(f64(CX)), ptr(fixed), len(fixed), ptr(f64(r)),
+                                        ptr(z)), "gputo_vcycle_f64")
+        return z
+
+    def level_info(self, level):
+        a, b, c, d = (ctypes.c_int() for _ in range(4))
+        check(self.lib.gputo_level_info(self.h, level, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                        ctypes.byref(d)), "gputo_level_info")
+        return a.value, b.value, c.value, d.value
+
+    def level_matrices(self, level):
+        nx, ny, nz, nfd = self.level_info(level)
+        nel = nx * ny * nz
+        ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+        A = np.zeros(nel * 576) if level > 0 else None
+        ad = np.zeros(ndof)
+        fd = np.zeros(max(nfd, 1))
+        check(self.lib.gputo_level_matrices(self.h, level, ptr(A), ptr(ad), ptr(fd)), "gputo_level_matrices")
+        return (None if A is None else A.reshape(nel, 24, 24)), ad, fd[:nfd]
+
+    def fixed_list(self, level):
+        n = ctypes.c_int()
+        check(self.lib.gputo_fixed_list(self.h, level, None, ctypes.byref(n)), "gputo_fixed_list")
+        out = np.zeros(max(n.value, 1))
+        check(self.lib.gputo_fixed_list(self.h, level, ptr(out), ctypes.byref(n)), "gputo_fixed_list")
+        return out[:n.value]
+
+    def restrict(self, level, r_fine):
+        nx, ny, nz, _ = self.level_info(level + 1)
+        out = np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1))
+        check(self.lib.gputo_restrict_f64(self.h, level, ptr(f64(r_fine)), ptr(out)), "gputo_restrict_f64")
+        return out
+
+    def interp_add(self, level, u_coarse, u_fine):
+        out = np.array(u_fine, dtype=np.float64, copy=True)
+        check(self.lib.gputo_interp_f64(self.h, level, ptr(f64(u_coarse)), ptr(out)), "gputo_interp_f64")
+        return out
+
+    def coarse_solve(self, b):
+        x = np.zeros_like(f64(b))
+        check(self.lib.gputo_coarse_solve_f64(self.h, ptr(f64(b)), ptr(x)), "gputo_coarse_solve_f64")
+        return x
+
+    def edofmat(self):
+        c = np.zeros((self.nel, 24), dtype=np.int32)
+        check(self.lib.gputo_edofmat(self.h, ptr(c)), "gputo_edofmat")
+        return c
+
+    def filter(self, x, rmin):
+        out = np.zeros(self.nel)
+        check(self.lib.gputo_filter_f64(self.h, ptr(f64(x)), float(rmin), ptr(out)), "gputo_filter_f64")
+        return out
+
+    def sens(self, Ke0, U_cframe):
+        ce = np.zeros(self.nel)
+        check(self.lib.gputo_sens_f64(self.h, ptr(f64(Ke0)), ptr(f64(U_cframe)), ptr(ce)), "gputo_sens_f64")
+        return ce
+
+    def device_bytes(self):
+        return self.lib.gputo_device_bytes(self.h)
+
+
+def _handle(nelx, nely, nelz, nl, preset=_lib.PRESET_REF_C, print_level=0):
+    # one persistent handle per mesh, as a MEX would keep it between calls
+    key = (int(nelx), int(nely), int(nelz), int(nl), preset)
+    s = _HANDLES.get(key)
+    if s is None:
+        s = Solver(nelx, nely, nelz, nl, preset=preset, print_level=print_level)
+        _HANDLES[key] = s
+    return s
+
+
+def release_handles():
+    for s in _HANDLES.values():
+        s.close()
+    _HANDLES.clear()
+
+
+def _scalar(v):
+    return float(np.asarray(v).ravel()[0])
+
+
+def mgcg1(*args):
+    """[cgres, cgiters] = mgcg1(...) with the reference's argument lists.
+
+    12/13 arguments: the CUDA MEX form (optional printLev);
+    13 arguments with a vector 6th argument: the OpenMP MEX form with WRK.
+    U (3rd argument) is overwritten in place.
+    """
+    if len(args) == 13 and np.asarray(args[5]).size > 1:  # omp form with WRK
+        Ke0, F, U, CX, fixed, WRK, nfd, nelx, nely, nelz, nl, tol, maxit = args
+        print_level = 0
+        lenW = _lib.load().gputo_est_workspace(int(_scalar(nfd)), int(_scalar(nl)), int(_scalar(nelx)),
+                                               int(_scalar(nely)), int(_scalar(nelz)))
+        if np.asarray(WRK).size < lenW:  # mgcg1.c:89-97
+            return 0.0, -1.0
+    elif len(args) in (12, 13):
+        Ke0, F, U, CX, fixed, nfd, nelx, nely, nelz, nl, tol, maxit = args[:12]
+        print_level = int(_scalar(args[12])) if len(args) == 13 else 0
+    else:
+        raise TypeError("mgcg1 takes 12 or 13 arguments")
+    if not (isinstance(U, np.ndarray) and U.dtype == np.float64 and U.flags["C_CONTIGUOUS"]):
+        raise TypeError("U must be a contiguous float64 array (it is overwritten in place)")
+    nfd = int(_scalar(nfd))
+    fixed = f64(fixed).ravel()[:nfd]
+    s = _handle(_scalar(nelx), _scalar(nely), _scalar(nelz), _scalar(nl), print_level=print_level)
+    _, relres, nit, rc = s.solve(Ke0, F, CX, fixed, _scalar(tol), int(_scalar(maxit)), U=U.reshape(-1))
+    if rc != 0:
+        return 0.0, float(rc)
+    return relres, float(nit)
+
+
+def mgcg9(Ke0, F, U, CX, fixed, nfd, nelx, nely, nelz, nl, tol, maxit, nswp, printLev, nu, hx, hy, hz):
+    """[cgres, cgiters] = mgcg9(...) with the argument list of
+    Homogenization/QNC/mgcg9.cu:124-182 (nu and hx..hz are parsed and unused
+    there too).  Solved by the mgcg1 engine (REF_C preset); the converged
+    displacements agree to the CG tolerance."""
+    return mgcg1(Ke0, F, U, CX, fixed, nfd, nelx, nely, nelz, nl, tol, maxit, printLev)

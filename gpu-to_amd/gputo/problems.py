@@ -1,0 +1,80 @@
+"""Problem set-up of the Cantilever drivers (Cantilever/omp/top3D125mod.m).
+
+All arrays use the reference's numbering: node n = (nely+1)(nelz+1) i +
+(nely+1) k + j, DOF 3n + d, fixed DOFs as 1-based doubles.
+"""
+import numpy as np
+
+
+def ke0_h8(nu=0.3, h=1.0):
+    """24x24 trilinear hex stiffness (E = 1), 2x2x2 Gauss, in the element
+    order ie={0,1,1,0,0,1,1,0}, je={0,0,1,1,0,0,1,1}, ke={0,0,0,0,1,1,1,1}
+    (Cantilever/omp/mgcg1.c:621-624).  Numerically equal to the Ke0 the
+    MATLAB drivers build from their integer table (top3D125mod.m:50-76)."""
+    ie = np.array([0, 1, 1, 0, 0, 1, 1, 0])
+    je = np.array([0, 0, 1, 1, 0, 0, 1, 1])
+    kk = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+    sx, sy, sz = 2 * ie - 1, 2 * je - 1, 2 * kk - 1
+    c1 = 1.0 / ((1 + nu) * (1 - 2 * nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = c1 * nu
+    np.fill_diagonal(D[:3, :3], c1 * (1 - nu))
+    D[3:, 3:] = np.eye(3) * c1 * (1 - 2 * nu) / 2
+    g = 1.0 / np.sqrt(3.0)
+    K = np.zeros((24, 24))
+    for s in (-g, g):
+        for e in (-g, g):
+            for t in (-g, g):
+                dx = 0.25 * sx * (1 + sy * e) * (1 + sz * t) / h
+                dy = 0.25 * sy * (1 + sx * s) * (1 + sz * t) / h
+                dz = 0.25 * sz * (1 + sx * s) * (1 + sy * e) / h
+                B = np.zeros((6, 24))
+                B[0, 0::3], B[1, 1::3], B[2, 2::3] = dx, dy, dz
+                B[3, 0::3], B[3, 1::3] = dy, dx
+                B[4, 1::3], B[4, 2::3] = dz, dy
+                B[5, 0::3], B[5, 2::3] = dz, dx
+                K += B.T @ D @ B * (h ** 3 / 8.0)
+    return K
+
+
+def node_id(nelx, nely, nelz, i, j, k):
+    return (nely + 1) * (nelz + 1) * i + (nely + 1) * k + j
+
+
+def fixed_x0_face(nelx, nely, nelz):
+    """fixed = 1 : 3*(nely+1)*(nelz+1): every DOF of the x = 0 face
+    (top3D125mod.m:80), 1-based doubles."""
+    return np.arange(1, 3 * (nely + 1) * (nelz + 1) + 1, dtype=np.float64)
+
+
+def uniform_edge_load(nelx, nely, nelz, total=1.0):
+    """Unit total downward load spread over the nodes (i = nelx, k = 0,
+    j = 0..nely) on their z-DOFs."""
+    F = np.zeros(3 * (nelx + 1) * (nely + 1) * (nelz + 1))
+    j = np.arange(nely + 1)
+    F[3 * node_id(nelx, nely, nelz, nelx, j, 0) + 2] = -total / (nely + 1)
+    return F
+
+
+def matlab_edge_load(nelx, nely, nelz):
+    """F(lcDof) = -sin((0:nely)/nely*pi) on the same z-DOFs
+    (top3D125mod.m:78,81)."""
+    F = np.zeros(3 * (nelx + 1) * (nely + 1) * (nelz + 1))
+    j = np.arange(nely + 1)
+    F[3 * node_id(nelx, nely, nelz, nelx, j, 0) + 2] = -np.sin(j / nely * np.pi)
+    return F
+
+
+def cantilever(nelx, nely, nelz, load="uniform"):
+    """(Ke0, F, fixed) of the Cantilever drivers."""
+    F = uniform_edge_load(nelx, nely, nelz) if load == "uniform" else matlab_edge_load(nelx, nely, nelz)
+    return ke0_h8(), F, fixed_x0_face(nelx, nely, nelz)
+
+
+def perturbed_density(nelx, nely, nelz, volfrac, seed=None, amplitude=0.01):
+    """x0 = volfrac (+ U(-amplitude, amplitude) from default_rng(seed))."""
+    nel = nelx * nely * nelz
+    x = np.full(nel, float(volfrac))
+    if seed is not None:
+        x += np.random.default_rng(seed).uniform(-amplitude, amplitude, nel)
+    return x

@@ -1,0 +1,181 @@
+/*
+ * gputo.h -- C ABI of libgputo.so, the MI355X-native top3D125 MGCG
+ * elasticity solver (matrix-free hex8 operator, geometric multigrid
+ * preconditioned CG, topology-optimisation driver kernels).
+ *
+ * Plain C types only: host pointers are double / int arrays laid out as the
+ * reference's MEX receives them (column vectors, DOF = 3*node + d, node =
+ * (nely+1)*(nelz+1)*i + (nely+1)*k + j, 0-based).  Entry points whose name
+ * ends in _d take device pointers (hipMalloc'd by the caller) instead.
+ *
+ * Every entry point returns 0 on success or a negative gputo_status.
+ * A handle is not thread-safe; independent handles may be used from
+ * different threads.  Each handle owns one HIP stream.
+ */
+#ifndef GPUTO_H
+#define GPUTO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gputo_handle gputo_handle;
+
+/* Status codes.  The reference reports only nit = -1 (workspace shortfall,
+ * Cantilever/omp/mgcg1.c:91-97) and otherwise calls exit(); here every
+ * failure is a return code and nit < 0 carries the same code. */
+enum gputo_status {
+    GPUTO_OK = 0,
+    GPUTO_ERR_WORKSPACE = -1,    /* device memory / coarsest size limit */
+    GPUTO_ERR_NOT_SPD = -2,      /* coarsest matrix not positive definite */
+    GPUTO_ERR_NONFINITE = -3,    /* NaN/Inf in an input or in the iteration */
+    GPUTO_ERR_DEVICE = -4,       /* HIP runtime error */
+    GPUTO_ERR_ARGUMENT = -5      /* invalid sizes or arguments */
+};
+
+/* Multigrid presets.
+ *  GPUTO_PRESET_REF_C: constants of the reference C/CUDA MEX
+ *    (pre-relaxation U = 0.7 F/AD, 3 pre and 3 post damped-Jacobi sweeps
+ *    U = 0.52 U + 0.48 (R + AD U)/AD, level 1 rediscretised from child
+ *    densities, levels >= 2 element Galerkin, coarsest 1e10 diagonal
+ *    penalty with the reference's node table;
+ *    Cantilever/omp/mgcg1.c:110,308,353,363-364, mgcg1.cu:188,1196,1205).
+ *  GPUTO_PRESET_MATLAB_CFG0: the MATLAB top3D125mgcg smoother (omega 0.6,
+ *    one sweep down from zero, one sweep up; Cantilever/TOP3D125MGCG/
+ *    top3D125mgcg.m:190-246) on element-Galerkin operators at every
+ *    coarse level. */
+enum gputo_preset { GPUTO_PRESET_REF_C = 0, GPUTO_PRESET_MATLAB_CFG0 = 1 };
+
+/* Arithmetic of the solve.
+ *  GPUTO_PREC_F64: everything in FP64 (the reference's arithmetic).
+ *  GPUTO_PREC_MIXED_F32: FP64 iterative refinement around an inner PCG
+ *    whose matvec and smoother run in FP32 (FP32 accumulation); dot
+ *    products, alpha/beta, the outer residual and the coarsest Cholesky
+ *    stay in FP64.
+ *  GPUTO_PREC_MIXED_BF16: as MIXED_F32 with the fine-level Ke x u
+ *    contraction on BF16 MFMA (split-BF16 operands, FP32 accumulation). */
+enum gputo_precision { GPUTO_PREC_F64 = 0, GPUTO_PREC_MIXED_F32 = 1, GPUTO_PREC_MIXED_BF16 = 2 };
+
+typedef struct gputo_params {
+    int preset;        /* enum gputo_preset */
+    int precision;     /* enum gputo_precision */
+    int learned;       /* 1: apply the learned level-2 correction inside flexible CG */
+    double inner_tol;  /* relative tolerance of one inner solve (mixed precision) */
+    int print_level;   /* 0 silent; >=1 summary on stderr; >=2 per-iteration */
+} gputo_params;
+
+/* Fill *p with the defaults (REF_C, F64, no learned stage, inner_tol 1e-4). */
+void gputo_default_params(gputo_params *p);
+
+/* Create a solver for an nelx x nely x nelz hex mesh with nl multigrid
+ * levels (each size divisible by 2^(nl-1)).  Replaces the per-call setup of
+ * mgcg1 (Cantilever/TOP3D125MGCGCUDA/mgcg1.cu:184-233, estWorkSpace
+ * 293-322): the device, workspace and stream persist across solves. */
+int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *params, gputo_handle **out);
+int gputo_destroy(gputo_handle *h);
+int gputo_set_params(gputo_handle *h, const gputo_params *params);
+
+/* The mgcg1 MEX solve (mgcg1.cu:120-182, mgcg1.c:51-105): K(CX) U = F with
+ * zero initial guess, Dirichlet DOFs `fixed` (1-based doubles, nfd of them),
+ * stop when ||r||/||F|| < tol or after maxit iterations.  U (ndof doubles)
+ * is overwritten with the solution in the MATLAB frame (y components
+ * negated, mgcg1.cu:282-283).  *relres gets the recursively updated
+ * relative residual, *nit the 0-based exit index (mgcg1.cu:285-286).
+ * Ke0 is the 24x24 element matrix as MATLAB passes it (column-major,
+ * symmetric). */
+int gputo_mgcg1(gputo_handle *h, const double *Ke0, const double *F, double *U, const double *CX,
+                const double *fixed, int nfd, double tol, int maxit, double *relres, int *nit);
+
+/* The same solve on device-resident data (no host copies, no y flip):
+ * d_F, d_U (ndof doubles), d_CX (nel doubles), d_fixed (nfd int32,
+ * 0-based DOF indices).  Ke0 is a host array.  Results as above. */
+int gputo_solve_d(gputo_handle *h, const double *Ke0, const double *d_F, double *d_U, const double *d_CX,
+                  const int *d_fixed, int nfd, double tol, int maxit, double *relres, int *nit);
+
+/* Workspace estimate in doubles, with the reference's formula
+ * (mgcg1.cu:293-322) -- kept so that callers that size memory from it keep
+ * working; this library's own footprint is gputo_device_bytes(). */
+long long gputo_est_workspace(int nfd, int nl, int nelx, int nely, int nelz);
+long long gputo_device_bytes(const gputo_handle *h);
+
+/* ---- kernel-level entry points (parity tests, microbenchmarks) ------- */
+/* Host arrays in/out; each call runs on the handle's stream and waits.  */
+
+/* y = K(CX) u with y[fixed] = 0 (multAV0c + ZeroBCc, mgcg1.cu:941-997,1326). */
+int gputo_matvec_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *u,
+                     const double *fixed, int nfd, double *y);
+/* r = F - K(CX) u, r[fixed] = 0 (defect0c + ZeroBCc, mgcg1.cu:1208-1264). */
+int gputo_defect_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *F,
+                     const double *u, const double *fixed, int nfd, double *r);
+/* Diagonal of K(CX) (prepAD0c, mgcg1.cu:1447-1487). */
+int gputo_diag_f64(gputo_handle *h, const double *Ke0, const double *CX, double *ad);
+/* One V-cycle z = V(r) on the hierarchy built for (Ke0, CX, fixed)
+ * (vcycle, mgcg1.cu:468-651). */
+int gputo_vcycle_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *fixed, int nfd,
+                     const double *r, double *z);
+/* Level data of the hierarchy built by the last solve / vcycle / setup:
+ * element matrices of level l (nel_l x 24 x 24 row-major, l >= 1),
+ * diagonal (ndof_l) and fixed list (1-based doubles, returned count). */
+int gputo_setup(gputo_handle *h, const double *Ke0, const double *CX, const double *fixed, int nfd);
+int gputo_level_info(const gputo_handle *h, int level, int *nelx, int *nely, int *nelz, int *nfd);
+int gputo_level_matrices(gputo_handle *h, int level, double *A, double *ad, double *fixed);
+/* Grid transfers on level l (fine) <-> l+1 (coarse) of the current
+ * hierarchy: restriction F_c = P^T r_f with F_c[fixed_c] = 0 (rstrFc,
+ * mgcg1.cu:792-831) and prolongation u_f += P u_c (interpc, 1587-1641). */
+int gputo_restrict_f64(gputo_handle *h, int level, const double *r_fine, double *f_coarse);
+int gputo_interp_f64(gputo_handle *h, int level, const double *u_coarse, double *u_fine);
+/* Mean duration (ms, HIP events on the handle's stream) of `reps`
+ * back-to-back launches of the level-0 operator y = K(CX) u of the given
+ * precision on the hierarchy of the last solve / setup (microbenchmark of
+ * multAV0c, mgcg1.cu:941-997). */
+int gputo_bench_operator(gputo_handle *h, int precision, int reps, double *avg_ms);
+/* Coarsest-level solve x = A_c^{-1} b (cholsl1, mgcg1.cu:2175-2232). */
+int gputo_coarse_solve_f64(gputo_handle *h, const double *b, double *x);
+
+/* ---- index maps (bit-exact parity with the reference) ---------------- */
+/* MATLAB connectivity cMat, 1-based DOFs, MATLAB local order
+ * (top3D125mod.m:37-41); nel x 24 int32 row-major. */
+int gputo_edofmat(gputo_handle *h, int *cmat);
+/* Fixed-DOF list of level l as the reference's rstrFD builds it
+ * (mgcg1.c:532-562): 1-based doubles in fine-list order. */
+int gputo_fixed_list(const gputo_handle *h, int level, double *fixed, int *nfd);
+
+/* ---- topology-optimisation driver (top3D125mod.m:105-165) ------------ */
+typedef struct gputo_to_config {
+    double volfrac, penal, rmin, eta, beta, move, Emin, E0;
+    int ft;            /* 1: density filter; 3: filter + volume-preserving projection */
+    double cg_tol;
+    int cg_maxit;
+} gputo_to_config;
+
+typedef struct gputo_to_stats {
+    double compliance, volume, change, eta, lmid, relres;
+    int cg_iters, status;
+    double seconds_total, seconds_solve;
+    int inner_iters, outer_iters, learned_accepted, learned_rejected;
+} gputo_to_stats;
+
+/* Start a design loop on this handle: x0 (nel doubles, design variables),
+ * F (ndof doubles, C frame), fixed (1-based doubles).  Device-resident
+ * state persists until gputo_to_end. */
+int gputo_to_begin(gputo_handle *h, const double *Ke0, const double *x0, const double *F, const double *fixed,
+                   int nfd, const gputo_to_config *cfg);
+/* One design iteration: filter, projection, CX, MGCG solve, sensitivity,
+ * filtered sensitivities, OC bisection (top3D125mod.m:105-165). */
+int gputo_to_iterate(gputo_handle *h, gputo_to_stats *st);
+/* Copy state back: x, xPhys (nel doubles, may be NULL), U (ndof, C frame,
+ * may be NULL). */
+int gputo_to_get(gputo_handle *h, double *x, double *xPhys, double *U);
+int gputo_to_end(gputo_handle *h);
+
+/* Device-side driver kernels on host arrays (parity tests). */
+int gputo_filter_f64(gputo_handle *h, const double *in, double rmin, double *out);
+int gputo_sens_f64(gputo_handle *h, const double *Ke0, const double *U_cframe, double *ce);
+
+/* Library version string. */
+const char *gputo_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUTO_H */

@@ -1,0 +1,42 @@
+"""The C-ABI library loads and exports every symbol include/gputo.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "gputo.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gputo_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_exported(gputo):
+    from gputo import _lib
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    from gputo import _lib
+    assert set(declared_symbols()) == set(_lib.SIGNATURES)
+
+
+def test_workspace_estimate_formula(gputo):
+    """estWorkSpace of mgcg1.cu:293-322, in doubles (pure host arithmetic)."""
+    lib = gputo.load()
+    nelx, nely, nelz, nl, nfd = 48, 24, 24, 4, 1875
+    ndof0 = 3 * 49 * 25 * 25
+    lw = ndof0 * 6 + nelx * nely * nelz + nfd
+    for lev in range(nl):
+        ex, ey, ez = nelx >> lev, nely >> lev, nelz >> lev
+        nd = 3 * (ex + 1) * (ey + 1) * (ez + 1)
+        lw += 2 * nd if lev == 0 else 4 * nd + 300 * ex * ey * ez + nfd
+        if lev == nl - 1:
+            lw += 3 * nd + nd * nd
+    assert lib.gputo_est_workspace(nfd, nl, nelx, nely, nelz) == lw
