@@ -1,0 +1,193 @@
+"""Headline benchmark: wall-seconds per topology-optimisation design iteration
+of the top3D125 cantilever at 192x96x96 hex (1,769,472 elements, 5,447,811
+DOFs, 5 multigrid levels, CG tolerance 1e-8) on one MI355X, with the
+design loop (filter, SIMP, MGCG solve, sensitivities, OC) resident in HBM.
+
+A step is one design iteration.  Prints one JSON line (rank 0, N = 1).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "gpu-to_amd"), os.path.join(ROOT, "oracle")]
+
+METRIC = "wall-s/TO iter @1.77M hex, mixed+learned vs FP64 MGCG; matvec HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+VARIANT_PREC = {"f64": 0, "mixed_f32": 1, "mixed_bf16": 2}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_variant(gputo, name, args, Ke, F, fixed, x0):
+    prec = VARIANT_PREC[name.replace("+learned", "")]
+    learned = name.endswith("+learned")
+    s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=learned,
+                     inner_tol=args.inner_tol)
+    run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
+                       cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
+    for w in range(args.warmup):
+        st = run.iterate()
+        log("[%s] warmup %d: C=%.6e nit=%d solve=%.3fs total=%.3fs" % (name, w, st["compliance"], st["cg_iters"],
+                                                                       st["seconds_solve"], st["seconds_total"]))
+    s.lib.gputo_device_synchronize()
+    t0 = time.perf_counter()
+    hist = [run.iterate() for _ in range(args.steps)]
+    s.lib.gputo_device_synchronize()
+    wall = time.perf_counter() - t0
+    for k, st in enumerate(hist):
+        log("[%s] step %d: C=%.8e nit=%d relres=%.2e solve=%.3fs total=%.3fs" % (
+            name, k, st["compliance"], st["cg_iters"], st["relres"], st["seconds_solve"], st["seconds_total"]))
+    x, xp, U = run.state()
+    out = {
+        "s_per_iter": wall / args.steps,
+        "solve_s_per_iter": float(np.mean([h["seconds_solve"] for h in hist])),
+        "cg_iters": [h["cg_iters"] for h in hist],
+        "compliance": [h["compliance"] for h in hist],
+        "relres": [h["relres"] for h in hist],
+        "U": U,
+    }
+    # dominant kernel: the level-0 operator, timed with HIP events on the
+    # solver's stream on the hierarchy of the last timed solve
+    avg = ctypes.c_double()
+    rc = s.lib.gputo_bench_operator(s.h, prec, args.op_reps, ctypes.byref(avg))
+    out["op_ms"] = avg.value if rc == 0 else None
+    run.close()
+    s.close()
+    return out
+
+
+def cpu_baseline(args, Ke, F, fixed, x0, nit_avg):
+    """Oracle (C/OpenMP restatement of mgcg1.c + top3D125mod.m) on the host:
+    two design iterations from x0 with the CG capped at 1 and 3 iterations
+    give the fixed cost and the per-CG-iteration cost, extrapolated to the
+    GPU's measured CG count (same algorithm, same iteration counts)."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    oracle.set_num_threads(threads)
+    times = {}
+    for cap in (1, 3):
+        st = oracle.DesignState(args.nelx, args.nely, args.nelz, x0)
+        t0 = time.perf_counter()
+        o = oracle.design_iter(st, Ke, F, fixed, args.nl, args.cg_tol, cap, 0.12, 3.0, np.sqrt(3.0), 1, 2.0, 0.2)
+        times[cap] = time.perf_counter() - t0
+        log("[cpu] design iteration with CG capped at %d: %.2fs (rc %d)" % (cap, times[cap], o["rc"]))
+    t_it = (times[3] - times[1]) / 2
+    t_fixed = times[1] - t_it
+    return {
+        "value": t_fixed + nit_avg * t_it,
+        "unit": "s/iter",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": ("oracle design iteration at %dx%dx%d from x0 with CG capped at 1 and 3 iterations "
+                   "(%.1fs + %.1fs); per-CG-iteration %.3fs and fixed %.3fs extrapolated to the GPU's "
+                   "mean %.1f CG iterations" % (args.nelx, args.nely, args.nelz, times[1], times[3], t_it,
+                                                 t_fixed, nit_avg)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nelx", type=int, default=192)
+    ap.add_argument("--nely", type=int, default=96)
+    ap.add_argument("--nelz", type=int, default=96)
+    ap.add_argument("--nl", type=int, default=5)
+    ap.add_argument("--cg-tol", type=float, default=1e-8)
+    ap.add_argument("--cg-maxit", type=int, default=300)
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--variants", default="f64")
+    ap.add_argument("--op-reps", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import gputo
+    lib = gputo.load()
+    Ke, F, fixed = gputo.cantilever(args.nelx, args.nely, args.nelz)
+    x0 = gputo.problems.perturbed_density(args.nelx, args.nely, args.nelz, 0.12, seed=args.seed)
+    nel = args.nelx * args.nely * args.nelz
+    ndof = F.size
+
+    variants = args.variants.split(",")
+    res = {}
+    for v in variants:
+        res[v] = run_variant(gputo, v, args, Ke, F, fixed, x0)
+    head = variants[-1]
+    r = res[head]
+    base = res.get("f64")
+
+    # roofline of the dominant kernel (level-0 operator)
+    s_bytes = 8 if head == "f64" else 4
+    op_bytes = (2 * ndof + nel) * s_bytes
+    op_flops = 1152.0 * nel
+    op_s = r["op_ms"] * 1e-3 if r["op_ms"] else None
+    if head == "f64":
+        roof = {"bound": "mfma", "achieved": op_flops / op_s / 1e12 if op_s else None, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "traffic": None}
+    else:
+        roof = {"bound": "hbm", "achieved": op_bytes / op_s / 1e9 if op_s else None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "traffic": None}
+    roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
+
+    nit_avg = float(np.mean(r["cg_iters"]))
+    out = {
+        "metric": METRIC,
+        "value": r["s_per_iter"],
+        "unit": "s/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["s_per_iter"] * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == "f64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "cantilever %dx%dx%d hex (%d elements, %d DOFs), nl=%d, cg_tol %g, x0=0.12+U(-0.01,0.01) "
+                        "seed %d, density filter rmin=sqrt(3), OC move 0.2, preset ref_c"
+                        % (args.nelx, args.nely, args.nelz, nel, ndof, args.nl, args.cg_tol, args.seed),
+            "variant": head,
+        },
+        "roofline": roof,
+        "matvec": {
+            "kernel": "level-0 operator y = K(E) u (%s)" % head,
+            "ms": r["op_ms"],
+            "gelem_per_s": nel / op_s / 1e9 if op_s else None,
+            "hbm_gbps_algorithmic": op_bytes / op_s / 1e9 if op_s else None,
+            "hbm_frac": op_bytes / op_s / 1e9 / HBM_PEAK_GBS if op_s else None,
+            "algorithmic_bytes": op_bytes,
+            "algorithmic_flops": op_flops,
+        },
+        "cg_iters": r["cg_iters"],
+        "solve_s_per_iter": r["solve_s_per_iter"],
+        "variants": {v: {k: res[v][k] for k in ("s_per_iter", "solve_s_per_iter", "cg_iters", "op_ms")}
+                     for v in variants},
+    }
+    if base is not None and head != "f64":
+        out["speedup_vs_f64"] = base["s_per_iter"] / r["s_per_iter"]
+        Ub, Uh = base["U"], r["U"]
+        out["u_rel_err_vs_f64"] = float(np.linalg.norm(Uh - Ub) / np.linalg.norm(Ub))
+        out["compliance_rel_err_vs_f64"] = float(abs(r["compliance"][-1] - base["compliance"][-1])
+                                                 / abs(base["compliance"][-1]))
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, Ke, F, fixed, x0, nit_avg)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

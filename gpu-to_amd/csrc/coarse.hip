@@ -402,30 +402,36 @@ int coarse_factor(hipStream_t s, Coarse &C)
 // (one 64-lane wave per row, fixed reduction order), then x[fixed] = 0
 // (mgcg1.c:431-432).
 // ---------------------------------------------------------------------------
-template <bool UPPER>
-__global__ __launch_bounds__(64) void k_trmv(const double *__restrict__ M, const double *__restrict__ b,
-                                            double *__restrict__ x, int N, int n, const uint8_t *__restrict__ mask,
+template <bool UPPER, class Ti, class To>
+__global__ __launch_bounds__(64) void k_trmv(const double *__restrict__ M, const Ti *__restrict__ b,
+                                            To *__restrict__ x, int N, int n, const uint8_t *__restrict__ mask,
                                             const int *ctl)
 {
     if (ctl && ctl[0]) return;
     const int row = blockIdx.x;
     const int lo = UPPER ? row : 0, hi = UPPER ? n - 1 : (row < n ? row : n - 1);
     double s = 0.0;
-    for (int c = lo + threadIdx.x; c <= hi; c += 64) s += M[(long)row * N + c] * b[c];
+    for (int c = lo + threadIdx.x; c <= hi; c += 64) s += M[(long)row * N + c] * (double)b[c];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (threadIdx.x == 0 && row < n) {
         if (UPPER && mask && ((mask[row / 3] >> (row % 3)) & 1u)) s = 0.0;
-        x[row] = s;
+        x[row] = (To)s;
     }
 }
 
-void launch_coarse_solve(hipStream_t s, const Coarse &C, const double *b, double *x, const uint8_t *mask,
-                         const int *ctl)
+// The solve is FP64 for both vector types: the right-hand side is widened,
+// the result rounded once on output.
+template <class T>
+void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl)
 {
     const int N = (C.n + NB - 1) / NB * NB;
-    k_trmv<false><<<C.n, 64, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);
-    k_trmv<true><<<C.n, 64, 0, s>>>(C.LinvT, C.y, x, N, C.n, mask, ctl);
+    k_trmv<false, T, double><<<C.n, 64, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);
+    k_trmv<true, double, T><<<C.n, 64, 0, s>>>(C.LinvT, C.y, x, N, C.n, mask, ctl);
 }
+template void launch_coarse_solve<double>(hipStream_t, const Coarse &, const double *, double *, const uint8_t *,
+                                          const int *);
+template void launch_coarse_solve<float>(hipStream_t, const Coarse &, const float *, float *, const uint8_t *,
+                                         const int *);
 
 }  // namespace gputo

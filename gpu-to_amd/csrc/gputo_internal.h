@@ -41,23 +41,25 @@ struct Tile {
 
 Tile choose_tile(const Grid &g);
 
-// Arguments of the node-gather operator kernels.
-struct OpArgs {
+// Arguments of the node-gather operator kernels (T = double or float).
+template <class T>
+struct OpArgsT {
     Grid g;
     Tile t;
-    const double *Ke;      // 24x24 row-major (level 0)
-    const double *CX;      // element scale (level 0)
-    const double *A;       // packed 300 x nel entry-major element matrices (levels >= 1)
-    const double *U;       // input vector
-    const double *F;       // right-hand side (defect / Jacobi)
-    const double *AD;      // diagonal (Jacobi)
+    const T *Ke;           // 24x24 row-major (level 0)
+    const T *CX;           // element scale (level 0)
+    const T *A;            // packed 300 x nel entry-major element matrices (levels >= 1)
+    const T *U;            // input vector
+    const T *F;            // right-hand side (defect / Jacobi)
+    const T *AD;           // diagonal (Jacobi)
     const uint8_t *mask;   // per node: bit d set when DOF 3n+d is fixed
-    double *out;           // result vector
-    double *out2;          // second result (prerelax+defect writes U and R)
+    T *out;                // result vector
+    T *out2;               // second result (prerelax+defect writes U and R)
     double *partials;      // per-block partial sums (fused dot), may be null
     const int *ctl;        // ctl[0] != 0: skip (converged)
     double omega, prerelax;
 };
+using OpArgs = OpArgsT<double>;
 
 enum OpMode {
     OP_MATVEC = 0,      // out = K U, masked; partials = sum U.out
@@ -92,6 +94,8 @@ struct Coarse {
     int *flag = nullptr;     // SPD failure flag
 };
 
+struct MixedState;
+
 }  // namespace gputo
 
 struct gputo_handle {
@@ -113,40 +117,43 @@ struct gputo_handle {
     bool hier_ready = false;
     std::vector<void *> allocs;
     struct ToState *to = nullptr;
-    struct MixedState *mx = nullptr;
+    gputo::MixedState *mx = nullptr;
 };
 
 namespace gputo {
 
-// ---- launch helpers (kernels_f64.hip) --------------------------------
-void launch_op0(hipStream_t s, int mode, const OpArgs &a);
-void launch_opA(hipStream_t s, int mode, const OpArgs &a);
+// ---- launch helpers (kernels.hip) ------------------------------------
+template <class T> void launch_op0(hipStream_t s, int mode, const OpArgsT<T> &a);
+template <class T> void launch_opA(hipStream_t s, int mode, const OpArgsT<T> &a);
 void launch_diag0(hipStream_t s, const Grid &g, const double *Ke, const double *CX, double *AD);
 void launch_diagA(hipStream_t s, const Grid &g, const double *A, double *AD);
-void launch_restrict(hipStream_t s, const Grid &gc, const double *Rf, double *Fc, const uint8_t *maskc, const int *ctl);
-void launch_interp(hipStream_t s, const Grid &gf, const double *Uc, double *Uf, const uint8_t *maskf, int zero_fixed,
+template <class T>
+void launch_restrict(hipStream_t s, const Grid &gc, const T *Rf, T *Fc, const uint8_t *maskc, const int *ctl);
+template <class T>
+void launch_interp(hipStream_t s, const Grid &gf, const T *Uc, T *Uf, const uint8_t *maskf, int zero_fixed,
                    const int *ctl);
-void launch_fill(hipStream_t s, double *v, long n, double val);
-void launch_dot_partial(hipStream_t s, const double *a, const double *b, long n, double *partials, const int *ctl);
-void launch_copy(hipStream_t s, const double *src, double *dst, long n, const int *ctl);
+template <class T> void launch_fill(hipStream_t s, T *v, long n, double val);
+template <class Ta, class Tb>
+void launch_dot_partial(hipStream_t s, const Ta *a, const Tb *b, long n, double *partials, const int *ctl);
+template <class Ti, class To> void launch_copy(hipStream_t s, const Ti *src, To *dst, long n, const int *ctl);
 void launch_coarse_l1(hipStream_t s, const Grid &gc, const double *Aelm, const double *CXf, double *A);
 void launch_coarse_galerkin(hipStream_t s, const Grid &gc, const double *Af, const double *Ke, const double *CXf,
                             double *Ac);
 void launch_corrcsa(hipStream_t s, const Grid &g, const int *fixed_d, int nfd, double *A, int mirrored);
 void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ);
 int coarse_factor(hipStream_t s, Coarse &C);
-void launch_coarse_solve(hipStream_t s, const Coarse &C, const double *b, double *x, const uint8_t *mask,
-                         const int *ctl);
+template <class T>
+void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl);
 
 // PCG scalar steps (single-block kernels reading the reduction partials).
 void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
 void launch_cg_rho0(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
 void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
-void launch_cg_update_ur(hipStream_t s, double *U, double *R, const double *P, const double *Q, long n, CgScalars *sc,
-                         double *partials);
+template <class T>
+void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials);
 void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit);
 void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit);
-void launch_cg_update_p(hipStream_t s, double *P, const double *Z, long n, CgScalars *sc);
+template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc);
 
 void check_hip(hipError_t e, const char *what);
 void to_release(gputo_handle *h);  // frees the design-loop state (driver.hip)

@@ -1,6 +1,9 @@
-// FP64 kernels of the MGCG hierarchy: node-gather operators with LDS-staged
-// nodal bricks, operator diagonals, grid transfers, coarse operators,
-// dense coarsest factorisation and the device-side PCG scalar steps.
+// Kernels of the MGCG hierarchy, instantiated for FP64 (the reference
+// arithmetic) and FP32 (the inner solve of the mixed-precision path, FP32
+// accumulation): node-gather operators with LDS-staged nodal bricks,
+// operator diagonals, grid transfers, BLAS-1 with deterministic two-stage
+// reductions (always accumulated in FP64) and the device-side PCG scalar
+// steps.
 //
 // Layout: DOF vector [3*nnod], node id = nny*nnz*i + nny*k + j (j fastest);
 // element scales [nel], element id = ny*nz*i + ny*k + j; coarse element
@@ -51,12 +54,12 @@ __device__ __forceinline__ double block_sum(double v)
 // Ke indices are compile-time constants, so Ke is read through the scalar
 // cache.  No atomics: each output DOF is written by exactly one thread.
 // ---------------------------------------------------------------------------
-template <int MODE, bool STORED>
-__global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
+template <class T, int MODE, bool STORED>
+__global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
 {
     if (a.ctl && a.ctl[0]) return;
-    __shared__ double su[3][kMaxHalo];
-    __shared__ double sE[STORED ? 1 : kMaxElemBox];
+    __shared__ T su[3][kMaxHalo];
+    __shared__ T sE[STORED ? 1 : kMaxElemBox];
     const Grid g = a.g;
     const Tile t = a.t;
     int b = blockIdx.x;
@@ -74,11 +77,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
         const int hk = row % HK, hi = row / HK;
         const int hj = col / 3, c = col - 3 * hj;
         const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
-        double v = 0.0;
+        T v = T(0);
         if (gi >= 0 && gi < g.nnx && gj >= 0 && gj < g.nny && gk >= 0 && gk < g.nnz) {
             const long dof = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + gj) + c;
             if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
-                v = a.prerelax * a.F[dof] / a.AD[dof];
+                v = T(a.prerelax) * a.F[dof] / a.AD[dof];
             else
                 v = a.U[dof];
         }
@@ -92,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
             const int r = q / EJ;
             const int ek = r % EK, ei = r / EK;
             const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = k0 - 1 + ek;
-            double v = 0.0;
+            T v = T(0);
             if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz)
                 v = a.CX[(long)g.ny * g.nz * gi + (long)g.ny * gk + gj];
             sE[q] = v;
@@ -109,12 +112,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
         const int gi = i0 + ti, gj = j0 + tj, gk = k0 + tk;
         if (gi < g.nnx && gj < g.nny && gk < g.nnz) {
             const int hb = (ti * HK + tk) * HJ + tj;  // halo index of neighbour offset (0,0,0)
-            double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+            T y0 = T(0), y1 = T(0), y2 = T(0);
             if (!STORED) {
-                const double *__restrict__ Ke = a.Ke;
-                double z[8][3];
+                const T *__restrict__ Ke = a.Ke;
+                T z[8][3];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) z[e][0] = z[e][1] = z[e][2] = 0.0;
+                for (int e = 0; e < 8; ++e) z[e][0] = z[e][1] = z[e][2] = T(0);
 #pragma unroll
                 for (int oi = 0; oi < 3; ++oi)
 #pragma unroll
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
 #pragma unroll
                         for (int oj = 0; oj < 3; ++oj) {
                             const int h = hb + (oi * HK + ok) * HJ + oj;
-                            const double u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
+                            const T u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
 #pragma unroll
                             for (int ei = 0; ei < 2; ++ei)
 #pragma unroll
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
                                         const int e = 4 * ei + 2 * ek + ej;
 #pragma unroll
                                         for (int c = 0; c < 3; ++c) {
-                                            const double *kr = Ke + 24 * (3 * la + c) + 3 * lb;
+                                            const T *kr = Ke + 24 * (3 * la + c) + 3 * lb;
                                             z[e][c] += kr[0] * u0 + kr[1] * u1 + kr[2] * u2;
                                         }
                                     }
@@ -149,14 +152,14 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
 #pragma unroll
                         for (int ej = 0; ej < 2; ++ej) {
                             const int e = 4 * ei + 2 * ek + ej;
-                            const double E = sE[eb + (ei * EK + ek) * EJ + ej];
+                            const T E = sE[eb + (ei * EK + ek) * EJ + ej];
                             y0 += E * z[e][0];
                             y1 += E * z[e][1];
                             y2 += E * z[e][2];
                         }
             } else {
                 const long nel = g.nel;
-                const double *__restrict__ A = a.A;
+                const T *__restrict__ A = a.A;
 #pragma unroll
                 for (int ei = 0; ei < 2; ++ei)
 #pragma unroll
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
                         for (int ej = 0; ej < 2; ++ej) {
                             const int ex = gi - 1 + ei, ey = gj - 1 + ej, ez = gk - 1 + ek;
                             if (ex < 0 || ex >= g.nx || ey < 0 || ey >= g.ny || ez < 0 || ez >= g.nz) continue;
-                            const double *__restrict__ Ae = A + ((long)g.ny * g.nz * ex + (long)g.ny * ez + ey);
+                            const T *__restrict__ Ae = A + ((long)g.ny * g.nz * ex + (long)g.ny * ez + ey);
                             const int la = lidx(1 - ei, 1 - ej, 1 - ek);
 #pragma unroll
                             for (int bx = 0; bx < 2; ++bx)
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
 #pragma unroll
                                     for (int by = 0; by < 2; ++by) {
                                         const int h = hb + ((ei + bx) * HK + (ek + bz)) * HJ + (ej + by);
-                                        const double u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
+                                        const T u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
                                         const int lb = lidx(bx, by, bz);
                                         y0 += Ae[pidx(3 * la + 0, 3 * lb + 0) * nel] * u0 +
                                               Ae[pidx(3 * la + 0, 3 * lb + 1) * nel] * u1 +
@@ -191,28 +194,29 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
             const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
             const long dof = 3 * nod;
             const unsigned m = a.mask[nod];
-            const double y[3] = {y0, y1, y2};
+            const T y[3] = {y0, y1, y2};
             const int hc = hb + (HK + 1) * HJ + 1;  // the node itself
+            const T om = T(a.omega), one = T(1);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const bool fixed = (m >> c) & 1u;
                 if (MODE == OP_MATVEC) {
-                    const double v = fixed ? 0.0 : y[c];
+                    const T v = fixed ? T(0) : y[c];
                     a.out[dof + c] = v;
-                    part += su[c][hc] * v;
+                    part += (double)su[c][hc] * (double)v;
                 } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
-                    const double rv = fixed ? 0.0 : a.F[dof + c] - y[c];
+                    const T rv = fixed ? T(0) : a.F[dof + c] - y[c];
                     a.out[dof + c] = rv;
                     if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = su[c][hc];
                 } else {
-                    const double f = a.F[dof + c];
-                    const double rv = fixed ? 0.0 : f - y[c];
-                    const double u = su[c][hc];
-                    const double ad = a.AD[dof + c];
+                    const T f = a.F[dof + c];
+                    const T rv = fixed ? T(0) : f - y[c];
+                    const T u = su[c][hc];
+                    const T ad = a.AD[dof + c];
                     // U = (1-w) U + w (R + AD U)/AD  (mgcg1.cu:1199-1206)
-                    const double un = (1.0 - a.omega) * u + a.omega * (rv + ad * u) / ad;
+                    const T un = (one - om) * u + om * (rv + ad * u) / ad;
                     a.out[dof + c] = un;
-                    part += f * un;
+                    part += (double)f * (double)un;
                 }
             }
         }
@@ -223,22 +227,34 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgs a)
     }
 }
 
-template <bool STORED>
-static void launch_gather(hipStream_t s, int mode, const OpArgs &a)
+template <class T, bool STORED>
+static void launch_gather(hipStream_t s, int mode, const OpArgsT<T> &a)
 {
     const dim3 grid((unsigned)a.t.blocks()), block(kBlock);
     switch (mode) {
-    case OP_MATVEC: k_gather<OP_MATVEC, STORED><<<grid, block, 0, s>>>(a); break;
-    case OP_DEFECT: k_gather<OP_DEFECT, STORED><<<grid, block, 0, s>>>(a); break;
-    case OP_JACOBI: k_gather<OP_JACOBI, STORED><<<grid, block, 0, s>>>(a); break;
-    case OP_PRE_JACOBI: k_gather<OP_PRE_JACOBI, STORED><<<grid, block, 0, s>>>(a); break;
-    case OP_PRE_DEFECT: k_gather<OP_PRE_DEFECT, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_MATVEC: k_gather<T, OP_MATVEC, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_DEFECT: k_gather<T, OP_DEFECT, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_JACOBI: k_gather<T, OP_JACOBI, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_JACOBI: k_gather<T, OP_PRE_JACOBI, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_DEFECT: k_gather<T, OP_PRE_DEFECT, STORED><<<grid, block, 0, s>>>(a); break;
     default: break;
     }
 }
 
-void launch_op0(hipStream_t s, int mode, const OpArgs &a) { launch_gather<false>(s, mode, a); }
-void launch_opA(hipStream_t s, int mode, const OpArgs &a) { launch_gather<true>(s, mode, a); }
+template <class T>
+void launch_op0(hipStream_t s, int mode, const OpArgsT<T> &a)
+{
+    launch_gather<T, false>(s, mode, a);
+}
+template <class T>
+void launch_opA(hipStream_t s, int mode, const OpArgsT<T> &a)
+{
+    launch_gather<T, true>(s, mode, a);
+}
+template void launch_op0<double>(hipStream_t, int, const OpArgsT<double> &);
+template void launch_op0<float>(hipStream_t, int, const OpArgsT<float> &);
+template void launch_opA<double>(hipStream_t, int, const OpArgsT<double> &);
+template void launch_opA<float>(hipStream_t, int, const OpArgsT<float> &);
 
 // ---------------------------------------------------------------------------
 // Operator diagonals (prepAD0c / prepADc, mgcg1.cu:1404-1487): one thread
@@ -290,7 +306,8 @@ void launch_diagA(hipStream_t s, const Grid &g, const double *A, double *AD)
 // mgcg1.cu:792-831): one thread per coarse node gathers the 27 fine nodes
 // around (2I,2J,2K) with weight 2^-(|di|+|dj|+|dk|).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const double *__restrict__ Rf, double *__restrict__ Fc,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const T *__restrict__ Rf, T *__restrict__ Fc,
                                                     const uint8_t *__restrict__ maskc, const int *ctl)
 {
     if (ctl && ctl[0]) return;
@@ -300,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const double *__re
     const int K = (int)((nod / gc.nny) % gc.nnz);
     const int I = (int)(nod / ((long)gc.nny * gc.nnz));
     const int fny = 2 * gc.ny + 1, fnz = 2 * gc.nz + 1, fnx = 2 * gc.nx + 1;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    T s0 = T(0), s1 = T(0), s2 = T(0);
     for (int di = -1; di <= 1; ++di) {
         const int iP = 2 * I + di;
         if (iP < 0 || iP >= fnx) continue;
@@ -310,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const double *__re
             for (int dj = -1; dj <= 1; ++dj) {
                 const int jP = 2 * J + dj;
                 if (jP < 0 || jP >= fny) continue;
-                const double w = 1.0 / (double)(1 << (abs(di) + abs(dj) + abs(dk)));
+                const T w = T(1) / (T)(1 << (abs(di) + abs(dj) + abs(dk)));
                 const long f = 3 * ((long)fny * fnz * iP + (long)fny * kP + jP);
                 s0 += w * Rf[f];
                 s1 += w * Rf[f + 1];
@@ -319,22 +336,26 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const double *__re
         }
     }
     const unsigned m = maskc[nod];
-    Fc[3 * nod + 0] = (m & 1u) ? 0.0 : s0;
-    Fc[3 * nod + 1] = (m & 2u) ? 0.0 : s1;
-    Fc[3 * nod + 2] = (m & 4u) ? 0.0 : s2;
+    Fc[3 * nod + 0] = (m & 1u) ? T(0) : s0;
+    Fc[3 * nod + 1] = (m & 2u) ? T(0) : s1;
+    Fc[3 * nod + 2] = (m & 4u) ? T(0) : s2;
 }
 
-void launch_restrict(hipStream_t s, const Grid &gc, const double *Rf, double *Fc, const uint8_t *maskc, const int *ctl)
+template <class T>
+void launch_restrict(hipStream_t s, const Grid &gc, const T *Rf, T *Fc, const uint8_t *maskc, const int *ctl)
 {
-    k_restrict<<<(unsigned)((gc.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gc, Rf, Fc, maskc, ctl);
+    k_restrict<T><<<(unsigned)((gc.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gc, Rf, Fc, maskc, ctl);
 }
+template void launch_restrict<double>(hipStream_t, const Grid &, const double *, double *, const uint8_t *, const int *);
+template void launch_restrict<float>(hipStream_t, const Grid &, const float *, float *, const uint8_t *, const int *);
 
 // ---------------------------------------------------------------------------
 // Prolongation U_f += P U_c (interpc, mgcg1.cu:1587-1641) followed by
 // U_f[fixed] = 0 (mgcg1.cu:631): one thread per fine node averages the 1,
 // 2, 4 or 8 coarse nodes around it according to the parity of (i,j,k).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const double *__restrict__ Uc, double *__restrict__ Uf,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict__ Uc, T *__restrict__ Uf,
                                                   const uint8_t *__restrict__ maskf, int zero_fixed, const int *ctl)
 {
     if (ctl && ctl[0]) return;
@@ -345,8 +366,8 @@ __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const double *__rest
     const int i = (int)(nod / ((long)gf.nny * gf.nnz));
     const int cny = gf.ny / 2 + 1, cnz = gf.nz / 2 + 1;
     const int ia = i >> 1, ib = (i + 1) >> 1, ja = j >> 1, jb = (j + 1) >> 1, ka = k >> 1, kb = (k + 1) >> 1;
-    const double w = 1.0 / (double)(1 << ((i & 1) + (j & 1) + (k & 1)));
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    const T w = T(1) / (T)(1 << ((i & 1) + (j & 1) + (k & 1)));
+    T s0 = T(0), s1 = T(0), s2 = T(0);
     // corners (ia|ib, ja|jb, ka|kb); for even indices a == b and the
     // duplicate corners are skipped so each coarse node counts once
     for (int ci = 0; ci < 2; ++ci) {
@@ -367,63 +388,88 @@ __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const double *__rest
     }
     const unsigned m = zero_fixed ? maskf[nod] : 0u;
     const long d = 3 * nod;
-    Uf[d + 0] = (m & 1u) ? 0.0 : Uf[d + 0] + w * s0;
-    Uf[d + 1] = (m & 2u) ? 0.0 : Uf[d + 1] + w * s1;
-    Uf[d + 2] = (m & 4u) ? 0.0 : Uf[d + 2] + w * s2;
+    Uf[d + 0] = (m & 1u) ? T(0) : Uf[d + 0] + w * s0;
+    Uf[d + 1] = (m & 2u) ? T(0) : Uf[d + 1] + w * s1;
+    Uf[d + 2] = (m & 4u) ? T(0) : Uf[d + 2] + w * s2;
 }
 
-void launch_interp(hipStream_t s, const Grid &gf, const double *Uc, double *Uf, const uint8_t *maskf, int zero_fixed,
+template <class T>
+void launch_interp(hipStream_t s, const Grid &gf, const T *Uc, T *Uf, const uint8_t *maskf, int zero_fixed,
                    const int *ctl)
 {
-    k_interp<<<(unsigned)((gf.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gf, Uc, Uf, maskf, zero_fixed, ctl);
+    k_interp<T><<<(unsigned)((gf.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gf, Uc, Uf, maskf, zero_fixed, ctl);
 }
+template void launch_interp<double>(hipStream_t, const Grid &, const double *, double *, const uint8_t *, int,
+                                    const int *);
+template void launch_interp<float>(hipStream_t, const Grid &, const float *, float *, const uint8_t *, int,
+                                   const int *);
 
 // ---------------------------------------------------------------------------
 // BLAS-1 and reductions.  Dot products are two-stage with a fixed grid of
 // kRedBlocks blocks and a fixed per-thread stride, so the summation order
-// (and the result) does not depend on timing.
+// (and the result) does not depend on timing.  Products and sums are FP64
+// for both vector types.
 // ---------------------------------------------------------------------------
-__global__ void k_fill(double *v, long n, double val)
+static unsigned grid_for(long n)
+{
+    long nb = (n + kBlock - 1) / kBlock;
+    if (nb > 4096) nb = 4096;
+    if (nb < 1) nb = 1;
+    return (unsigned)nb;
+}
+
+template <class T>
+__global__ void k_fill(T *v, long n, T val)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[i] = val;
 }
 
-void launch_fill(hipStream_t s, double *v, long n, double val)
+template <class T>
+void launch_fill(hipStream_t s, T *v, long n, double val)
 {
-    long nb = (n + kBlock - 1) / kBlock;
-    if (nb > 4096) nb = 4096;
-    if (nb < 1) nb = 1;
-    k_fill<<<(unsigned)nb, kBlock, 0, s>>>(v, n, val);
+    k_fill<T><<<grid_for(n), kBlock, 0, s>>>(v, n, T(val));
 }
+template void launch_fill<double>(hipStream_t, double *, long, double);
+template void launch_fill<float>(hipStream_t, float *, long, double);
 
-__global__ void k_copy(const double *__restrict__ src, double *__restrict__ dst, long n, const int *ctl)
+template <class Ti, class To>
+__global__ void k_copy(const Ti *__restrict__ src, To *__restrict__ dst, long n, const int *ctl)
 {
     if (ctl && ctl[0]) return;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = src[i];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = (To)src[i];
 }
 
-void launch_copy(hipStream_t s, const double *src, double *dst, long n, const int *ctl)
+template <class Ti, class To>
+void launch_copy(hipStream_t s, const Ti *src, To *dst, long n, const int *ctl)
 {
-    long nb = (n + kBlock - 1) / kBlock;
-    if (nb > 4096) nb = 4096;
-    if (nb < 1) nb = 1;
-    k_copy<<<(unsigned)nb, kBlock, 0, s>>>(src, dst, n, ctl);
+    k_copy<Ti, To><<<grid_for(n), kBlock, 0, s>>>(src, dst, n, ctl);
 }
+template void launch_copy<double, double>(hipStream_t, const double *, double *, long, const int *);
+template void launch_copy<float, float>(hipStream_t, const float *, float *, long, const int *);
+template void launch_copy<double, float>(hipStream_t, const double *, float *, long, const int *);
+template void launch_copy<float, double>(hipStream_t, const float *, double *, long, const int *);
 
-__global__ __launch_bounds__(kBlock) void k_dot_partial(const double *__restrict__ a, const double *__restrict__ b,
-                                                       long n, double *partials, const int *ctl)
+template <class Ta, class Tb>
+__global__ __launch_bounds__(kBlock) void k_dot_partial(const Ta *__restrict__ a, const Tb *__restrict__ b, long n,
+                                                       double *partials, const int *ctl)
 {
     if (ctl && ctl[0]) return;
     double s = 0.0;
-    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) s += a[i] * b[i];
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock)
+        s += (double)a[i] * (double)b[i];
     s = block_sum(s);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-void launch_dot_partial(hipStream_t s, const double *a, const double *b, long n, double *partials, const int *ctl)
+template <class Ta, class Tb>
+void launch_dot_partial(hipStream_t s, const Ta *a, const Tb *b, long n, double *partials, const int *ctl)
 {
-    k_dot_partial<<<kRedBlocks, kBlock, 0, s>>>(a, b, n, partials, ctl);
+    k_dot_partial<Ta, Tb><<<kRedBlocks, kBlock, 0, s>>>(a, b, n, partials, ctl);
 }
+template void launch_dot_partial<double, double>(hipStream_t, const double *, const double *, long, double *,
+                                                 const int *);
+template void launch_dot_partial<float, float>(hipStream_t, const float *, const float *, long, double *, const int *);
 
 __device__ double sum_partials(const double *p, int n)
 {
@@ -478,18 +524,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars *sc, const double
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_update_ur(double *__restrict__ U, double *__restrict__ R,
-                                                        const double *__restrict__ P, const double *__restrict__ Q,
-                                                        long n, const CgScalars *sc, double *partials)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ P,
+                                                        const T *__restrict__ Q, long n, const CgScalars *sc,
+                                                        double *partials)
 {
     if (sc->done) return;
-    const double al = sc->alpha;
+    const T al = (T)sc->alpha;
     double s = 0.0;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) {
         U[i] += al * P[i];
-        const double r = R[i] - al * Q[i];
+        const T r = R[i] - al * Q[i];
         R[i] = r;
-        s += r * r;
+        s += (double)r * (double)r;
     }
     s = block_sum(s);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -529,11 +576,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(CgScalars *sc, const double 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_update_p(double *__restrict__ P, const double *__restrict__ Z, long n,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ P, const T *__restrict__ Z, long n,
                                                        const CgScalars *sc)
 {
     if (sc->done) return;
-    const double be = sc->beta;
+    const T be = (T)sc->beta;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) P[i] = Z[i] + be * P[i];
 }
 
@@ -549,11 +597,15 @@ void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int n
 {
     k_cg_alpha<<<1, kBlock, 0, s>>>(sc, partials, nparts);
 }
-void launch_cg_update_ur(hipStream_t s, double *U, double *R, const double *P, const double *Q, long n, CgScalars *sc,
-                         double *partials)
+template <class T>
+void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials)
 {
-    k_cg_update_ur<<<kRedBlocks, kBlock, 0, s>>>(U, R, P, Q, n, sc, partials);
+    k_cg_update_ur<T><<<kRedBlocks, kBlock, 0, s>>>(U, R, P, Q, n, sc, partials);
 }
+template void launch_cg_update_ur<double>(hipStream_t, double *, double *, const double *, const double *, long,
+                                          CgScalars *, double *);
+template void launch_cg_update_ur<float>(hipStream_t, float *, float *, const float *, const float *, long,
+                                         CgScalars *, double *);
 void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit)
 {
     k_cg_check<<<1, kBlock, 0, s>>>(sc, partials, nparts, tol, maxit);
@@ -562,11 +614,12 @@ void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int np
 {
     k_cg_beta<<<1, kBlock, 0, s>>>(sc, partials, nparts, maxit);
 }
-void launch_cg_update_p(hipStream_t s, double *P, const double *Z, long n, CgScalars *sc)
+template <class T>
+void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc)
 {
-    long nb = (n + kBlock - 1) / kBlock;
-    if (nb > 4096) nb = 4096;
-    k_cg_update_p<<<(unsigned)nb, kBlock, 0, s>>>(P, Z, n, sc);
+    k_cg_update_p<T><<<grid_for(n), kBlock, 0, s>>>(P, Z, n, sc);
 }
+template void launch_cg_update_p<double>(hipStream_t, double *, const double *, long, CgScalars *);
+template void launch_cg_update_p<float>(hipStream_t, float *, const float *, long, CgScalars *);
 
 }  // namespace gputo

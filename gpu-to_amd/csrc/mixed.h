@@ -1,15 +1,42 @@
-// Mixed-precision solve (FP64 iterative refinement around an FP32 / BF16
-// inner PCG) and the learned coarse correction.
+// Mixed-precision solve (FP64 iterative refinement around an FP32 inner PCG
+// with an FP32 node-gather or split-BF16 MFMA level-0 operator).
 #pragma once
 
+#include <vector>
+
 #include "gputo_internal.h"
+#include "vcycle.h"
 
 namespace gputo {
 
+struct MfmaState;
+
+struct MixedState {
+    HierView<float> H;                 // FP32 hierarchy view
+    float *A32[kMaxLevels] = {};       // FP32 packed element matrices
+    float *Ke32 = nullptr, *CX32 = nullptr;
+    float *X = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr, *Z = nullptr;  // inner CG vectors
+    CgScalars *sc = nullptr, *sc_host = nullptr;
+    double *norm = nullptr;            // ||r|| of the outer residual (device)
+    double norm_host[2] = {0, 0};
+    MfmaState *mf = nullptr;
+    int last_inner = 0, last_outer = 0, last_learned_ok = 0, last_learned_bad = 0;
+    long long bytes = 0;
+    std::vector<void *> allocs;
+};
+
 void mixed_create(gputo_handle *h);
 void mixed_destroy(gputo_handle *h);
-int mixed_bench_operator(gputo_handle *h, int precision, int reps, double *avg_ms);
 void mixed_stats(gputo_handle *h, gputo_to_stats *st);
 int mixed_solve(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit);
+int mixed_apply_l0(gputo_handle *h, int precision, const double *u, double *y);
+int mixed_bench_operator(gputo_handle *h, int precision, int reps, double *avg_ms);
+
+// split-BF16 MFMA level-0 operator (mfma.hip)
+void mfma_create(gputo_handle *h, MixedState *m);
+void mfma_destroy(gputo_handle *h, MixedState *m);
+void mfma_prepare(gputo_handle *h, MixedState *m);  // Ke splits, E in FP32
+int mfma_blocks(const gputo_handle *h);             // blocks (= partials) of one launch
+long mfma_tile_blocks(const Grid &g0);              // the same, before the state exists
 
 }  // namespace gputo

@@ -8,6 +8,7 @@
 
 #include "gputo_internal.h"
 #include "mixed.h"
+#include "vcycle.h"
 
 namespace gputo {
 
@@ -168,18 +169,6 @@ T *dalloc(gputo_handle *h, size_t n)
     return static_cast<T *>(p);
 }
 
-struct Preset {
-    int npre, npost;
-    double omega, prerelax;
-    int l1_galerkin, zero_after_interp, corr_mirrored;
-};
-
-Preset preset_of(int p)
-{
-    if (p == GPUTO_PRESET_MATLAB_CFG0) return {0, 1, 0.6, 0.6, 1, 1, 0};
-    return {3, 3, 0.48, 0.7, 0, 1, 1};
-}
-
 int map_exception()
 {
     try {
@@ -195,90 +184,26 @@ int map_exception()
     }
 }
 
-OpArgs op_args(const gputo_handle *h, int l)
+HierView<double> f64_view(const gputo_handle *h)
 {
-    OpArgs a{};
-    a.g = h->L[l].g;
-    a.t = h->L[l].t;
-    a.Ke = h->Ke;
-    a.CX = h->CX;
-    a.A = h->L[l].A;
-    a.mask = h->L[l].mask;
-    a.AD = h->L[l].AD;
-    a.ctl = &h->sc->done;
-    const Preset p = preset_of(h->prm.preset);
-    a.omega = p.omega;
-    a.prerelax = p.prerelax;
-    return a;
+    HierView<double> H;
+    for (int l = 0; l < h->nl; ++l) {
+        const Level &v = h->L[l];
+        H.L[l].U = v.U;
+        H.L[l].F = v.F;
+        H.L[l].R = v.R;
+        H.L[l].AD = v.AD;
+        H.L[l].Tmp = v.T;
+        H.L[l].A = v.A;
+    }
+    H.Ke = h->Ke;
+    H.CX = h->CX;
+    return H;
 }
 
-void launch_level_op(const gputo_handle *h, int l, int mode, const OpArgs &a)
-{
-    if (l == 0)
-        launch_op0(h->stream, mode, a);
-    else
-        launch_opA(h->stream, mode, a);
-}
+OpArgs op_args(const gputo_handle *h, int l) { return view_args(h, f64_view(h), l, &h->sc->done); }
 
-// n damped-Jacobi sweeps on level l ending in U; `pre` forms the first
-// sweep's input as prerelax*F/AD (preRelc, mgcg1.cu:1190-1197).
-void sweeps(gputo_handle *h, int l, const double *F, double *U, int n, bool pre)
-{
-    Level &v = h->L[l];
-    const double *src = U;
-    for (int s = 0; s < n; ++s) {
-        double *dst = ((n - 1 - s) % 2 == 0) ? U : v.T;
-        if (!(pre && s == 0) && dst == src) {
-            launch_copy(h->stream, U, v.T, v.g.ndof, &h->sc->done);
-            src = v.T;
-        }
-        OpArgs a = op_args(h, l);
-        a.U = src;
-        a.F = F;
-        a.out = dst;
-        launch_level_op(h, l, (pre && s == 0) ? OP_PRE_JACOBI : OP_JACOBI, a);
-        src = dst;
-    }
-}
-
-// vcycle (mgcg1.cu:468-651; mgcg1.c:283-447): U0 = V(F0).
-void vcycle(gputo_handle *h, const double *F0, double *U0)
-{
-    const Preset p = preset_of(h->prm.preset);
-    const int nl = h->nl;
-    const int *ctl = &h->sc->done;
-    for (int l = 0; l < nl; ++l) {
-        Level &v = h->L[l];
-        const double *F = l == 0 ? F0 : v.F;
-        double *U = l == 0 ? U0 : v.U;
-        if (l > 0) launch_restrict(h->stream, v.g, h->L[l - 1].R, v.F, v.mask, ctl);
-        if (l < nl - 1) {
-            if (p.npre == 0) {
-                OpArgs a = op_args(h, l);
-                a.F = F;
-                a.out = v.R;
-                a.out2 = U;
-                launch_level_op(h, l, OP_PRE_DEFECT, a);
-            } else {
-                sweeps(h, l, F, U, p.npre, true);
-                OpArgs a = op_args(h, l);
-                a.U = U;
-                a.F = F;
-                a.out = v.R;
-                launch_level_op(h, l, OP_DEFECT, a);
-            }
-        } else {
-            launch_coarse_solve(h->stream, h->C, F, U, v.mask, ctl);
-        }
-    }
-    for (int l = nl - 2; l >= 0; --l) {
-        Level &v = h->L[l];
-        const double *F = l == 0 ? F0 : v.F;
-        double *U = l == 0 ? U0 : v.U;
-        launch_interp(h->stream, v.g, h->L[l + 1].U, U, v.mask, p.zero_after_interp, ctl);
-        sweeps(h, l, F, U, p.npost, false);
-    }
-}
+void vcycle(gputo_handle *h, const double *F0, double *U0) { vcycle_t(h, f64_view(h), F0, U0, &h->sc->done); }
 
 int validate_fixed(const std::vector<int> &f, long ndof)
 {
@@ -406,6 +331,8 @@ void gputo_default_params(gputo_params *p)
 
 const char *gputo_version(void) { return "gputo-mi355x 0.1 (gfx950)"; }
 
+int gputo_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : GPUTO_ERR_DEVICE; }
+
 long long gputo_est_workspace(int nfd, int nl, int nelx, int nely, int nelz)
 {
     // estWorkSpace (mgcg1.cu:293-322), in doubles
@@ -459,6 +386,7 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
             }
         }
         const Grid &g0 = h->L[0].g;
+        maxblocks = (int)std::max<long>(maxblocks, mfma_tile_blocks(g0));  // partials of the MFMA level-0 kernel
         h->Ke = dalloc<double>(h, 576);
         h->Aelm = dalloc<double>(h, 2400);
         h->CX = dalloc<double>(h, g0.nel);
@@ -483,7 +411,7 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
         C.flag = dalloc<int>(h, 1);
         std::vector<double> tab = level1_gauss_table(0.3, 2.0);
         HIPCHK(hipMemcpy(h->Aelm, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-        mixed_create(h);
+        if (h->prm.precision != GPUTO_PREC_F64) mixed_create(h);
     } catch (...) {
         int rc = map_exception();
         gputo_destroy(h);
@@ -646,6 +574,30 @@ int gputo_matvec_f64(gputo_handle *h, const double *Ke0, const double *CX, const
                      double *y)
 {
     return host_op(h, 0, Ke0, CX, nullptr, u, fixed, nfd, y);
+}
+
+int gputo_matvec_mixed(gputo_handle *h, int precision, const double *Ke0, const double *CX, const double *u,
+                       const double *fixed, int nfd, double *y)
+{
+    if (!h || !u || !y || (precision != GPUTO_PREC_MIXED_F32 && precision != GPUTO_PREC_MIXED_BF16))
+        return GPUTO_ERR_ARGUMENT;
+    try {
+        const Grid &g0 = h->L[0].g;
+        std::vector<int> fx;
+        if (int rc = upload_problem(h, Ke0, CX, fixed, nfd, fx)) return rc;
+        if (int rc = validate_fixed(fx, g0.ndof)) return rc;
+        std::vector<uint8_t> m = node_mask(fx, g0);
+        hipStream_t s = h->stream;
+        HIPCHK(hipMemcpyAsync(h->L[0].mask, m.data(), m.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h->P, u, g0.ndof * sizeof(double), hipMemcpyHostToDevice, s));
+        mixed_apply_l0(h, precision, h->P, h->Q);
+        HIPCHK(hipMemcpyAsync(y, h->Q, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        h->hier_ready = false;
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
 }
 
 int gputo_defect_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *F, const double *u,

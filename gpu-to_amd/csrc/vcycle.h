@@ -1,0 +1,142 @@
+// The multigrid V-cycle, written once over a typed view of the hierarchy so
+// the FP64 solve and the FP32 inner solve of the mixed-precision path share
+// it (vcycle, Cantilever/TOP3D125MGCGCUDA/mgcg1.cu:468-651;
+// Cantilever/omp/mgcg1.c:283-447).
+#pragma once
+
+#include "gputo_internal.h"
+
+namespace gputo {
+
+// Smoother / hierarchy constants of a preset.
+struct Preset {
+    int npre, npost;
+    double omega, prerelax;
+    int l1_galerkin, zero_after_interp, corr_mirrored;
+};
+
+inline Preset preset_of(int p)
+{
+    // REF_C: mgcg1.c:110,308,353,363-364, mgcg1.cu:188,631,1196,1205;
+    // MATLAB_CFG0: top3D125mgcg.m:147,194,243 (omega 0.6, one sweep from zero)
+    if (p == GPUTO_PRESET_MATLAB_CFG0) return {0, 1, 0.6, 0.6, 1, 1, 0};
+    return {3, 3, 0.48, 0.7, 0, 1, 1};
+}
+
+// Level-0 operator implementation of a view.
+enum L0Kind { L0_GATHER = 0, L0_MFMA_BF16 = 1 };
+
+template <class T>
+struct LevelView {
+    T *U = nullptr, *F = nullptr, *R = nullptr, *AD = nullptr, *Tmp = nullptr;
+    const T *A = nullptr;  // packed element matrices (levels >= 1)
+};
+
+template <class T>
+struct HierView {
+    LevelView<T> L[kMaxLevels];
+    const T *Ke = nullptr;   // level-0 element matrix (24x24 row-major)
+    const T *CX = nullptr;   // level-0 element scales
+    int l0 = L0_GATHER;
+    const void *l0_ctx = nullptr;  // state of a non-gather level-0 kernel
+};
+
+// level-0 BF16-MFMA operator (mfma.hip); ctx is the MixedState
+void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const void *ctx);
+
+template <class T>
+OpArgsT<T> view_args(const gputo_handle *h, const HierView<T> &H, int l, const int *ctl)
+{
+    OpArgsT<T> a{};
+    a.g = h->L[l].g;
+    a.t = h->L[l].t;
+    a.Ke = H.Ke;
+    a.CX = H.CX;
+    a.A = H.L[l].A;
+    a.mask = h->L[l].mask;
+    a.AD = H.L[l].AD;
+    a.ctl = ctl;
+    const Preset p = preset_of(h->prm.preset);
+    a.omega = p.omega;
+    a.prerelax = p.prerelax;
+    return a;
+}
+
+template <class T>
+void level_op(const gputo_handle *h, const HierView<T> &H, int l, int mode, const OpArgsT<T> &a)
+{
+    if (l > 0) {
+        launch_opA<T>(h->stream, mode, a);
+    } else if constexpr (std::is_same<T, float>::value) {
+        if (H.l0 == L0_MFMA_BF16)
+            launch_op0_mfma(h->stream, mode, a, H.l0_ctx);
+        else
+            launch_op0<T>(h->stream, mode, a);
+    } else {
+        launch_op0<T>(h->stream, mode, a);
+    }
+}
+
+// n damped-Jacobi sweeps on level l ending in U; `pre` forms the first
+// sweep's input as prerelax*F/AD (preRelc, mgcg1.cu:1190-1197).
+template <class T>
+void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T *U, int n, bool pre, const int *ctl)
+{
+    const LevelView<T> &v = H.L[l];
+    const T *src = U;
+    for (int s = 0; s < n; ++s) {
+        T *dst = ((n - 1 - s) % 2 == 0) ? U : v.Tmp;
+        if (!(pre && s == 0) && dst == src) {
+            launch_copy<T, T>(h->stream, U, v.Tmp, h->L[l].g.ndof, ctl);
+            src = v.Tmp;
+        }
+        OpArgsT<T> a = view_args(h, H, l, ctl);
+        a.U = src;
+        a.F = F;
+        a.out = dst;
+        level_op(h, H, l, (pre && s == 0) ? OP_PRE_JACOBI : OP_JACOBI, a);
+        src = dst;
+    }
+}
+
+// U0 = V(F0).
+template <class T>
+void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, const int *ctl)
+{
+    const Preset p = preset_of(h->prm.preset);
+    const int nl = h->nl;
+    for (int l = 0; l < nl; ++l) {
+        const LevelView<T> &v = H.L[l];
+        const Level &lv = h->L[l];
+        const T *F = l == 0 ? F0 : v.F;
+        T *U = l == 0 ? U0 : v.U;
+        if (l > 0) launch_restrict<T>(h->stream, lv.g, H.L[l - 1].R, v.F, lv.mask, ctl);
+        if (l < nl - 1) {
+            if (p.npre == 0) {
+                OpArgsT<T> a = view_args(h, H, l, ctl);
+                a.F = F;
+                a.out = v.R;
+                a.out2 = U;
+                level_op(h, H, l, OP_PRE_DEFECT, a);
+            } else {
+                sweeps_t(h, H, l, F, U, p.npre, true, ctl);
+                OpArgsT<T> a = view_args(h, H, l, ctl);
+                a.U = U;
+                a.F = F;
+                a.out = v.R;
+                level_op(h, H, l, OP_DEFECT, a);
+            }
+        } else {
+            launch_coarse_solve<T>(h->stream, h->C, F, U, lv.mask, ctl);
+        }
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        const LevelView<T> &v = H.L[l];
+        const T *F = l == 0 ? F0 : v.F;
+        T *U = l == 0 ? U0 : v.U;
+        launch_interp<T>(h->stream, h->L[l].g, H.L[l + 1].U, U, h->L[l].mask, p.zero_after_interp, ctl);
+        sweeps_t(h, H, l, F, U, p.npost, false, ctl);
+    }
+}
+
+}  // namespace gputo

@@ -91,6 +91,8 @@ SIGNATURES = {
     "gputo_est_workspace": (ctypes.c_longlong, [_I, _I, _I, _I, _I]),
     "gputo_device_bytes": (ctypes.c_longlong, [_P]),
     "gputo_matvec_f64": (_I, [_P, _P, _P, _P, _P, _I, _P]),
+    "gputo_matvec_mixed": (_I, [_P, _I, _P, _P, _P, _P, _I, _P]),
+    "gputo_selftest_mfma": (_I, [_pD]),
     "gputo_defect_f64": (_I, [_P, _P, _P, _P, _P, _P, _I, _P]),
     "gputo_diag_f64": (_I, [_P, _P, _P, _P]),
     "gputo_vcycle_f64": (_I, [_P, _P, _P, _P, _I, _P, _P]),
@@ -110,6 +112,7 @@ SIGNATURES = {
     "gputo_filter_f64": (_I, [_P, _P, _D, _P]),
     "gputo_sens_f64": (_I, [_P, _P, _P, _P]),
     "gputo_version": (ctypes.c_char_p, []),
+    "gputo_device_synchronize": (_I, []),
 }
 
 _LIB = None

@@ -81,6 +81,13 @@ class Solver:
                                         ptr(y)), "gputo_matvec_f64")
         return y
 
+    def matvec_mixed(self, precision, Ke0, CX, u, fixed):
+        y = np.zeros(self.ndof)
+        fixed = f64(fixed).ravel()
+        check(self.lib.gputo_matvec_mixed(self.h, precision, ptr(f64(Ke0)), ptr(f64(CX)), ptr(f64(u)), ptr(fixed),
+                                          len(fixed), ptr(y)), "gputo_matvec_mixed")
+        return y
+
     def defect(self, Ke0, CX, F, u, fixed):
         r = np.zeros(self.ndof)
         fixed = f64(fixed).ravel()

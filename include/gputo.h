@@ -104,6 +104,14 @@ long long gputo_device_bytes(const gputo_handle *h);
 /* y = K(CX) u with y[fixed] = 0 (multAV0c + ZeroBCc, mgcg1.cu:941-997,1326). */
 int gputo_matvec_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *u,
                      const double *fixed, int nfd, double *y);
+/* The same product with the reduced-precision level-0 operator of the
+ * mixed-precision solve (precision GPUTO_PREC_MIXED_F32: FP32 node-gather;
+ * GPUTO_PREC_MIXED_BF16: split-BF16 MFMA), FP64 in and out. */
+int gputo_matvec_mixed(gputo_handle *h, int precision, const double *Ke0, const double *CX, const double *u,
+                       const double *fixed, int nfd, double *y);
+/* Lane-layout check of the BF16 MFMA instruction the level-0 kernel uses:
+ * *max_err = max |D - A B| on a small exact integer product (0 expected). */
+int gputo_selftest_mfma(double *max_err);
 /* r = F - K(CX) u, r[fixed] = 0 (defect0c + ZeroBCc, mgcg1.cu:1208-1264). */
 int gputo_defect_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *F,
                      const double *u, const double *fixed, int nfd, double *r);
@@ -174,6 +182,8 @@ int gputo_sens_f64(gputo_handle *h, const double *Ke0, const double *U_cframe, d
 
 /* Library version string. */
 const char *gputo_version(void);
+/* Wait for all work on the current device (benchmark timing brackets). */
+int gputo_device_synchronize(void);
 
 #ifdef __cplusplus
 }

@@ -38,8 +38,10 @@ def cantilever_case(nelx, nely, nelz, seed=0, kind="perturbed"):
         x = np.full(nel, 0.12)
     elif kind == "binary":  # disconnected stiff islands in near-void: CG stagnates
         x = np.where(rng.uniform(size=nel) < 0.3, 1.0, 1e-3)
-    elif kind == "contrast":  # two-phase field, contrast 1e3 in E
-        x = np.where(rng.uniform(size=nel) < 0.3, 1.0, 0.1)
+    elif kind == "smooth":  # smoothed random field in [0.05, 1]: E contrast 8e3
+        import scipy.ndimage as ndi
+        r = ndi.uniform_filter(rng.uniform(size=(nelx, nelz, nely)), size=3, mode="reflect")
+        x = (0.05 + 0.95 * (r - r.min()) / (r.max() - r.min())).ravel()
     else:
         x = np.clip(0.12 + rng.uniform(-0.1, 0.1, nel), 0.01, 1.0)
     CX = 1e-9 + x ** 3 * (1 - 1e-9)

@@ -1,0 +1,65 @@
+"""Mixed-precision path: FP32 / split-BF16 MFMA level-0 operator and the FP64
+iterative-refinement solve, checked against the FP64 oracle.
+
+Tolerances: the FP32 node-gather product agrees with the FP64 oracle to
+1e-6 of max|y| (FP32 rounding of Ke, E, u and 192 accumulations); the
+split-BF16 product (hi*hi + hi*lo + lo*hi, the lo*lo term dropped) to 1e-4.
+The refined solve reaches the FP64 residual tolerance, so its displacements
+and compliance agree with the plain FP64 solve to the tolerance.
+"""
+import numpy as np
+import pytest
+
+from conftest import cantilever_case
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_mfma_lane_layout(gputo):
+    import ctypes
+    err = ctypes.c_double(-1.0)
+    assert gputo.load().gputo_selftest_mfma(ctypes.byref(err)) == 0
+    assert err.value == 0.0
+
+
+@pytest.mark.parametrize("mesh", [(8, 4, 4, 3), (16, 8, 8, 3), (24, 12, 12, 3), (48, 24, 24, 4)])
+@pytest.mark.parametrize("prec,tol", [(1, 1e-6), (2, 1e-4)])
+def test_reduced_precision_operator(gputo, oracle, mesh, prec, tol):
+    nelx, nely, nelz, nl = mesh
+    print("case", mesh, prec, flush=True)
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=21, kind="smooth")
+    s = gputo.Solver(nelx, nely, nelz, nl)
+    u = np.random.default_rng(22).uniform(-1, 1, s.ndof)
+    y = s.matvec_mixed(prec, Ke, CX, u, fixed)
+    yo = oracle.matvec0(Ke, CX, u, fixed, nelx, nely, nelz)
+    assert rel(y, yo) < tol
+    assert np.array_equal(y, s.matvec_mixed(prec, Ke, CX, u, fixed))  # deterministic
+    s.close()
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("mesh", [(16, 8, 8, 3), (48, 24, 24, 4)])
+def test_mixed_solve_matches_f64(gputo, oracle, prec, mesh):
+    nelx, nely, nelz, nl = mesh
+    print("case", mesh, prec, flush=True)
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=23, kind="perturbed")
+    s64 = gputo.Solver(nelx, nely, nelz, nl)
+    U64, r64, n64, rc = s64.solve(Ke, F, CX, fixed, 1e-10, 300)
+    assert rc == 0
+    sm = gputo.Solver(nelx, nely, nelz, nl, precision=prec, inner_tol=1e-4)
+    Um, rm, nm, rc = sm.solve(Ke, F, CX, fixed, 1e-10, 300)
+    assert rc == 0 and rm < 1e-10
+    # the outer residual is the true FP64 residual: check it with the oracle
+    Uc = Um.copy()
+    Uc[1::3] *= -1
+    r = oracle.defect0(Ke, F, Uc, CX, fixed, nelx, nely, nelz)
+    assert np.linalg.norm(r) / np.linalg.norm(F) < 1e-10
+    assert rel(Um, U64) < 1e-7
+    C64, Cm = F @ U64, F @ Um
+    assert abs(Cm - C64) <= 1e-9 * abs(C64)
+    s64.close()
+    sm.close()

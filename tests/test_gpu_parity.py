@@ -136,7 +136,7 @@ def test_vcycle(solvers, oracle, preset):
 
 
 @pytest.mark.parametrize("mesh", [(8, 4, 4, 3), (16, 8, 8, 3), (24, 12, 12, 3), (32, 16, 16, 4)])
-@pytest.mark.parametrize("kind", ["perturbed", "contrast"])
+@pytest.mark.parametrize("kind", ["perturbed", "smooth"])
 def test_mgcg1_vs_oracle(solvers, oracle, mesh, kind):
     nelx, nely, nelz, nl = mesh
     print("case", mesh, kind, flush=True)
@@ -156,8 +156,10 @@ def test_mgcg1_vs_oracle(solvers, oracle, mesh, kind):
 
 
 def test_mgcg1_stagnation_matches_oracle(solvers, oracle):
-    """Random stiff islands in near-void (E ratio 1e-9): neither side converges;
-    the CG trajectories still agree after maxit iterations."""
+    """Random stiff islands in near-void (E ratio 1e-9): the reference
+    algorithm stagnates (its damped Jacobi factor 0.48 exceeds 2/lambda_max
+    of D^-1 K on such fields); the CG trajectories still agree after maxit
+    iterations."""
     nelx, nely, nelz, nl = 8, 4, 4, 3
     Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=12, kind="binary")
     s = solvers(nelx, nely, nelz, nl)
