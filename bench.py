@@ -20,7 +20,15 @@ sys.path[:0] = [os.path.join(ROOT, "gpu-to_amd"), os.path.join(ROOT, "oracle")]
 METRIC = "wall-s/TO iter @1.77M hex, mixed+learned vs FP64 MGCG; matvec HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-VARIANT_PREC = {"f64": 0, "mixed_f32": 1, "mixed_bf16": 2}
+# variant -> (precision, level-0 kernel of the inner solve); kernel ids as
+# gputo_params.l0_kernel (0 = FP64 node-gather for the plain path)
+VARIANTS = {
+    "f64": (0, 0),
+    "mixed_gather": (1, 1),  # FP32 inner solve, FP32 node-gather level-0 operator
+    "mixed_f32": (1, 2),     # FP32 inner solve, FP32 MFMA level-0 operator
+    "mixed_bf16x2": (2, 3),  # split-BF16 MFMA, hi+lo
+    "mixed_bf16": (2, 4),    # split-BF16 MFMA, hi+mid+lo
+}
 
 
 def log(*a):
@@ -28,10 +36,10 @@ def log(*a):
 
 
 def run_variant(gputo, name, args, Ke, F, fixed, x0):
-    prec = VARIANT_PREC[name.replace("+learned", "")]
+    prec, kernel = VARIANTS[name.replace("+learned", "")]
     learned = name.endswith("+learned")
     s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=learned,
-                     inner_tol=args.inner_tol)
+                     inner_tol=args.inner_tol, l0_kernel=kernel)
     run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
                        cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
     for w in range(args.warmup):
@@ -58,7 +66,7 @@ def run_variant(gputo, name, args, Ke, F, fixed, x0):
     # dominant kernel: the level-0 operator, timed with HIP events on the
     # solver's stream on the hierarchy of the last timed solve
     avg = ctypes.c_double()
-    rc = s.lib.gputo_bench_operator(s.h, prec, args.op_reps, ctypes.byref(avg))
+    rc = s.lib.gputo_bench_operator(s.h, kernel, args.op_reps, ctypes.byref(avg))
     out["op_ms"] = avg.value if rc == 0 else None
     run.close()
     s.close()
@@ -105,7 +113,7 @@ def main():
     ap.add_argument("--nl", type=int, default=5)
     ap.add_argument("--cg-tol", type=float, default=1e-8)
     ap.add_argument("--cg-maxit", type=int, default=300)
-    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--inner-tol", type=float, default=1e-6)
     ap.add_argument("--variants", default="f64")
     ap.add_argument("--op-reps", type=int, default=50)
     ap.add_argument("--seed", type=int, default=0)

@@ -78,6 +78,7 @@ struct CgScalars {
 struct Level {
     Grid g;
     Tile t;
+    Tile tmf;                     // tiling of the matrix-free level-1 operator
     double *U = nullptr, *F = nullptr, *R = nullptr, *AD = nullptr, *T = nullptr;
     double *A = nullptr;          // packed element matrices (levels >= 1)
     uint8_t *mask = nullptr;      // fixed-DOF node mask

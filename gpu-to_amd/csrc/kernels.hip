@@ -60,6 +60,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
     if (a.ctl && a.ctl[0]) return;
     __shared__ T su[3][kMaxHalo];
     __shared__ T sE[STORED ? 1 : kMaxElemBox];
+    __shared__ T sK[STORED ? 1 : 576];  // Ke, read with wave-uniform (broadcast) LDS loads
+    if (!STORED)
+        for (int q = threadIdx.x; q < 576; q += kBlock) sK[q] = a.Ke[q];
     const Grid g = a.g;
     const Tile t = a.t;
     int b = blockIdx.x;
@@ -114,49 +117,38 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
             const int hb = (ti * HK + tk) * HJ + tj;  // halo index of neighbour offset (0,0,0)
             T y0 = T(0), y1 = T(0), y2 = T(0);
             if (!STORED) {
-                const T *__restrict__ Ke = a.Ke;
-                T z[8][3];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) z[e][0] = z[e][1] = z[e][2] = T(0);
-#pragma unroll
-                for (int oi = 0; oi < 3; ++oi)
-#pragma unroll
-                    for (int ok = 0; ok < 3; ++ok)
-#pragma unroll
-                        for (int oj = 0; oj < 3; ++oj) {
-                            const int h = hb + (oi * HK + ok) * HJ + oj;
-                            const T u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
-#pragma unroll
-                            for (int ei = 0; ei < 2; ++ei)
-#pragma unroll
-                                for (int ek = 0; ek < 2; ++ek)
-#pragma unroll
-                                    for (int ej = 0; ej < 2; ++ej) {
-                                        const int bx = oi - ei, by = oj - ej, bz = ok - ek;
-                                        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
-                                        const int la = lidx(1 - ei, 1 - ej, 1 - ek);
-                                        const int lb = lidx(bx, by, bz);
-                                        const int e = 4 * ei + 2 * ek + ej;
-#pragma unroll
-                                        for (int c = 0; c < 3; ++c) {
-                                            const T *kr = Ke + 24 * (3 * la + c) + 3 * lb;
-                                            z[e][c] += kr[0] * u0 + kr[1] * u1 + kr[2] * u2;
-                                        }
-                                    }
-                        }
+                // one incident element at a time (runtime loop keeps the live
+                // set small): t = Ke[3a..3a+2, :] u_e, y += E_e t, with the
+                // node's local index a and the Ke rows uniform across the wave
                 const int eb = (ti * EK + tk) * EJ + tj;
+#pragma unroll 1
+                for (int e = 0; e < 8; ++e) {
+                    const int ei = e >> 2, ek = (e >> 1) & 1, ej = e & 1;
+                    const T E = sE[eb + (ei * EK + ek) * EJ + ej];
+                    const T *kr = sK + 72 * (4 * (1 - ek) + ((1 - ej) ? ((1 - ei) ? 2 : 3) : ((1 - ei) ? 1 : 0)));
+                    const int he = hb + (ei * HK + ek) * HJ + ej;
+                    T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
-                for (int ei = 0; ei < 2; ++ei)
-#pragma unroll
-                    for (int ek = 0; ek < 2; ++ek)
-#pragma unroll
-                        for (int ej = 0; ej < 2; ++ej) {
-                            const int e = 4 * ei + 2 * ek + ej;
-                            const T E = sE[eb + (ei * EK + ek) * EJ + ej];
-                            y0 += E * z[e][0];
-                            y1 += E * z[e][1];
-                            y2 += E * z[e][2];
-                        }
+                    for (int lb = 0; lb < 8; ++lb) {
+                        const int bx = (lb == 1 || lb == 2 || lb == 5 || lb == 6) ? 1 : 0;
+                        const int by = (lb == 2 || lb == 3 || lb == 6 || lb == 7) ? 1 : 0;
+                        const int bz = lb >= 4 ? 1 : 0;
+                        const int h = he + (bx * HK + bz) * HJ + by;
+                        const T u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
+                        t0 = fma(kr[3 * lb + 0], u0, t0);
+                        t0 = fma(kr[3 * lb + 1], u1, t0);
+                        t0 = fma(kr[3 * lb + 2], u2, t0);
+                        t1 = fma(kr[24 + 3 * lb + 0], u0, t1);
+                        t1 = fma(kr[24 + 3 * lb + 1], u1, t1);
+                        t1 = fma(kr[24 + 3 * lb + 2], u2, t1);
+                        t2 = fma(kr[48 + 3 * lb + 0], u0, t2);
+                        t2 = fma(kr[48 + 3 * lb + 1], u1, t2);
+                        t2 = fma(kr[48 + 3 * lb + 2], u2, t2);
+                    }
+                    y0 = fma(E, t0, y0);
+                    y1 = fma(E, t1, y1);
+                    y2 = fma(E, t2, y2);
+                }
             } else {
                 const long nel = g.nel;
                 const T *__restrict__ A = a.A;

@@ -1,0 +1,298 @@
+// Matrix-free level-1 operator of the REF_C hierarchy.  The reference stores
+// the rediscretised level-1 element matrices A_e = sum_g CX[child_g] Aelm_g
+// (coarseAe2c, Cantilever/TOP3D125MGCGCUDA/mgcg1.cu:2042-2070; 300 doubles
+// per element, 531 MB per apply at 192x96x96) and applies them with defectc
+// (mgcg1.cu:1266-1324).  Here the same operator is applied from the child
+// densities alone, Gauss point by Gauss point:
+//   A_e u = sum_g CX_g |J| B_g^T D B_g u,     h = 2, |J| w_g = 1,
+// with B_g the strain-displacement matrix of the coarse hex at Gauss point
+// g = 4 iP + 2 kP + jP, (s,e,t) = (g_iP, g_jP, g_kP), g_0 = -1/sqrt3,
+// g_1 = +1/sqrt3 (the Gauss-point / child pairing of coarseAe2,
+// Cantilever/omp/mgcg1.c:1539-1584).
+//
+// Eight lanes work on one coarse element, one Gauss point each; their 24
+// nodal contributions are summed by a fixed xor butterfly that leaves each
+// lane the 3 values of one node, which it adds into LDS node accumulators.
+// Elements are visited in 8-colour order (no two elements of a colour share
+// a node), so the accumulation is conflict-free and deterministic.
+#include "vcycle.h"
+
+namespace gputo {
+
+constexpr int kL1Halo = 1024;  // halo nodes per block
+constexpr int kL1Own = 512;    // owned nodes per block
+
+template <class T>
+__device__ __forceinline__ T lane_w(int x, T xi)
+{
+    // linear shape factor in one direction: (1 -/+ xi)/2
+    return x ? T(0.5) * (T(1) + xi) : T(0.5) * (T(1) - xi);
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(256) void k_l1_mf(OpArgsT<T> a, Tile t, const T *__restrict__ CXf, T lam2mu, T lam,
+                                              T mu)
+{
+    if (a.ctl && a.ctl[0]) return;
+    __shared__ T su[3 * kL1Halo];
+    __shared__ T acc[3 * kL1Own];
+    const Grid g = a.g;  // the coarse (level-1) grid
+    int b = blockIdx.x;
+    const int bj = b % t.ntj;
+    b /= t.ntj;
+    const int bk = b % t.ntk;
+    const int bi = b / t.ntk;
+    const int j0 = bj * t.TJ, k0 = bk * t.TK, i0 = bi * t.TI;
+    const int HJ = t.TJ + 2, HK = t.TK + 2, HI = t.TI + 2;
+    const int EJ = t.TJ + 1, EK = t.TK + 1, EI = t.TI + 1;
+    const int nown = t.TJ * t.TK * t.TI;
+
+    const int rowlen = 3 * HJ;
+    for (int q = threadIdx.x; q < rowlen * HK * HI; q += 256) {
+        const int row = q / rowlen;
+        const int col = q - row * rowlen;
+        const int hk = row % HK, hi = row / HK;
+        const int hj = col / 3, c = col - 3 * hj;
+        const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
+        T v = T(0);
+        if (gi >= 0 && gi < g.nnx && gj >= 0 && gj < g.nny && gk >= 0 && gk < g.nnz) {
+            const long dof = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + gj) + c;
+            if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
+                v = T(a.prerelax) * a.F[dof] / a.AD[dof];
+            else
+                v = a.U[dof];
+        }
+        su[3 * ((hi * HK + hk) * HJ + hj) + c] = v;
+    }
+    for (int q = threadIdx.x; q < 3 * nown; q += 256) acc[q] = T(0);
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 3;  // 32 groups of 8 lanes
+    const int gp = lane & 7;                                       // Gauss point of this lane
+    const int is = gp >> 2, it = (gp >> 1) & 1, ie = gp & 1;
+    const T g1 = T(0.57735026918962576451);                        // 1/sqrt(3)
+    const T sx = is ? g1 : -g1, sy = ie ? g1 : -g1, sz = it ? g1 : -g1;
+    const T wx[2] = {lane_w(0, sx), lane_w(1, sx)};
+    const T wy[2] = {lane_w(0, sy), lane_w(1, sy)};
+    const T wz[2] = {lane_w(0, sz), lane_w(1, sz)};
+    const T ih = T(0.5);  // 1/h, h = 2; |J| w_g = h^3/8 = 1
+    const int fny = 2 * g.ny, fnz = 2 * g.nz;
+
+    for (int c = 0; c < 8; ++c) {
+        const int pj = c & 1, pk = (c >> 1) & 1, pi = (c >> 2) & 1;
+        const int nj = (EJ - pj + 1) / 2, nk = (EK - pk + 1) / 2, ni = (EI - pi + 1) / 2;
+        const int cnt = nj * nk * ni;
+        for (int q0 = 0; q0 < cnt; q0 += 32) {
+            const int q = q0 + grp;
+            int ej = 0, ek = 0, ei = 0;
+            bool valid = q < cnt;
+            if (valid) {
+                ej = pj + 2 * (q % nj);
+                const int r = q / nj;
+                ek = pk + 2 * (r % nk);
+                ei = pi + 2 * (r / nk);
+                const int ex = i0 - 1 + ei, ey = j0 - 1 + ej, ez = k0 - 1 + ek;
+                valid = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz;
+            }
+            T y[24];
+            if (valid) {
+                const int ex = i0 - 1 + ei, ey = j0 - 1 + ej, ez = k0 - 1 + ek;
+                const T cx = CXf[(long)fny * fnz * (2 * ex + is) + (long)fny * (2 * ez + it) + 2 * ey + ie];
+                // nodal values u[x][y][z][c] of the element
+                T u[2][2][2][3];
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int yy = 0; yy < 2; ++yy)
+#pragma unroll
+                        for (int z = 0; z < 2; ++z) {
+                            const int h = 3 * (((ei + x) * HK + (ek + z)) * HJ + (ej + yy));
+                            u[x][yy][z][0] = su[h];
+                            u[x][yy][z][1] = su[h + 1];
+                            u[x][yy][z][2] = su[h + 2];
+                        }
+                // displacement gradient G[c][d] at this Gauss point
+                T G[3][3];
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+                    for (int p = 0; p < 2; ++p)
+#pragma unroll
+                        for (int q2 = 0; q2 < 2; ++q2) {
+                            gx = fma(wy[p] * wz[q2], u[1][p][q2][cc] - u[0][p][q2][cc], gx);
+                            gy = fma(wx[p] * wz[q2], u[p][1][q2][cc] - u[p][0][q2][cc], gy);
+                            gz = fma(wx[p] * wy[q2], u[p][q2][1][cc] - u[p][q2][0][cc], gz);
+                        }
+                    G[cc][0] = ih * gx;
+                    G[cc][1] = ih * gy;
+                    G[cc][2] = ih * gz;
+                }
+                // stress S = cx D eps (isotropic, engineering shear strains)
+                const T tr = G[0][0] + G[1][1] + G[2][2];
+                T S[3][3];
+                S[0][0] = cx * (lam * tr + (lam2mu - lam) * G[0][0]);
+                S[1][1] = cx * (lam * tr + (lam2mu - lam) * G[1][1]);
+                S[2][2] = cx * (lam * tr + (lam2mu - lam) * G[2][2]);
+                S[0][1] = S[1][0] = cx * mu * (G[0][1] + G[1][0]);
+                S[1][2] = S[2][1] = cx * mu * (G[1][2] + G[2][1]);
+                S[0][2] = S[2][0] = cx * mu * (G[2][0] + G[0][2]);
+                // nodal forces y_a^c = sum_d dN_a/dx_d S_cd, node order ie/je/ke
+#pragma unroll
+                for (int n = 0; n < 8; ++n) {
+                    const int x = (n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0;
+                    const int yy = (n == 2 || n == 3 || n == 6 || n == 7) ? 1 : 0;
+                    const int z = n >= 4 ? 1 : 0;
+                    const T dx = (x ? ih : -ih) * wy[yy] * wz[z];
+                    const T dy = (yy ? ih : -ih) * wx[x] * wz[z];
+                    const T dz = (z ? ih : -ih) * wx[x] * wy[yy];
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) y[3 * n + cc] = dx * S[cc][0] + dy * S[cc][1] + dz * S[cc][2];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 24; ++m) y[m] = T(0);
+            }
+            // sum over the 8 Gauss-point lanes, reduce-scatter by lane bits:
+            // bit 2 splits nodes {0..3 | 4..7}, bit 1 {0,1 | 2,3}, bit 0 {0 | 1}
+            T h12[12];
+            {
+                const bool hi = (gp >> 2) & 1;
+#pragma unroll
+                for (int m = 0; m < 12; ++m) {
+                    const T keep = hi ? y[12 + m] : y[m];
+                    const T send = hi ? y[m] : y[12 + m];
+                    h12[m] = keep + __shfl_xor(send, 4, 64);
+                }
+            }
+            T h6[6];
+            {
+                const bool hi = (gp >> 1) & 1;
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    const T keep = hi ? h12[6 + m] : h12[m];
+                    const T send = hi ? h12[m] : h12[6 + m];
+                    h6[m] = keep + __shfl_xor(send, 2, 64);
+                }
+            }
+            T h3[3];
+            {
+                const bool hi = gp & 1;
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const T keep = hi ? h6[3 + m] : h6[m];
+                    const T send = hi ? h6[m] : h6[3 + m];
+                    h3[m] = keep + __shfl_xor(send, 1, 64);
+                }
+            }
+            // lane gp now holds node n = gp of the element (ie/je/ke order)
+            if (valid) {
+                const int n = gp;
+                const int x = (n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0;
+                const int yy = (n == 2 || n == 3 || n == 6 || n == 7) ? 1 : 0;
+                const int z = n >= 4 ? 1 : 0;
+                const int hj = ej + yy, hk = ek + z, hi = ei + x;  // halo coords
+                if (hj >= 1 && hj <= t.TJ && hk >= 1 && hk <= t.TK && hi >= 1 && hi <= t.TI) {
+                    const int own = 3 * (((hi - 1) * t.TK + (hk - 1)) * t.TJ + (hj - 1));
+                    acc[own] += h3[0];
+                    acc[own + 1] += h3[1];
+                    acc[own + 2] += h3[2];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    double part = 0.0;
+    const T om = T(a.omega), one = T(1);
+    for (int own = threadIdx.x; own < nown; own += 256) {
+        const int tj = own % t.TJ;
+        const int r = own / t.TJ;
+        const int tk = r % t.TK, ti = r / t.TK;
+        const int gi = i0 + ti, gj = j0 + tj, gk = k0 + tk;
+        if (gi >= g.nnx || gj >= g.nny || gk >= g.nnz) continue;
+        const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
+        const long dof = 3 * nod;
+        const unsigned msk = a.mask[nod];
+        const int hc = 3 * (((ti + 1) * HK + (tk + 1)) * HJ + (tj + 1));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const bool fixed = (msk >> c) & 1u;
+            const T yv = acc[3 * own + c];
+            const T u = su[hc + c];
+            if (MODE == OP_MATVEC) {
+                const T v = fixed ? T(0) : yv;
+                a.out[dof + c] = v;
+                part += (double)u * (double)v;
+            } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
+                const T rv = fixed ? T(0) : a.F[dof + c] - yv;
+                a.out[dof + c] = rv;
+                if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = u;
+            } else {
+                const T f = a.F[dof + c];
+                const T rv = fixed ? T(0) : f - yv;
+                const T ad = a.AD[dof + c];
+                const T un = (one - om) * u + om * (rv + ad * u) / ad;
+                a.out[dof + c] = un;
+                part += (double)f * (double)un;
+            }
+        }
+    }
+    if (a.partials) {
+        __shared__ double ws[4];
+        double v = part;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+    }
+}
+
+static int l1_balance(int n, int tmax)
+{
+    if (tmax >= n) return n;
+    const int nt = (n + tmax - 1) / tmax;
+    return (n + nt - 1) / nt;
+}
+
+Tile l1_tile(const Grid &g)
+{
+    Tile t;
+    t.TJ = l1_balance(g.nny, 7);
+    t.TK = l1_balance(g.nnz, 7);
+    t.TI = l1_balance(g.nnx, 7);
+    while ((t.TJ + 2) * (t.TK + 2) * (t.TI + 2) > kL1Halo || t.TJ * t.TK * t.TI > kL1Own) {
+        if (t.TI > 1)
+            t.TI--;
+        else if (t.TK > 1)
+            t.TK--;
+        else
+            t.TJ--;
+    }
+    t.ntj = (g.nny + t.TJ - 1) / t.TJ;
+    t.ntk = (g.nnz + t.TK - 1) / t.TK;
+    t.nti = (g.nnx + t.TI - 1) / t.TI;
+    return t;
+}
+
+template <class T>
+void launch_op1_mf(hipStream_t s, int mode, const OpArgsT<T> &a, const T *CXf, const Tile &t, double nu)
+{
+    // Lame constants of D (E = 1): D11 = lam + 2mu, D12 = lam, D44 = mu
+    const double c1 = 1.0 / ((1 + nu) * (1 - 2 * nu));
+    const T l2m = T(c1 * (1 - nu)), lam = T(c1 * nu), mu = T(c1 * (1 - 2 * nu) / 2);
+    const dim3 grid((unsigned)t.blocks()), block(256);
+    switch (mode) {
+    case OP_MATVEC: k_l1_mf<T, OP_MATVEC><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    case OP_DEFECT: k_l1_mf<T, OP_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    case OP_JACOBI: k_l1_mf<T, OP_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    case OP_PRE_JACOBI: k_l1_mf<T, OP_PRE_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    case OP_PRE_DEFECT: k_l1_mf<T, OP_PRE_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    default: break;
+    }
+}
+template void launch_op1_mf<double>(hipStream_t, int, const OpArgsT<double> &, const double *, const Tile &, double);
+template void launch_op1_mf<float>(hipStream_t, int, const OpArgsT<float> &, const float *, const Tile &, double);
+
+}  // namespace gputo

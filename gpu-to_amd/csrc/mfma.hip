@@ -1,16 +1,21 @@
-// Level-0 operator of the mixed-precision inner solve on MFMA: the shared
-// 24x24 element matrix runs against 24 x 16 tiles of element vectors with
-// v_mfma_f32_16x16x32_bf16 (gfx950), split-BF16 operands (hi + lo, three
-// products hi*hi + hi*lo + lo*hi) and FP32 accumulation.
+// Level-0 operator of the mixed-precision inner solve on the matrix cores:
+// the shared 24x24 element matrix runs against 24 x 16 tiles of element
+// vectors (16 elements per MFMA column tile, Ke rows padded 24 -> 32) with
+// FP32 accumulation.  Three flavours of the same kernel:
+//   MF_F32   v_mfma_f32_16x16x4_f32, exact FP32 products (K = 24 in 6 steps)
+//   MF_BF16  v_mfma_f32_16x16x32_bf16 on split operands x = hi + lo,
+//            products hi*hi + hi*lo + lo*hi (K padded 24 -> 32)
+//   MF_BF16X3 three-way split x = hi + mid + lo (exact for FP32 inputs) and
+//            the six products whose order is >= 2^-24
 //
 // One block owns a TJ x TK x TI brick of nodes.  It stages the nodal values
-// of the brick plus a one-node halo and the element scales of the
-// (TJ+1)(TK+1)(TI+1) element box in LDS, multiplies every element of the box
-// (16 elements per MFMA column tile, rows of Ke padded 24 -> 32, K padded
-// 24 -> 32), scales each column by E_e, and adds the rows that belong to
-// owned nodes into LDS accumulators in 8-colour element order: elements of
-// one colour share no node, so a colour pass is conflict-free and every node
-// receives its 8 contributions in a fixed order.  No atomics; deterministic.
+// of the brick plus a one-node halo and builds a colour-sorted table of the
+// (TJ+1)(TK+1)(TI+1) elements of its box (origin offset, E) in LDS.  Each
+// wave multiplies 16 elements of one colour at a time, scales each column by
+// E_e and adds the 24 rows into LDS node accumulators with ds_add_f32.
+// Elements of one colour share no node, so a colour pass never adds twice to
+// one address, and every node receives its 8 element contributions in the
+// fixed colour order: no global atomics, bitwise deterministic.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -21,6 +26,8 @@ namespace gputo {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+enum MfFlavor { MF_F32 = 0, MF_BF16 = 1, MF_BF16X3 = 2 };
 
 constexpr int kMfHalo = 2048;  // halo nodes per block
 constexpr int kMfElem = 1280;  // element box per block
@@ -45,13 +52,33 @@ __device__ __forceinline__ double mf_block_sum(double v)
     return ws[0] + ws[1] + ws[2] + ws[3];
 }
 
-template <int MODE>
+// halo-relative offset (in floats) of element DOF k for an element whose
+// origin node has halo index 0
+__device__ __forceinline__ int dof_off(int k, int HJ, int HK)
+{
+    const int ln = k / 3, comp = k - 3 * (k / 3);
+    return 3 * ((mIE[ln] * HK + mKE[ln]) * HJ + mJE[ln]) + comp;
+}
+
+__device__ __forceinline__ void split2(float v, __bf16 &h, __bf16 &l)
+{
+    h = (__bf16)v;
+    l = (__bf16)(v - (float)h);
+}
+
+__device__ __forceinline__ void split3(float v, __bf16 &h, __bf16 &m, __bf16 &l)
+{
+    h = (__bf16)v;
+    const float r = v - (float)h;
+    m = (__bf16)r;
+    l = (__bf16)(r - (float)m);
+}
+
+template <int FL, int MODE>
 __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
 {
     if (a.ctl && a.ctl[0]) return;
-    __shared__ float su[3 * kMfHalo];
-    __shared__ float sE[kMfElem];
-    __shared__ float acc[3 * kMfOwn];
+    extern __shared__ float smem[];
     const Grid g = a.g;
     int b = blockIdx.x;
     const int bj = b % t.ntj;
@@ -61,9 +88,12 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
     const int j0 = bj * t.TJ, k0 = bk * t.TK, i0 = bi * t.TI;
     const int HJ = t.TJ + 2, HK = t.TK + 2, HI = t.TI + 2;
     const int EJ = t.TJ + 1, EK = t.TK + 1, EI = t.TI + 1;
-    const int nown = t.TJ * t.TK * t.TI;
+    const int NH = HJ * HK * HI, NE = EJ * EK * EI;
+    float *su = smem;            // [3*NH] nodal values, AoS
+    float *acc = su + 3 * NH;    // [3*NH] node accumulators
+    float *cE = acc + 3 * NH;    // [NE]   element scale, colour order
+    int *ch = (int *)(cE + NE);  // [NE]   3 * halo index of element origin, colour order
 
-    // stage nodal values (AoS, node h -> su[3h + c]) and element scales
     const int rowlen = 3 * HJ;
     for (int q = threadIdx.x; q < rowlen * HK * HI; q += 256) {
         const int row = q / rowlen;
@@ -79,96 +109,149 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
             else
                 v = a.U[dof];
         }
-        su[3 * ((hi * HK + hk) * HJ + hj) + c] = v;
+        const int s = 3 * ((hi * HK + hk) * HJ + hj) + c;
+        su[s] = v;
+        acc[s] = 0.0f;
     }
-    for (int q = threadIdx.x; q < EJ * EK * EI; q += 256) {
+    // colour-sorted element table: colour c = (ej&1) + 2(ek&1) + 4(ei&1)
+    for (int q = threadIdx.x; q < NE; q += 256) {
         const int ej = q % EJ;
         const int r = q / EJ;
         const int ek = r % EK, ei = r / EK;
+        const int pj = ej & 1, pk = ek & 1, pi = ei & 1;
+        const int c = pj + 2 * pk + 4 * pi;
+        int off = 0;
+        for (int cc = 0; cc < c; ++cc) {
+            const int qj = cc & 1, qk = (cc >> 1) & 1, qi = (cc >> 2) & 1;
+            off += ((EJ - qj + 1) / 2) * ((EK - qk + 1) / 2) * ((EI - qi + 1) / 2);
+        }
+        const int nj = (EJ - pj + 1) / 2, nk = (EK - pk + 1) / 2;
+        const int pos = off + ((ei >> 1) * nk + (ek >> 1)) * nj + (ej >> 1);
         const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = k0 - 1 + ek;
-        float v = 0.0f;
+        float E = 0.0f;
         if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz)
-            v = a.CX[(long)g.ny * g.nz * gi + (long)g.ny * gk + gj];
-        sE[q] = v;
+            E = a.CX[(long)g.ny * g.nz * gi + (long)g.ny * gk + gj];
+        cE[pos] = E;
+        ch[pos] = 3 * ((ei * HK + ek) * HJ + ej);
     }
-    for (int q = threadIdx.x; q < 3 * nown; q += 256) acc[q] = 0.0f;
 
-    // Ke fragments: lane holds rows 16*tile + (lane % 16), K slice 8*(lane/16) .. +7
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lg = lane >> 4;
-    bf16x8 Ahi[2], Alo[2];
+    // A fragments (Ke rows 16*tile + lr) and per-lane DOF offsets
+    float Af[2][6];
+    bf16x8 Ah[2], Am[2], Al[2];
+    int koff[8];
+    if (FL == MF_F32) {
 #pragma unroll
-    for (int tt = 0; tt < 2; ++tt) {
-        const int row = 16 * tt + lr;
+        for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = 8 * lg + i;
-            const float v = (row < 24 && k < 24) ? a.Ke[24 * row + k] : 0.0f;
-            const __bf16 hi = (__bf16)v;
-            Ahi[tt][i] = hi;
-            Alo[tt][i] = (__bf16)(v - (float)hi);
+            for (int s = 0; s < 6; ++s) {
+                const int row = 16 * tt + lr;
+                Af[tt][s] = row < 24 ? a.Ke[24 * row + 4 * s + lg] : 0.0f;
+            }
+#pragma unroll
+        for (int s = 0; s < 6; ++s) koff[s] = dof_off(4 * s + lg, HJ, HK);
+    } else {
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt) {
+            const int row = 16 * tt + lr;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = 8 * lg + i;
+                const float v = (row < 24 && k < 24) ? a.Ke[24 * row + k] : 0.0f;
+                __bf16 h, m, l;
+                if (FL == MF_BF16X3) {
+                    split3(v, h, m, l);
+                    Am[tt][i] = m;
+                } else {
+                    split2(v, h, l);
+                }
+                Ah[tt][i] = h;
+                Al[tt][i] = l;
+            }
         }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) koff[i] = (8 * lg + i < 24) ? dof_off(8 * lg + i, HJ, HK) : -1;
     }
+    int ooff[2][4];
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = 16 * tt + 4 * lg + r;
+            ooff[tt][r] = m < 24 ? dof_off(m, HJ, HK) : -1;
+        }
     __syncthreads();
 
+    int off = 0;
     for (int c = 0; c < 8; ++c) {
         const int pj = c & 1, pk = (c >> 1) & 1, pi = (c >> 2) & 1;
-        const int nj = (EJ - pj + 1) / 2, nk = (EK - pk + 1) / 2, ni = (EI - pi + 1) / 2;
-        const int cnt = nj * nk * ni;
+        const int cnt = ((EJ - pj + 1) / 2) * ((EK - pk + 1) / 2) * ((EI - pi + 1) / 2);
         const int nb = (cnt + 15) / 16;
         for (int bb = wave; bb < nb; bb += 4) {
             const int idx = 16 * bb + lr;
             const bool valid = idx < cnt;
-            int ej = 0, ek = 0, ei = 0;
-            if (valid) {
-                ej = pj + 2 * (idx % nj);
-                const int r = idx / nj;
-                ek = pk + 2 * (r % nk);
-                ei = pi + 2 * (r / nk);
-            }
-            bf16x8 Bhi, Blo;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int k = 8 * lg + i;
-                float v = 0.0f;
-                if (valid && k < 24) {
-                    const int ln = k / 3, comp = k - 3 * (k / 3);
-                    const int h = ((ei + mIE[ln]) * HK + (ek + mKE[ln])) * HJ + (ej + mJE[ln]);
-                    v = su[3 * h + comp];
-                }
-                const __bf16 hi = (__bf16)v;
-                Bhi[i] = hi;
-                Blo[i] = (__bf16)(v - (float)hi);
-            }
+            const int base = valid ? ch[off + idx] : 0;
+            const float E = valid ? cE[off + idx] : 0.0f;
             f32x4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
-            d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi[0], Bhi, d0, 0, 0, 0);
-            d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi[0], Blo, d0, 0, 0, 0);
-            d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Alo[0], Bhi, d0, 0, 0, 0);
-            d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi[1], Bhi, d1, 0, 0, 0);
-            d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi[1], Blo, d1, 0, 0, 0);
-            d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Alo[1], Bhi, d1, 0, 0, 0);
-            if (valid) {
-                const float E = sE[(ei * EK + ek) * EJ + ej];
+            if (FL == MF_F32) {
+#pragma unroll
+                for (int s = 0; s < 6; ++s) {
+                    const float bv = valid ? su[base + koff[s]] : 0.0f;
+                    d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[0][s], bv, d0, 0, 0, 0);
+                    d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[1][s], bv, d1, 0, 0, 0);
+                }
+            } else {
+                bf16x8 Bh, Bm, Bl;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float v = (valid && koff[i] >= 0) ? su[base + koff[i]] : 0.0f;
+                    __bf16 h, m, l;
+                    if (FL == MF_BF16X3) {
+                        split3(v, h, m, l);
+                        Bm[i] = m;
+                    } else {
+                        split2(v, h, l);
+                    }
+                    Bh[i] = h;
+                    Bl[i] = l;
+                }
 #pragma unroll
                 for (int tt = 0; tt < 2; ++tt) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int m = 16 * tt + 4 * lg + r;  // element DOF of this output row
-                        if (m >= 24) continue;
-                        const int ln = m / 3, comp = m - 3 * (m / 3);
-                        const int hj = ej + mJE[ln], hk = ek + mKE[ln], hi = ei + mIE[ln];
-                        if (hj < 1 || hj > t.TJ || hk < 1 || hk > t.TK || hi < 1 || hi > t.TI) continue;
-                        const int own = ((hi - 1) * t.TK + (hk - 1)) * t.TJ + (hj - 1);
-                        acc[3 * own + comp] += E * (tt ? d1[r] : d0[r]);
+                    f32x4 d = {0.f, 0.f, 0.f, 0.f};
+                    // smallest terms first
+                    if (FL == MF_BF16X3) {
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[tt], Bh, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[tt], Bm, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[tt], Bl, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[tt], Bh, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[tt], Bm, d, 0, 0, 0);
+                    } else {
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[tt], Bh, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[tt], Bl, d, 0, 0, 0);
                     }
+                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[tt], Bh, d, 0, 0, 0);
+                    if (tt == 0)
+                        d0 = d;
+                    else
+                        d1 = d;
+                }
+            }
+            if (valid) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    atomicAdd(&acc[base + ooff[0][r]], E * d0[r]);
+                    if (ooff[1][r] >= 0) atomicAdd(&acc[base + ooff[1][r]], E * d1[r]);
                 }
             }
         }
+        off += cnt;
         __syncthreads();
     }
 
     double part = 0.0;
     const float om = (float)a.omega;
+    const int nown = t.TJ * t.TK * t.TI;
     for (int own = threadIdx.x; own < nown; own += 256) {
         const int tj = own % t.TJ;
         const int r = own / t.TJ;
@@ -178,12 +261,12 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
         const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
         const long dof = 3 * nod;
         const unsigned msk = a.mask[nod];
-        const int hc = ((ti + 1) * HK + (tk + 1)) * HJ + (tj + 1);
+        const int hc = 3 * (((ti + 1) * HK + (tk + 1)) * HJ + (tj + 1));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const bool fixed = (msk >> c) & 1u;
-            const float y = acc[3 * own + c];
-            const float u = su[3 * hc + c];
+            const float y = acc[hc + c];
+            const float u = su[hc + c];
             if (MODE == OP_MATVEC) {
                 const float v = fixed ? 0.0f : y;
                 a.out[dof + c] = v;
@@ -236,19 +319,38 @@ static Tile mfma_tile(const Grid &g)
     return t;
 }
 
-void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const void *ctx)
+static size_t mf_lds_bytes(const Tile &t)
+{
+    const size_t NH = (size_t)(t.TJ + 2) * (t.TK + 2) * (t.TI + 2);
+    const size_t NE = (size_t)(t.TJ + 1) * (t.TK + 1) * (t.TI + 1);
+    return (6 * NH + 2 * NE) * sizeof(float);
+}
+
+template <int FL>
+static void launch_flavor(hipStream_t s, int mode, const OpArgsT<float> &a, const Tile &t)
+{
+    const dim3 grid((unsigned)t.blocks()), block(256);
+    const size_t lds = mf_lds_bytes(t);
+    switch (mode) {
+    case OP_MATVEC: k_mfma_op<FL, OP_MATVEC><<<grid, block, lds, s>>>(a, t); break;
+    case OP_DEFECT: k_mfma_op<FL, OP_DEFECT><<<grid, block, lds, s>>>(a, t); break;
+    case OP_JACOBI: k_mfma_op<FL, OP_JACOBI><<<grid, block, lds, s>>>(a, t); break;
+    case OP_PRE_JACOBI: k_mfma_op<FL, OP_PRE_JACOBI><<<grid, block, lds, s>>>(a, t); break;
+    case OP_PRE_DEFECT: k_mfma_op<FL, OP_PRE_DEFECT><<<grid, block, lds, s>>>(a, t); break;
+    default: break;
+    }
+}
+
+void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const void *ctx, int flavor)
 {
     const MixedState *m = static_cast<const MixedState *>(ctx);
     const Tile &t = m->mf->t;
-    const dim3 grid((unsigned)t.blocks()), block(256);
-    switch (mode) {
-    case OP_MATVEC: k_mfma_op<OP_MATVEC><<<grid, block, 0, s>>>(a, t); break;
-    case OP_DEFECT: k_mfma_op<OP_DEFECT><<<grid, block, 0, s>>>(a, t); break;
-    case OP_JACOBI: k_mfma_op<OP_JACOBI><<<grid, block, 0, s>>>(a, t); break;
-    case OP_PRE_JACOBI: k_mfma_op<OP_PRE_JACOBI><<<grid, block, 0, s>>>(a, t); break;
-    case OP_PRE_DEFECT: k_mfma_op<OP_PRE_DEFECT><<<grid, block, 0, s>>>(a, t); break;
-    default: break;
-    }
+    if (flavor == MF_F32)
+        launch_flavor<MF_F32>(s, mode, a, t);
+    else if (flavor == MF_BF16X3)
+        launch_flavor<MF_BF16X3>(s, mode, a, t);
+    else
+        launch_flavor<MF_BF16>(s, mode, a, t);
 }
 
 long mfma_tile_blocks(const Grid &g) { return mfma_tile(g).blocks(); }
@@ -270,24 +372,31 @@ void mfma_prepare(gputo_handle *, MixedState *) {}
 int mfma_blocks(const gputo_handle *h) { return (int)h->mx->mf->t.blocks(); }
 
 // ---------------------------------------------------------------------------
-// Lane-layout self-check of the BF16 MFMA: D = A B for a 16x32 A and a
-// 32x16 B of small integers (exact in BF16), with the fragment layout the
-// operator kernel assumes.  One wave, all accesses in bounds.
+// Lane-layout self-checks of the two MFMA instructions the level-0 kernel
+// uses: D = A B for small integer matrices (exact in BF16 and FP32) with the
+// fragment layouts the operator kernel assumes.  One wave, all accesses in
+// bounds.
 // ---------------------------------------------------------------------------
-__global__ void k_mfma_selftest(const float *A, const float *B, float *D)
+__global__ void k_mfma_selftest(const float *A, const float *B, float *D, const float *A4, const float *B4,
+                                float *D4)
 {
     const int lane = threadIdx.x;
     const int lr = lane & 15, lg = lane >> 4;
     bf16x8 fa, fb;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        fa[i] = (__bf16)A[32 * lr + 8 * lg + i];
-        fb[i] = (__bf16)B[16 * (8 * lg + i) + lr];
+        fa[i] = (__bf16)A[32 * lr + 8 * lg + i];  // A: 16 x 32
+        fb[i] = (__bf16)B[16 * (8 * lg + i) + lr];  // B: 32 x 16
     }
     f32x4 d = {0.f, 0.f, 0.f, 0.f};
     d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, d, 0, 0, 0);
+    f32x4 d4 = {0.f, 0.f, 0.f, 0.f};
+    d4 = __builtin_amdgcn_mfma_f32_16x16x4f32(A4[4 * lr + lg], B4[16 * lg + lr], d4, 0, 0, 0);  // 16x4 * 4x16
 #pragma unroll
-    for (int r = 0; r < 4; ++r) D[16 * (4 * lg + r) + lr] = d[r];
+    for (int r = 0; r < 4; ++r) {
+        D[16 * (4 * lg + r) + lr] = d[r];
+        D4[16 * (4 * lg + r) + lr] = d4[r];
+    }
 }
 
 }  // namespace gputo
@@ -296,29 +405,41 @@ extern "C" int gputo_selftest_mfma(double *max_err)
 {
     using namespace gputo;
     if (!max_err) return GPUTO_ERR_ARGUMENT;
-    std::vector<float> A(16 * 32), B(32 * 16), D(256), R(256, 0.f);
+    std::vector<float> A(16 * 32), B(32 * 16), R(256, 0.f), A4(16 * 4), B4(4 * 16), R4(256, 0.f);
     for (int i = 0; i < 16 * 32; ++i) A[i] = (float)((i * 7 + 3) % 11 - 5);
     for (int i = 0; i < 32 * 16; ++i) B[i] = (float)((i * 5 + 1) % 9 - 4);
+    for (int i = 0; i < 64; ++i) {
+        A4[i] = (float)((i * 3 + 2) % 7 - 3);
+        B4[i] = (float)((i * 11 + 5) % 13 - 6);
+    }
     for (int r = 0; r < 16; ++r)
-        for (int c = 0; c < 16; ++c)
+        for (int c = 0; c < 16; ++c) {
             for (int k = 0; k < 32; ++k) R[16 * r + c] += A[32 * r + k] * B[16 * k + c];
-    float *dA = nullptr, *dB = nullptr, *dD = nullptr;
-    if (hipMalloc(&dA, A.size() * 4) != hipSuccess || hipMalloc(&dB, B.size() * 4) != hipSuccess ||
-        hipMalloc(&dD, D.size() * 4) != hipSuccess)
-        return GPUTO_ERR_DEVICE;
+            for (int k = 0; k < 4; ++k) R4[16 * r + c] += A4[4 * r + k] * B4[16 * k + c];
+        }
+    float *dev = nullptr;
+    const size_t n = A.size() + B.size() + 256 + A4.size() + B4.size() + 256;
+    if (hipMalloc(&dev, n * sizeof(float)) != hipSuccess) return GPUTO_ERR_DEVICE;
+    float *dA = dev, *dB = dA + A.size(), *dD = dB + B.size(), *dA4 = dD + 256, *dB4 = dA4 + A4.size(),
+          *dD4 = dB4 + B4.size();
     hipError_t e = hipMemcpy(dA, A.data(), A.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dB, B.data(), B.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dA4, A4.data(), A4.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dB4, B4.data(), B4.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        k_mfma_selftest<<<1, 64>>>(dA, dB, dD);
+        k_mfma_selftest<<<1, 64>>>(dA, dB, dD, dA4, dB4, dD4);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpy(D.data(), dD, D.size() * 4, hipMemcpyDeviceToHost);
-    (void)hipFree(dA);
-    (void)hipFree(dB);
-    (void)hipFree(dD);
+    std::vector<float> D(256), D4(256);
+    if (e == hipSuccess) e = hipMemcpy(D.data(), dD, 256 * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(D4.data(), dD4, 256 * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(dev);
     if (e != hipSuccess) return GPUTO_ERR_DEVICE;
     double err = 0.0;
-    for (int i = 0; i < 256; ++i) err = std::max(err, (double)std::fabs(D[i] - R[i]));
+    for (int i = 0; i < 256; ++i) {
+        err = std::max(err, (double)std::fabs(D[i] - R[i]));
+        err = std::max(err, (double)std::fabs(D4[i] - R4[i]));
+    }
     *max_err = err;
     return 0;
 }

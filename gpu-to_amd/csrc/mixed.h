@@ -29,8 +29,9 @@ void mixed_create(gputo_handle *h);
 void mixed_destroy(gputo_handle *h);
 void mixed_stats(gputo_handle *h, gputo_to_stats *st);
 int mixed_solve(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit);
-int mixed_apply_l0(gputo_handle *h, int precision, const double *u, double *y);
-int mixed_bench_operator(gputo_handle *h, int precision, int reps, double *avg_ms);
+int mixed_apply_l0(gputo_handle *h, int kind, const double *u, double *y);
+int mixed_bench_operator(gputo_handle *h, int kind, int reps, double *avg_ms);
+HierView<float> mixed_level_view(gputo_handle *h);
 
 // split-BF16 MFMA level-0 operator (mfma.hip)
 void mfma_create(gputo_handle *h, MixedState *m);

@@ -82,6 +82,14 @@ void mixed_stats(gputo_handle *h, gputo_to_stats *st)
     st->learned_rejected = h->mx->last_learned_bad;
 }
 
+// Level-0 kernel of the inner solve: explicit choice, else MFMA FP32 for
+// MIXED_F32 and the three-way split-BF16 MFMA for MIXED_BF16.
+static int l0_kind(const gputo_params &p, int precision)
+{
+    if (p.l0_kernel >= L0_GATHER && p.l0_kernel <= L0_MFMA_BF16X3) return p.l0_kernel;
+    return precision == GPUTO_PREC_MIXED_BF16 ? L0_MFMA_BF16X3 : L0_MFMA_F32;
+}
+
 // FP32 copies of the FP64 hierarchy (operators, diagonals) built by
 // setup_hierarchy.
 static void mixed_prepare(gputo_handle *h)
@@ -91,12 +99,13 @@ static void mixed_prepare(gputo_handle *h)
     const Grid &g0 = h->L[0].g;
     launch_copy<double, float>(s, h->Ke, m->Ke32, 576, nullptr);
     launch_copy<double, float>(s, h->CX, m->CX32, g0.nel, nullptr);
+    m->H.l1_mf = use_l1_mf(h->prm, h->nl);
     for (int l = 0; l < h->nl; ++l) {
         const Level &v = h->L[l];
         launch_copy<double, float>(s, v.AD, m->H.L[l].AD, v.g.ndof, nullptr);
-        if (l > 0) launch_copy<double, float>(s, v.A, m->A32[l], 300 * v.g.nel, nullptr);
+        if (l > 0 && !(l == 1 && m->H.l1_mf)) launch_copy<double, float>(s, v.A, m->A32[l], 300 * v.g.nel, nullptr);
     }
-    m->H.l0 = (h->prm.precision == GPUTO_PREC_MIXED_BF16) ? L0_MFMA_BF16 : L0_GATHER;
+    m->H.l0 = l0_kind(h->prm, h->prm.precision);
     m->H.l0_ctx = m;
     if (m->H.l0 == L0_MFMA_BF16) mfma_prepare(h, m);
 }
@@ -168,7 +177,7 @@ static int inner_pcg(gputo_handle *h, double itol, int maxit)
             a.out = m->Q;
             a.partials = h->partials;
             level_op(h, H, 0, OP_MATVEC, a);
-            launch_cg_alpha(s, m->sc, h->partials, H.l0 == L0_MFMA_BF16 ? mfma_blocks(h) : nmv);
+            launch_cg_alpha(s, m->sc, h->partials, H.l0 >= L0_MFMA_F32 ? mfma_blocks(h) : nmv);
             launch_cg_update_ur<float>(s, m->X, m->R, m->P, m->Q, n, m->sc, h->partials);
             launch_cg_check(s, m->sc, h->partials, kRedBlocks, itol, maxit);
             vcycle_t<float>(h, H, m->R, m->Z, ctl);
@@ -251,10 +260,10 @@ int mixed_solve(gputo_handle *h, const double *F, double *U, double tol, int max
     return 0;
 }
 
-// y = K(CX) u with the reduced-precision level-0 operator (FP32 gather or
-// split-BF16 MFMA), u and y FP64 device vectors; Ke, CX and the level-0 mask
-// must be current on the handle.
-int mixed_apply_l0(gputo_handle *h, int precision, const double *u, double *y)
+// y = K(CX) u with the reduced-precision level-0 kernel `kind` (an L0Kind),
+// u and y FP64 device vectors; Ke, CX and the level-0 mask must be current
+// on the handle.
+int mixed_apply_l0(gputo_handle *h, int kind, const double *u, double *y)
 {
     if (!h->mx) mixed_create(h);
     MixedState *m = h->mx;
@@ -262,7 +271,7 @@ int mixed_apply_l0(gputo_handle *h, int precision, const double *u, double *y)
     const Grid &g0 = h->L[0].g;
     launch_copy<double, float>(s, h->Ke, m->Ke32, 576, nullptr);
     launch_copy<double, float>(s, h->CX, m->CX32, g0.nel, nullptr);
-    m->H.l0 = (precision == GPUTO_PREC_MIXED_BF16) ? L0_MFMA_BF16 : L0_GATHER;
+    m->H.l0 = kind;
     m->H.l0_ctx = m;
     launch_copy<double, float>(s, u, m->P, g0.ndof, nullptr);
     OpArgsT<float> a = view_args(h, m->H, 0, nullptr);
@@ -274,15 +283,26 @@ int mixed_apply_l0(gputo_handle *h, int precision, const double *u, double *y)
     return 0;
 }
 
-int mixed_bench_operator(gputo_handle *h, int precision, int reps, double *avg_ms)
+// FP32 view of the current hierarchy with every level's stored matrices
+// cast (level-operator checks and benchmarks); vectors X/P/Q are free.
+HierView<float> mixed_level_view(gputo_handle *h)
+{
+    if (!h->mx) mixed_create(h);
+    MixedState *m = h->mx;
+    mixed_prepare(h);
+    if (m->H.l1_mf && h->nl > 1)
+        launch_copy<double, float>(h->stream, h->L[1].A, m->A32[1], 300 * h->L[1].g.nel, nullptr);
+    return m->H;
+}
+
+// mean launch time of the reduced-precision level-0 kernel `kind`
+int mixed_bench_operator(gputo_handle *h, int kind, int reps, double *avg_ms)
 {
     if (!h->mx) mixed_create(h);
     MixedState *m = h->mx;
     hipStream_t s = h->stream;
-    const int saved = h->prm.precision;
-    h->prm.precision = precision;
     mixed_prepare(h);
-    h->prm.precision = saved;
+    m->H.l0 = kind;
     launch_fill<float>(s, m->P, h->L[0].g.ndof, 1.0);
     OpArgsT<float> a = view_args(h, m->H, 0, nullptr);
     a.U = m->P;

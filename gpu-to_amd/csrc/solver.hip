@@ -198,6 +198,7 @@ HierView<double> f64_view(const gputo_handle *h)
     }
     H.Ke = h->Ke;
     H.CX = h->CX;
+    H.l1_mf = use_l1_mf(h->prm, h->nl);
     return H;
 }
 
@@ -325,7 +326,9 @@ void gputo_default_params(gputo_params *p)
     p->preset = GPUTO_PRESET_REF_C;
     p->precision = GPUTO_PREC_F64;
     p->learned = 0;
-    p->inner_tol = 1e-4;
+    p->inner_tol = 1e-6;
+    p->l0_kernel = 0;
+    p->l1_kernel = 0;
     p->print_level = 0;
 }
 
@@ -374,6 +377,10 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
             v.g = make_grid(nelx >> l, nely >> l, nelz >> l);
             v.t = choose_tile(v.g);
             maxblocks = std::max<long>(maxblocks, v.t.blocks());
+            if (l == 1) {
+                v.tmf = l1_tile(v.g);
+                maxblocks = std::max<long>(maxblocks, v.tmf.blocks());
+            }
             v.R = dalloc<double>(h, v.g.ndof);
             v.AD = dalloc<double>(h, v.g.ndof);
             v.T = dalloc<double>(h, v.g.ndof);
@@ -576,11 +583,10 @@ int gputo_matvec_f64(gputo_handle *h, const double *Ke0, const double *CX, const
     return host_op(h, 0, Ke0, CX, nullptr, u, fixed, nfd, y);
 }
 
-int gputo_matvec_mixed(gputo_handle *h, int precision, const double *Ke0, const double *CX, const double *u,
+int gputo_matvec_mixed(gputo_handle *h, int kernel, const double *Ke0, const double *CX, const double *u,
                        const double *fixed, int nfd, double *y)
 {
-    if (!h || !u || !y || (precision != GPUTO_PREC_MIXED_F32 && precision != GPUTO_PREC_MIXED_BF16))
-        return GPUTO_ERR_ARGUMENT;
+    if (!h || !u || !y || kernel < 1 || kernel > 4) return GPUTO_ERR_ARGUMENT;
     try {
         const Grid &g0 = h->L[0].g;
         std::vector<int> fx;
@@ -590,7 +596,7 @@ int gputo_matvec_mixed(gputo_handle *h, int precision, const double *Ke0, const 
         hipStream_t s = h->stream;
         HIPCHK(hipMemcpyAsync(h->L[0].mask, m.data(), m.size(), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(h->P, u, g0.ndof * sizeof(double), hipMemcpyHostToDevice, s));
-        mixed_apply_l0(h, precision, h->P, h->Q);
+        mixed_apply_l0(h, kernel, h->P, h->Q);
         HIPCHK(hipMemcpyAsync(y, h->Q, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         h->hier_ready = false;
@@ -717,14 +723,14 @@ int gputo_interp_f64(gputo_handle *h, int level, const double *u_coarse, double 
     }
 }
 
-int gputo_bench_operator(gputo_handle *h, int precision, int reps, double *avg_ms)
+int gputo_bench_operator(gputo_handle *h, int kernel, int reps, double *avg_ms)
 {
-    if (!h || !h->hier_ready || reps < 1 || !avg_ms) return GPUTO_ERR_ARGUMENT;
+    if (!h || !h->hier_ready || reps < 1 || !avg_ms || kernel < 0 || kernel > 4) return GPUTO_ERR_ARGUMENT;
     try {
         hipStream_t s = h->stream;
         const Grid &g0 = h->L[0].g;
         launch_fill(s, h->P, g0.ndof, 1.0);
-        if (precision != GPUTO_PREC_F64) return mixed_bench_operator(h, precision, reps, avg_ms);
+        if (kernel != 0) return mixed_bench_operator(h, kernel, reps, avg_ms);
         OpArgs a = op_args(h, 0);
         a.ctl = nullptr;
         a.U = h->P;
@@ -741,6 +747,100 @@ int gputo_bench_operator(gputo_handle *h, int precision, int reps, double *avg_m
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         hipEventDestroy(e0);
         hipEventDestroy(e1);
+        HIPCHK(hipGetLastError());
+        *avg_ms = ms / reps;
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+}  // extern "C"
+
+namespace {
+
+// Launch one level-`level` operator application of mode `mode` (0 matvec,
+// 1 defect, 2 damped Jacobi) with operator `op` on device vectors.
+template <class T>
+void level_apply_t(gputo_handle *h, HierView<T> H, int level, int op, int mode, const T *u, const T *f, T *y)
+{
+    H.l1_mf = (level == 1 && op == 2);
+    OpArgsT<T> a = view_args(h, H, level, nullptr);
+    a.U = u;
+    a.F = f;
+    a.out = y;
+    const int m = mode == 0 ? OP_MATVEC : mode == 1 ? OP_DEFECT : OP_JACOBI;
+    level_op(h, H, level, m, a);
+}
+
+int level_args_ok(const gputo_handle *h, int level, int op, int mode)
+{
+    if (!h || !h->hier_ready || level < 0 || level >= h->nl - 1 || mode < 0 || mode > 2) return 0;
+    if (op == 2) return level == 1 && !preset_of(h->prm.preset).l1_galerkin;
+    return op == 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gputo_level_apply(gputo_handle *h, int level, int op, int mode, int fp32, const double *u, const double *f,
+                      double *y)
+{
+    if (!level_args_ok(h, level, op, mode) || !u || !y || (mode > 0 && !f)) return GPUTO_ERR_ARGUMENT;
+    try {
+        hipStream_t s = h->stream;
+        const long n = h->L[level].g.ndof;
+        HIPCHK(hipMemcpyAsync(h->P, u, n * sizeof(double), hipMemcpyHostToDevice, s));
+        if (f) HIPCHK(hipMemcpyAsync(h->Z, f, n * sizeof(double), hipMemcpyHostToDevice, s));
+        if (fp32) {
+            HierView<float> H = mixed_level_view(h);
+            MixedState *m = h->mx;
+            launch_copy<double, float>(s, h->P, m->P, n, nullptr);
+            launch_copy<double, float>(s, h->Z, m->X, n, nullptr);
+            level_apply_t<float>(h, H, level, op, mode, m->P, m->X, m->Q);
+            launch_copy<float, double>(s, m->Q, h->Q, n, nullptr);
+        } else {
+            level_apply_t<double>(h, f64_view(h), level, op, mode, h->P, h->Z, h->Q);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(y, h->Q, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+int gputo_bench_level(gputo_handle *h, int level, int op, int fp32, int reps, double *avg_ms)
+{
+    if (!level_args_ok(h, level, op, 0) || reps < 1 || !avg_ms) return GPUTO_ERR_ARGUMENT;
+    try {
+        hipStream_t s = h->stream;
+        const long n = h->L[level].g.ndof;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        if (fp32) {
+            HierView<float> H = mixed_level_view(h);
+            MixedState *m = h->mx;
+            launch_fill<float>(s, m->P, n, 1.0);
+            level_apply_t<float>(h, H, level, op, 0, m->P, nullptr, m->Q);  // warm-up
+            HIPCHK(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) level_apply_t<float>(h, H, level, op, 0, m->P, nullptr, m->Q);
+        } else {
+            const HierView<double> H = f64_view(h);
+            launch_fill<double>(s, h->P, n, 1.0);
+            level_apply_t<double>(h, H, level, op, 0, h->P, nullptr, h->Q);
+            HIPCHK(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) level_apply_t<double>(h, H, level, op, 0, h->P, nullptr, h->Q);
+        }
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
         HIPCHK(hipGetLastError());
         *avg_ms = ms / reps;
         return 0;

@@ -24,7 +24,8 @@ inline Preset preset_of(int p)
 }
 
 // Level-0 operator implementation of a view.
-enum L0Kind { L0_GATHER = 0, L0_MFMA_BF16 = 1 };
+// (values match gputo_params.l0_kernel; MFMA flavour = kind - L0_MFMA_F32)
+enum L0Kind { L0_AUTO = 0, L0_GATHER = 1, L0_MFMA_F32 = 2, L0_MFMA_BF16 = 3, L0_MFMA_BF16X3 = 4 };
 
 template <class T>
 struct LevelView {
@@ -39,10 +40,26 @@ struct HierView {
     const T *CX = nullptr;   // level-0 element scales
     int l0 = L0_GATHER;
     const void *l0_ctx = nullptr;  // state of a non-gather level-0 kernel
+    bool l1_mf = false;            // level 1 applied matrix-free from CX
 };
 
 // level-0 BF16-MFMA operator (mfma.hip); ctx is the MixedState
-void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const void *ctx);
+void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const void *ctx, int flavor);
+
+// matrix-free rediscretised level-1 operator (l1op.hip); CXf = level-0 CX
+template <class T>
+void launch_op1_mf(hipStream_t s, int mode, const OpArgsT<T> &a, const T *CXf, const Tile &t, double nu);
+Tile l1_tile(const Grid &g);
+constexpr double kL1Nu = 0.3;  // Poisson ratio of the level-1 rediscretisation (mgcg1.c:1539-1584)
+
+// Whether level 1 runs matrix-free under the handle's parameters.
+inline bool use_l1_mf(const gputo_params &prm, int nl)
+{
+    if (nl < 3) return false;  // level 1 is the coarsest: its stored matrix feeds the Cholesky
+    const bool rediscretised = !preset_of(prm.preset).l1_galerkin;
+    if (prm.l1_kernel == 1) return false;
+    return rediscretised;
+}
 
 template <class T>
 OpArgsT<T> view_args(const gputo_handle *h, const HierView<T> &H, int l, const int *ctl)
@@ -65,11 +82,13 @@ OpArgsT<T> view_args(const gputo_handle *h, const HierView<T> &H, int l, const i
 template <class T>
 void level_op(const gputo_handle *h, const HierView<T> &H, int l, int mode, const OpArgsT<T> &a)
 {
-    if (l > 0) {
+    if (l == 1 && H.l1_mf) {
+        launch_op1_mf<T>(h->stream, mode, a, H.CX, h->L[1].tmf, kL1Nu);
+    } else if (l > 0) {
         launch_opA<T>(h->stream, mode, a);
     } else if constexpr (std::is_same<T, float>::value) {
-        if (H.l0 == L0_MFMA_BF16)
-            launch_op0_mfma(h->stream, mode, a, H.l0_ctx);
+        if (H.l0 >= L0_MFMA_F32)
+            launch_op0_mfma(h->stream, mode, a, H.l0_ctx, H.l0 - L0_MFMA_F32);
         else
             launch_op0<T>(h->stream, mode, a);
     } else {

@@ -25,6 +25,18 @@ PREC_F64 = 0
 PREC_MIXED_F32 = 1
 PREC_MIXED_BF16 = 2
 
+# level-0 kernels (gputo_params.l0_kernel, gputo_matvec_mixed, gputo_bench_operator)
+L0_F64_GATHER = 0
+L0_GATHER = 1
+L0_MFMA_F32 = 2
+L0_MFMA_BF16 = 3
+L0_MFMA_BF16X3 = 4
+
+# level-1 operator (gputo_params.l1_kernel)
+L1_AUTO = 0
+L1_STORED = 1
+L1_MATRIX_FREE = 2
+
 
 class GputoParams(ctypes.Structure):
     _fields_ = [
@@ -33,6 +45,8 @@ class GputoParams(ctypes.Structure):
         ("learned", ctypes.c_int),
         ("inner_tol", ctypes.c_double),
         ("print_level", ctypes.c_int),
+        ("l0_kernel", ctypes.c_int),
+        ("l1_kernel", ctypes.c_int),
     ]
 
 
@@ -103,6 +117,8 @@ SIGNATURES = {
     "gputo_interp_f64": (_I, [_P, _I, _P, _P]),
     "gputo_coarse_solve_f64": (_I, [_P, _P, _P]),
     "gputo_bench_operator": (_I, [_P, _I, _I, _pD]),
+    "gputo_level_apply": (_I, [_P, _I, _I, _I, _I, _P, _P, _P]),
+    "gputo_bench_level": (_I, [_P, _I, _I, _I, _I, _pD]),
     "gputo_edofmat": (_I, [_P, _P]),
     "gputo_fixed_list": (_I, [_P, _I, _P, _pI]),
     "gputo_to_begin": (_I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(ToConfig)]),

@@ -27,7 +27,7 @@ class Solver:
     """One libgputo handle for an nelx x nely x nelz mesh with nl levels."""
 
     def __init__(self, nelx, nely, nelz, nl, preset=_lib.PRESET_REF_C, precision=_lib.PREC_F64, learned=False,
-                 inner_tol=1e-4, print_level=0):
+                 inner_tol=1e-6, print_level=0, l0_kernel=0, l1_kernel=0):
         self.lib = _lib.load()
         self.nelx, self.nely, self.nelz, self.nl = int(nelx), int(nely), int(nelz), int(nl)
         self.nel = self.nelx * self.nely * self.nelz
@@ -39,6 +39,8 @@ class Solver:
         self.params.learned = int(bool(learned))
         self.params.inner_tol = inner_tol
         self.params.print_level = print_level
+        self.params.l0_kernel = l0_kernel
+        self.params.l1_kernel = l1_kernel
         h = ctypes.c_void_p()
         check(self.lib.gputo_create(self.nelx, self.nely, self.nelz, self.nl, ctypes.byref(self.params),
                                     ctypes.byref(h)), "gputo_create")
@@ -81,10 +83,10 @@ class Solver:
                                         ptr(y)), "gputo_matvec_f64")
         return y
 
-    def matvec_mixed(self, precision, Ke0, CX, u, fixed):
+    def matvec_mixed(self, kernel, Ke0, CX, u, fixed):
         y = np.zeros(self.ndof)
         fixed = f64(fixed).ravel()
-        check(self.lib.gputo_matvec_mixed(self.h, precision, ptr(f64(Ke0)), ptr(f64(CX)), ptr(f64(u)), ptr(fixed),
+        check(self.lib.gputo_matvec_mixed(self.h, kernel, ptr(f64(Ke0)), ptr(f64(CX)), ptr(f64(u)), ptr(fixed),
                                           len(fixed), ptr(y)), "gputo_matvec_mixed")
         return y
 
@@ -126,6 +128,21 @@ class Solver:
         fd = np.zeros(max(nfd, 1))
         check(self.lib.gputo_level_matrices(self.h, level, ptr(A), ptr(ad), ptr(fd)), "gputo_level_matrices")
         return (None if A is None else A.reshape(nel, 24, 24)), ad, fd[:nfd]
+
+    def level_apply(self, level, op, mode, u, f=None, fp32=False):
+        """One level-`level` operator application (gputo_level_apply): mode 0
+        y = A u, 1 y = f - A u, 2 one damped-Jacobi sweep; op 1 stored, 2
+        matrix-free (level 1)."""
+        y = np.zeros(len(u))
+        check(self.lib.gputo_level_apply(self.h, level, op, mode, int(bool(fp32)), ptr(f64(u)),
+                                         None if f is None else ptr(f64(f)), ptr(y)), "gputo_level_apply")
+        return y
+
+    def bench_level(self, level, op, fp32=False, reps=50):
+        avg = ctypes.c_double()
+        check(self.lib.gputo_bench_level(self.h, level, op, int(bool(fp32)), reps, ctypes.byref(avg)),
+              "gputo_bench_level")
+        return avg.value
 
     def fixed_list(self, level):
         n = ctypes.c_int()

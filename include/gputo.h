@@ -62,9 +62,17 @@ typedef struct gputo_params {
     int learned;       /* 1: apply the learned level-2 correction inside flexible CG */
     double inner_tol;  /* relative tolerance of one inner solve (mixed precision) */
     int print_level;   /* 0 silent; >=1 summary on stderr; >=2 per-iteration */
+    int l0_kernel;     /* level-0 operator of the inner solve: 0 by precision, 1 FP32
+                          node-gather, 2 FP32 MFMA, 3 split-BF16 MFMA (hi+lo),
+                          4 split-BF16 MFMA (hi+mid+lo) */
+    int l1_kernel;     /* level-1 operator in the V-cycle: 0 by preset (matrix-free
+                          where level 1 is rediscretised from the child densities,
+                          stored element matrices otherwise), 1 stored element
+                          matrices, 2 matrix-free (rediscretised level 1 only) */
 } gputo_params;
 
-/* Fill *p with the defaults (REF_C, F64, no learned stage, inner_tol 1e-4). */
+/* Fill *p with the defaults (REF_C, F64, no learned stage, inner_tol 1e-6,
+ * level-0 kernel chosen by precision). */
 void gputo_default_params(gputo_params *p);
 
 /* Create a solver for an nelx x nely x nelz hex mesh with nl multigrid
@@ -104,13 +112,14 @@ long long gputo_device_bytes(const gputo_handle *h);
 /* y = K(CX) u with y[fixed] = 0 (multAV0c + ZeroBCc, mgcg1.cu:941-997,1326). */
 int gputo_matvec_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *u,
                      const double *fixed, int nfd, double *y);
-/* The same product with the reduced-precision level-0 operator of the
- * mixed-precision solve (precision GPUTO_PREC_MIXED_F32: FP32 node-gather;
- * GPUTO_PREC_MIXED_BF16: split-BF16 MFMA), FP64 in and out. */
-int gputo_matvec_mixed(gputo_handle *h, int precision, const double *Ke0, const double *CX, const double *u,
+/* The same product with a reduced-precision level-0 kernel of the
+ * mixed-precision solve, FP64 in and out: kernel 1 FP32 node-gather, 2 FP32
+ * MFMA, 3 split-BF16 MFMA (hi+lo), 4 split-BF16 MFMA (hi+mid+lo). */
+int gputo_matvec_mixed(gputo_handle *h, int kernel, const double *Ke0, const double *CX, const double *u,
                        const double *fixed, int nfd, double *y);
-/* Lane-layout check of the BF16 MFMA instruction the level-0 kernel uses:
- * *max_err = max |D - A B| on a small exact integer product (0 expected). */
+/* Lane-layout check of the BF16 and FP32 MFMA instructions the level-0
+ * kernels use: *max_err = max |D - A B| on small exact integer products
+ * (0 expected). */
 int gputo_selftest_mfma(double *max_err);
 /* r = F - K(CX) u, r[fixed] = 0 (defect0c + ZeroBCc, mgcg1.cu:1208-1264). */
 int gputo_defect_f64(gputo_handle *h, const double *Ke0, const double *CX, const double *F,
@@ -133,10 +142,23 @@ int gputo_level_matrices(gputo_handle *h, int level, double *A, double *ad, doub
 int gputo_restrict_f64(gputo_handle *h, int level, const double *r_fine, double *f_coarse);
 int gputo_interp_f64(gputo_handle *h, int level, const double *u_coarse, double *u_fine);
 /* Mean duration (ms, HIP events on the handle's stream) of `reps`
- * back-to-back launches of the level-0 operator y = K(CX) u of the given
- * precision on the hierarchy of the last solve / setup (microbenchmark of
- * multAV0c, mgcg1.cu:941-997). */
-int gputo_bench_operator(gputo_handle *h, int precision, int reps, double *avg_ms);
+ * back-to-back launches of the level-0 operator y = K(CX) u on the
+ * hierarchy of the last solve / setup (microbenchmark of multAV0c,
+ * mgcg1.cu:941-997): kernel 0 FP64 node-gather, 1-4 as gputo_matvec_mixed. */
+int gputo_bench_operator(gputo_handle *h, int kernel, int reps, double *avg_ms);
+/* One application of the operator of level `level` (0 <= level < nl-1) of
+ * the hierarchy of the last solve / setup: y = A u (mode 0, masked; multAV,
+ * Cantilever/omp/mgcg1.c:667-715), y = f - A u (mode 1; defectc,
+ * Cantilever/TOP3D125MGCGCUDA/mgcg1.cu:1266-1324) or one damped-Jacobi
+ * sweep from u with the preset's omega (mode 2; defectc then Jacobc,
+ * mgcg1.cu:1199-1206).  op 1: stored element matrices (level 0: the fine
+ * node-gather); op 2: matrix-free level-1 operator (rediscretised level 1
+ * only).  fp32 != 0 runs the FP32 kernels of the mixed path.  Host vectors. */
+int gputo_level_apply(gputo_handle *h, int level, int op, int mode, int fp32, const double *u, const double *f,
+                      double *y);
+/* Mean duration (ms, HIP events on the handle's stream) of `reps`
+ * back-to-back matvecs of gputo_level_apply's operator. */
+int gputo_bench_level(gputo_handle *h, int level, int op, int fp32, int reps, double *avg_ms);
 /* Coarsest-level solve x = A_c^{-1} b (cholsl1, mgcg1.cu:2175-2232). */
 int gputo_coarse_solve_f64(gputo_handle *h, const double *b, double *x);
 

@@ -1,0 +1,100 @@
+"""Level operators of the hierarchy through gputo_level_apply: the
+matrix-free level-1 operator (Gauss-point form of the rediscretised level-1
+element matrices, coarseAe2c, Cantilever/TOP3D125MGCGCUDA/mgcg1.cu:2042-2070)
+against the oracle's stored level-1 matrices applied element by element
+(multAV, Cantilever/omp/mgcg1.c:667-715), plus size-independent properties:
+zero action on rigid translations and agreement with the stored GPU kernel.
+
+Tolerances (relative to max|y|): FP64 1e-13 (summation order only), FP32
+2e-6 (FP32 rounding of u, CX and the Gauss-point arithmetic).
+"""
+import numpy as np
+import pytest
+
+from conftest import cantilever_case
+
+pytestmark = pytest.mark.gpu
+
+OMEGA_REF_C = 0.48  # damped-Jacobi weight of the reference smoother (Jacobc, mgcg1.cu:1206)
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+MESHES = [(8, 4, 4, 3), (16, 8, 8, 3), (20, 12, 8, 3), (24, 12, 12, 4), (56, 28, 28, 3)]
+
+
+@pytest.mark.parametrize("mesh", MESHES)
+@pytest.mark.parametrize("fp32", [False, True])
+def test_level1_matrix_free_vs_oracle(gputo, oracle, mesh, fp32):
+    nelx, nely, nelz, nl = mesh
+    print("case", mesh, fp32, flush=True)
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=31, kind="smooth")
+    s = gputo.Solver(nelx, nely, nelz, nl, precision=1 if fp32 else 0)
+    s.setup(Ke, CX, fixed)
+    A1, ad1, fd1 = oracle.level_data(Ke, CX, fixed, nelx, nely, nelz, nl, 1)
+    nx, ny, nz = nelx // 2, nely // 2, nelz // 2
+    rng = np.random.default_rng(32)
+    u = rng.uniform(-1, 1, ad1.size)
+    f = rng.uniform(-1, 1, ad1.size)
+    tol = 2e-6 if fp32 else 1e-13
+    y_or = oracle.matvec_el(A1, u, fd1, nx, ny, nz)
+    y = s.level_apply(1, 2, 0, u, fp32=fp32)
+    assert rel(y, y_or) < tol
+    assert np.array_equal(y, s.level_apply(1, 2, 0, u, fp32=fp32))  # deterministic
+    # defect f - A u, zero at fixed DOFs
+    r_or = f - y_or
+    r_or[fd1.astype(np.int64) - 1] = 0.0
+    r = s.level_apply(1, 2, 1, u, f, fp32=fp32)
+    assert rel(r, r_or) < tol
+    # one damped-Jacobi sweep (1-w) u + w (r + AD u)/AD
+    j_or = (1 - OMEGA_REF_C) * u + OMEGA_REF_C * (r_or + ad1 * u) / ad1
+    j = s.level_apply(1, 2, 2, u, f, fp32=fp32)
+    assert rel(j, j_or) < tol
+    # the stored-matrix kernel computes the same operator
+    assert rel(s.level_apply(1, 1, 0, u, fp32=fp32), y_or) < tol
+    s.close()
+
+
+def test_level1_matrix_free_rigid_translation(gputo):
+    nelx, nely, nelz, nl = 24, 12, 12, 3
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=33, kind="smooth")
+    s = gputo.Solver(nelx, nely, nelz, nl)
+    s.setup(Ke, CX, fixed)  # rows of A sum to zero per direction; fixed rows are zeroed
+    n1 = 3 * (nelx // 2 + 1) * (nely // 2 + 1) * (nelz // 2 + 1)
+    scale = np.abs(s.level_apply(1, 2, 0, np.random.default_rng(34).uniform(-1, 1, n1))).max()
+    for d in range(3):
+        t = np.zeros(n1)
+        t[d::3] = 1.0
+        assert np.abs(s.level_apply(1, 2, 0, t)).max() < 1e-13 * scale
+    s.close()
+
+
+def test_level1_kernel_choice_same_solve(gputo):
+    """The V-cycle with the matrix-free level 1 and with the stored level-1
+    matrices give the same PCG solve (iteration counts within one, U to 1e-9)."""
+    nelx, nely, nelz, nl = 32, 16, 16, 4
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=35, kind="smooth")
+    out = []
+    for l1 in (1, 2):
+        s = gputo.Solver(nelx, nely, nelz, nl, l1_kernel=l1)
+        U, r, n, rc = s.solve(Ke, F, CX, fixed, 1e-10, 200)
+        assert rc == 0
+        out.append((U, n))
+        s.close()
+    assert abs(out[0][1] - out[1][1]) <= 1
+    assert rel(out[1][0], out[0][0]) < 1e-9
+
+
+def test_level_apply_rejects_bad_arguments(gputo):
+    nelx, nely, nelz, nl = 8, 4, 4, 3
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=36, kind="smooth")
+    s = gputo.Solver(nelx, nely, nelz, nl, preset=gputo.PRESET_MATLAB_CFG0)
+    s.setup(Ke, CX, fixed)
+    n1 = 3 * 5 * 3 * 3
+    with pytest.raises(gputo.GputoError):
+        s.level_apply(1, 2, 0, np.zeros(n1))  # Galerkin level 1 has no matrix-free form
+    with pytest.raises(gputo.GputoError):
+        s.level_apply(2, 1, 0, np.zeros(3 * 3 * 2 * 2))  # the coarsest level is solved directly
+    s.close()
