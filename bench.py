@@ -39,7 +39,7 @@ def run_variant(gputo, name, args, Ke, F, fixed, x0):
     prec, kernel = VARIANTS[name.replace("+learned", "")]
     learned = name.endswith("+learned")
     s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=learned,
-                     inner_tol=args.inner_tol, l0_kernel=kernel)
+                     l0_kernel=kernel)
     run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
                        cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
     for w in range(args.warmup):
@@ -113,7 +113,6 @@ def main():
     ap.add_argument("--nl", type=int, default=5)
     ap.add_argument("--cg-tol", type=float, default=1e-8)
     ap.add_argument("--cg-maxit", type=int, default=300)
-    ap.add_argument("--inner-tol", type=float, default=1e-6)
     ap.add_argument("--variants", default="f64")
     ap.add_argument("--op-reps", type=int, default=50)
     ap.add_argument("--seed", type=int, default=0)

@@ -2,6 +2,10 @@
 // densities, element-Galerkin operators, the coarsest-level penalty,
 // dense assembly, blocked FP64 Cholesky, the explicit inverse of the
 // factor and the two triangular products that apply it in every V-cycle.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "gputo_internal.h"
 
 namespace gputo {
@@ -124,6 +128,163 @@ void launch_coarse_galerkin(hipStream_t s, const Grid &gc, const double *Af, con
 }
 
 // ---------------------------------------------------------------------------
+// Element-Galerkin operators from per-fine-element tables.  The Galerkin
+// product of level l (coarseAe3c, mgcg1.cu:1653-1743: A_c = P^T (sum of the 8
+// children) P with the trilinear P of mgcg1.cu:1653-1676) is linear in the
+// level-0 element scales, so the level-l element matrix is
+//   A_e = sum_f CX[f] T_l[f],   f over the 8^l level-0 elements inside e,
+// with T_l[f] = P_chain(f)^T B(f) P_chain(f) fixed per fine position: B = Ke
+// when level 1 is itself Galerkin, B = the Gauss-point matrix of f's position
+// in its level-1 parent when level 1 is rediscretised (coarseAe2c,
+// mgcg1.cu:2042-2070).  The tables are built once on the host; the device
+// work per level is one [nel_l x 8^l] x [8^l x 300] product.  Same operator
+// as the child-by-child product, summed in a different order.
+// ---------------------------------------------------------------------------
+constexpr int kGtE = 16;  // elements per block
+constexpr int kGtF = 16;  // table rows staged per step
+
+template <int KP>
+__global__ __launch_bounds__(256) void k_galerkin_table(Grid gc, int depth, const double *__restrict__ T,
+                                                       const double *__restrict__ CX0, double *__restrict__ A)
+{
+    constexpr int PW = 16 * KP;  // packed entries per block
+    __shared__ double sT[kGtF][PW];
+    __shared__ double sC[kGtE][kGtF + 1];
+    const int nchunk = (300 + PW - 1) / PW;
+    const long eb = blockIdx.x / nchunk;
+    const int p0 = (blockIdx.x % nchunk) * PW;
+    const int el = threadIdx.x / 16, pl = threadIdx.x % 16;
+    const int S = 1 << depth, nf = S * S * S;
+    const long ny0 = (long)gc.ny * S, nz0 = (long)gc.nz * S;
+    // the element staged by this thread's sC slot (el, ff = pl)
+    const long es = eb * kGtE + el;
+    long ebase = -1;
+    if (es < gc.nel) {
+        const long i = es / ((long)gc.ny * gc.nz), k = (es / gc.ny) % gc.nz, j = es % gc.ny;
+        ebase = (i * S) * nz0 * ny0 + (k * S) * ny0 + j * S;
+    }
+    double acc[KP];
+#pragma unroll
+    for (int q = 0; q < KP; ++q) acc[q] = 0.0;
+    for (int f0 = 0; f0 < nf; f0 += kGtF) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < kGtF * PW; q += 256) {
+            const int ff = q / PW, pp = q % PW, p = p0 + pp;
+            sT[ff][pp] = (f0 + ff < nf && p < 300) ? T[(long)(f0 + ff) * 300 + p] : 0.0;
+        }
+        {
+            const int f = f0 + pl;
+            double c = 0.0;
+            if (ebase >= 0 && f < nf) {
+                const int fj = f % S, fk = (f / S) % S, fi = f / (S * S);
+                c = CX0[ebase + (long)fi * nz0 * ny0 + (long)fk * ny0 + fj];
+            }
+            sC[el][pl] = c;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ff = 0; ff < kGtF; ++ff) {
+            const double c = sC[el][ff];
+#pragma unroll
+            for (int q = 0; q < KP; ++q) acc[q] = fma(c, sT[ff][pl + 16 * q], acc[q]);
+        }
+    }
+    if (es >= gc.nel) return;
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+        const int p = p0 + pl + 16 * q;
+        if (p < 300 && p < p0 + PW) A[(long)p * gc.nel + es] = acc[q];
+    }
+}
+
+void launch_galerkin_table(hipStream_t s, const Grid &gc, int depth, const double *T, const double *CX0, double *A)
+{
+    const long eblocks = (gc.nel + kGtE - 1) / kGtE;
+    // split the 300 entries over more blocks when there are few elements
+    if (eblocks >= 512)
+        k_galerkin_table<19><<<(unsigned)eblocks, 256, 0, s>>>(gc, depth, T, CX0, A);
+    else if (eblocks >= 128)
+        k_galerkin_table<10><<<(unsigned)(eblocks * 2), 256, 0, s>>>(gc, depth, T, CX0, A);
+    else
+        k_galerkin_table<5><<<(unsigned)(eblocks * 4), 256, 0, s>>>(gc, depth, T, CX0, A);
+}
+
+// Host: the tables T_l for l = 1..maxdepth, packed [8^l][300] with the
+// fine position f = (fi*S + fk)*S + fj, S = 2^l.  base1: the 8 full matrices
+// of depth 1 (index 4 fi + 2 fk + fj), or nullptr to start from Ke at depth 0.
+static void child_prolongation(int cx, int cy, int cz, double P[24][24])
+{
+    // P[3a+d][3b+d] = trilinear weight of parent node b at child node a
+    static const int ie[8] = {0, 1, 1, 0, 0, 1, 1, 0}, je[8] = {0, 0, 1, 1, 0, 0, 1, 1}, ke[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    for (int r = 0; r < 24; ++r)
+        for (int c = 0; c < 24; ++c) P[r][c] = 0.0;
+    for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b) {
+            const double w = (1.0 - 0.5 * std::abs(cx + ie[a] - 2 * ie[b])) * (1.0 - 0.5 * std::abs(cy + je[a] - 2 * je[b])) *
+                             (1.0 - 0.5 * std::abs(cz + ke[a] - 2 * ke[b]));
+            for (int d = 0; d < 3; ++d) P[3 * a + d][3 * b + d] = w;
+        }
+}
+
+std::vector<std::vector<double>> galerkin_tables(const double *Ke, const double *base1, int maxdepth)
+{
+    std::vector<std::vector<double>> out(maxdepth + 1);
+    std::vector<double> cur;  // full matrices of the current depth, [S^3][576]
+    int d = 0;
+    if (base1) {
+        cur.assign(8 * 576, 0.0);
+        for (int fi = 0; fi < 2; ++fi)
+            for (int fk = 0; fk < 2; ++fk)
+                for (int fj = 0; fj < 2; ++fj)
+                    std::copy(base1 + 576 * (4 * fi + 2 * fk + fj), base1 + 576 * (4 * fi + 2 * fk + fj) + 576,
+                              cur.begin() + 576 * ((fi * 2 + fk) * 2 + fj));
+        d = 1;
+    } else {
+        cur.assign(Ke, Ke + 576);
+    }
+    auto pack = [](const std::vector<double> &full, int n) {
+        std::vector<double> t((size_t)n * 300);
+        for (int f = 0; f < n; ++f)
+            for (int m = 0; m < 24; ++m)
+                for (int c = 0; c <= m; ++c) t[(size_t)f * 300 + m * (m + 1) / 2 + c] = full[(size_t)f * 576 + 24 * m + c];
+        return t;
+    };
+    if (d >= 1 && d <= maxdepth) out[d] = pack(cur, 8);
+    double P[8][24][24];
+    for (int c = 0; c < 8; ++c) child_prolongation(c & 1, (c >> 1) & 1, c >> 2, P[c]);
+    while (d < maxdepth) {
+        const int S = 1 << d, S2 = 2 * S;
+        std::vector<double> nxt((size_t)S2 * S2 * S2 * 576);
+        double tmp[24][24];
+        for (int fi = 0; fi < S2; ++fi)
+            for (int fk = 0; fk < S2; ++fk)
+                for (int fj = 0; fj < S2; ++fj) {
+                    const int cx = fi / S, cy = fj / S, cz = fk / S;
+                    const double(*Pc)[24] = P[cx + 2 * cy + 4 * cz];
+                    const double *B = &cur[(size_t)(((fi % S) * S + (fk % S)) * S + (fj % S)) * 576];
+                    // tmp = B P, then out = P^T tmp
+                    for (int r = 0; r < 24; ++r)
+                        for (int c = 0; c < 24; ++c) {
+                            double acc = 0.0;
+                            for (int q = 0; q < 24; ++q) acc += B[24 * r + q] * Pc[q][c];
+                            tmp[r][c] = acc;
+                        }
+                    double *o = &nxt[(size_t)((fi * S2 + fk) * S2 + fj) * 576];
+                    for (int r = 0; r < 24; ++r)
+                        for (int c = 0; c < 24; ++c) {
+                            double acc = 0.0;
+                            for (int q = 0; q < 24; ++q) acc += Pc[q][r] * tmp[q][c];
+                            o[24 * r + c] = acc;
+                        }
+                }
+        cur.swap(nxt);
+        ++d;
+        out[d] = pack(cur, S2 * S2 * S2);
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
 // Coarsest-level penalty (CorrCsAc, mgcg1.cu:1368-1402): the diagonal of
 // every incident element matrix is set to 1e10 for each fixed DOF.  The
 // reference's table {5,6,1,2,4,7,0,3} is the y-mirror of the node's own
@@ -204,39 +365,42 @@ void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ)
 // (lower triangle used), NB = 32: factor the diagonal tile in LDS, solve
 // the panel below it, update the trailing lower triangle.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_potrf_diag(double *A, int N, int k0, int *flag)
+__global__ __launch_bounds__(64) void k_potrf_diag(double *A, int N, int k0, int *flag)
 {
-    __shared__ double L[NB][NB + 1];
-    for (int q = threadIdx.x; q < NB * NB; q += 256) {
-        const int r = q / NB, c = q % NB;
-        L[r][c] = c <= r ? A[(long)(k0 + r) * N + k0 + c] : 0.0;
-    }
-    __syncthreads();
+    // one wave: lane r < NB holds row r of the diagonal tile in registers;
+    // column jj's pivot and multipliers are exchanged by lane shuffles, so
+    // the 32 elimination steps need no barrier
+    const int lane = threadIdx.x;
+    const int r = lane < NB ? lane : NB - 1;  // lanes >= NB shadow row NB-1, results dropped
+    double L[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) L[c] = c <= r ? A[(long)(k0 + r) * N + k0 + c] : 0.0;
+    bool bad = false;
+#pragma unroll
     for (int jj = 0; jj < NB; ++jj) {
-        if (threadIdx.x == 0) {
-            double d = L[jj][jj];
-            if (!(d > 0.0)) {  // not positive definite (or NaN)
-                flag[0] = 1;
-                d = 1.0;
-            }
-            L[jj][jj] = sqrt(d);
+        double d = __shfl(L[jj], jj, 64);  // updated A[jj][jj]
+        if (!(d > 0.0)) {  // not positive definite (or NaN)
+            bad = true;
+            d = 1.0;
         }
-        __syncthreads();
-        if (threadIdx.x > jj && threadIdx.x < NB) L[threadIdx.x][jj] /= L[jj][jj];
-        __syncthreads();
-        for (int q = threadIdx.x; q < NB * NB; q += 256) {
-            const int r = q / NB, c = q % NB;
-            if (r > jj && c > jj && c <= r) L[r][c] -= L[r][jj] * L[c][jj];
+        d = sqrt(d);
+        const double lj = r == jj ? d : (r > jj ? L[jj] / d : 0.0);
+        L[jj] = lj;
+#pragma unroll
+        for (int c = jj + 1; c < NB; ++c) {
+            const double lc = __shfl(lj, c, 64);  // L[c][jj]
+            if (c <= r) L[c] -= lj * lc;
         }
-        __syncthreads();
     }
-    for (int q = threadIdx.x; q < NB * NB; q += 256) {
-        const int r = q / NB, c = q % NB;
-        if (c <= r) A[(long)(k0 + r) * N + k0 + c] = L[r][c];
+    if (lane < NB) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c <= r) A[(long)(k0 + r) * N + k0 + c] = L[c];
     }
+    if (bad && lane == 0) flag[0] = 1;
 }
 
-__global__ __launch_bounds__(256) void k_trsm_panel(double *A, int N, int k0)
+__global__ __launch_bounds__(256) void k_trsm_panel(double *A, int N, int k0, int rows)
 {
     __shared__ double L[NB][NB + 1];
     for (int q = threadIdx.x; q < NB * NB; q += 256) {
@@ -245,7 +409,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double *A, int N, int k0)
     }
     __syncthreads();
     const long row = (long)k0 + NB + (long)blockIdx.x * 256 + threadIdx.x;
-    if (row >= N) return;
+    if (row >= (long)k0 + NB + rows) return;
     double x[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) x[c] = A[row * N + k0 + c];
@@ -317,13 +481,15 @@ __global__ __launch_bounds__(64) void k_trtri_diag(const double *A, double *Linv
 }
 
 // Block row ib of Linv: Linv[ib][jb] = -inv(L[ib][ib]) sum_{kb=jb}^{ib-1}
-// L[ib][kb] Linv[kb][jb], one block per jb < ib.
-__global__ __launch_bounds__(256) void k_linv_row(const double *A, double *Linv, int N, int ib)
+// L[ib][kb] Linv[kb][jb], one block per jb < ib; the sum runs over the
+// factor's band only.
+__global__ __launch_bounds__(256) void k_linv_row(const double *A, double *Linv, int N, int ib, int nbt)
 {
     __shared__ double Ta[NB][NB + 1], Tb[NB][NB + 1];
     const int jb = blockIdx.x;
     double acc[4] = {0, 0, 0, 0};
-    for (int kb = jb; kb < ib; ++kb) {
+    // tiles L[ib][kb] with kb < ib - nbt lie outside the band (exact zeros)
+    for (int kb = max(jb, ib - nbt); kb < ib; ++kb) {
         __syncthreads();
         for (int q = threadIdx.x; q < NB * NB; q += 256) {
             const int r = q / NB, c = q % NB;
@@ -376,20 +542,25 @@ int coarse_factor(hipStream_t s, Coarse &C)
 {
     const int N = (C.n + NB - 1) / NB * NB;
     const int nb = N / NB;
+    // the factor of a banded matrix keeps its band: panel rows and trailing
+    // tiles beyond it are exact zeros and are skipped
+    const int bw = std::max(1, std::min(C.bw, N));
+    const int band_rows = (bw + NB - 1) / NB * NB;
     hipMemsetAsync(C.flag, 0, sizeof(int), s);
     for (int kb = 0; kb < nb; ++kb) {
         const int k0 = kb * NB;
-        k_potrf_diag<<<1, 256, 0, s>>>(C.AQ, N, k0, C.flag);
-        const int rows = N - k0 - NB;
+        k_potrf_diag<<<1, 64, 0, s>>>(C.AQ, N, k0, C.flag);
+        const int rows = std::min(N - k0 - NB, band_rows);
         if (rows > 0) {
-            k_trsm_panel<<<(rows + 255) / 256, 256, 0, s>>>(C.AQ, N, k0);
+            k_trsm_panel<<<(rows + 255) / 256, 256, 0, s>>>(C.AQ, N, k0, rows);
             const int nt = rows / NB;
             k_syrk_update<<<nt * (nt + 1) / 2, 256, 0, s>>>(C.AQ, N, k0);
         }
     }
     launch_fill(s, C.Linv, (long)N * N, 0.0);
     k_trtri_diag<<<nb, 64, 0, s>>>(C.AQ, C.Linv, N);
-    for (int ib = 1; ib < nb; ++ib) k_linv_row<<<ib, 256, 0, s>>>(C.AQ, C.Linv, N, ib);
+    const int nbt = (bw + 2 * NB - 2) / NB;
+    for (int ib = 1; ib < nb; ++ib) k_linv_row<<<ib, 256, 0, s>>>(C.AQ, C.Linv, N, ib, nbt);
     k_transpose<<<dim3(nb, nb), 256, 0, s>>>(C.Linv, C.LinvT, N);
     int flag = 0;
     hipMemcpyAsync(&flag, C.flag, sizeof(int), hipMemcpyDeviceToHost, s);

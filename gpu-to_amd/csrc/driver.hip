@@ -509,10 +509,7 @@ int gputo_to_iterate(gputo_handle *h, gputo_to_stats *st)
         check_hip(hipMemcpyAsync(h->CX, t->CX, ne * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
         int rc = setup_hierarchy(h, t->fixed);
         if (rc == 0) {
-            if (h->prm.precision == GPUTO_PREC_F64)
-                rc = run_pcg(h, t->F, t->U, c.cg_tol, c.cg_maxit, &relres, &nit);
-            else
-                rc = mixed_solve(h, t->F, t->U, c.cg_tol, c.cg_maxit, &relres, &nit);
+            rc = run_pcg(h, t->F, t->U, c.cg_tol, c.cg_maxit, &relres, &nit);
         }
         hipEventRecord(e1, s);
         hipEventSynchronize(e1);

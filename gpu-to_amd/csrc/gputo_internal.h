@@ -18,6 +18,7 @@ namespace gputo {
 constexpr int kMaxHalo = 864;
 constexpr int kMaxElemBox = 640;
 constexpr int kBlock = 256;          // threads per block of the gather kernels
+constexpr int kStNodes = kBlock / 8; // nodes per block of the stored-matrix operator
 constexpr int kRedBlocks = 1024;     // blocks of the two-stage reductions
 constexpr int kMaxLevels = 12;
 constexpr int kMaxCoarseDofs = 16384;
@@ -39,7 +40,8 @@ struct Tile {
     long blocks() const { return (long)ntj * ntk * nti; }
 };
 
-Tile choose_tile(const Grid &g);
+Tile choose_tile(const Grid &g);   // level-0 node-gather bricks
+Tile stored_tile(const Grid &g);   // stored-matrix operator (at most kStNodes nodes)
 
 // Arguments of the node-gather operator kernels (T = double or float).
 template <class T>
@@ -73,6 +75,10 @@ enum OpMode {
 struct CgScalars {
     double rho, rho_p, alpha, beta, res0, relres, pq, rr;
     int done, nit, it, bad;
+    // mixed precision: residual replacement state
+    double rmax;     // largest recursive residual norm since the last replacement
+    int noru;        // 0: replace the recursive residual by b - A x this iteration
+    int nru;         // replacements done
 };
 
 struct Level {
@@ -88,6 +94,7 @@ struct Level {
 
 struct Coarse {
     int n = 0;
+    int bw = 0;              // lower bandwidth of the assembled matrix (node-major DOFs)
     double *AQ = nullptr;    // dense matrix, then its Cholesky factor (lower)
     double *Linv = nullptr;  // inverse of the factor (lower, row-major)
     double *LinvT = nullptr; // its transpose (row-major)
@@ -119,6 +126,12 @@ struct gputo_handle {
     std::vector<void *> allocs;
     struct ToState *to = nullptr;
     gputo::MixedState *mx = nullptr;
+    // per-fine-element Galerkin tables of levels 1..gt_max (device, packed
+    // [8^l][300]; null where the level is built from the level above)
+    double *gt[gputo::kMaxLevels] = {};
+    int gt_max = 0;
+    int gt_preset = -1;          // preset the tables were built for (-1: none yet)
+    std::vector<double> gt_ke;  // Ke they were built from (Galerkin level 1)
 };
 
 namespace gputo {
@@ -140,6 +153,9 @@ template <class Ti, class To> void launch_copy(hipStream_t s, const Ti *src, To 
 void launch_coarse_l1(hipStream_t s, const Grid &gc, const double *Aelm, const double *CXf, double *A);
 void launch_coarse_galerkin(hipStream_t s, const Grid &gc, const double *Af, const double *Ke, const double *CXf,
                             double *Ac);
+void launch_galerkin_table(hipStream_t s, const Grid &gc, int depth, const double *T, const double *CX0, double *A);
+std::vector<std::vector<double>> galerkin_tables(const double *Ke, const double *base1, int maxdepth);
+constexpr int kGtMaxDepth = 4;  // deepest level built from tables (8^4 x 300 doubles = 9.8 MB)
 void launch_corrcsa(hipStream_t s, const Grid &g, const int *fixed_d, int nfd, double *A, int mirrored);
 void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ);
 int coarse_factor(hipStream_t s, Coarse &C);
@@ -157,6 +173,25 @@ void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int np
 template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc);
 
 void check_hip(hipError_t e, const char *what);
+
+// Deterministic sum over a kBlock-thread block: fixed butterfly per wave,
+// then the wave sums in order.  Every thread of the block must call it.
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double ws[kBlock / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) ws[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) s += ws[i];
+    return s;
+}
+
+
 void to_release(gputo_handle *h);  // frees the design-loop state (driver.hip)
 
 }  // namespace gputo

@@ -25,44 +25,26 @@ __host__ __device__ constexpr int lidx(int x, int y, int z)
 
 __host__ __device__ constexpr int pidx(int m, int n) { return n <= m ? m * (m + 1) / 2 + n : n * (n + 1) / 2 + m; }
 
-__device__ __forceinline__ double block_sum(double v)
-{
-    // deterministic sum over a 256-thread block: fixed butterfly per wave,
-    // then the 4 wave sums in order
-    __shared__ double ws[kBlock / 64];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) ws[w] = v;
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < kBlock / 64; ++i) s += ws[i];
-    return s;
-}
 
 // ---------------------------------------------------------------------------
-// Node-gather operator.  One block owns a TJ x TK x TI brick of nodes; it
-// stages the nodal values of the brick plus a one-node halo in LDS
-// (structure of arrays, one plane per component) and, for level 0, the
-// element scales of the (TJ+1)(TK+1)(TI+1) element box.  Each thread then
-// forms the 3 rows of K U at its node from the 27 neighbouring nodes:
+// Level-0 node-gather operator (multAV0c / defect0c, Cantilever/
+// TOP3D125MGCGCUDA/mgcg1.cu:941-997,1208-1264).  One block owns a
+// TJ x TK x TI brick of nodes; it stages the nodal values of the brick plus a
+// one-node halo in LDS (structure of arrays, one plane per component), the
+// element scales of the (TJ+1)(TK+1)(TI+1) element box and Ke.  Each thread
+// then forms the 3 rows of K U at its node from the 27 neighbouring nodes:
 //   y_c = sum_e E_e sum_b Ke[3a_e + c][3b + d] u_{b,d}
-// with the element products accumulated per incident element (z_e) so that
-// every neighbour value is read from LDS once and scaled once per element.
-// Ke indices are compile-time constants, so Ke is read through the scalar
-// cache.  No atomics: each output DOF is written by exactly one thread.
+// one incident element at a time (Ke rows wave-uniform LDS broadcasts).
+// No atomics: each output DOF is written by exactly one thread.
 // ---------------------------------------------------------------------------
-template <class T, int MODE, bool STORED>
+template <class T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
 {
     if (a.ctl && a.ctl[0]) return;
     __shared__ T su[3][kMaxHalo];
-    __shared__ T sE[STORED ? 1 : kMaxElemBox];
-    __shared__ T sK[STORED ? 1 : 576];  // Ke, read with wave-uniform (broadcast) LDS loads
-    if (!STORED)
-        for (int q = threadIdx.x; q < 576; q += kBlock) sK[q] = a.Ke[q];
+    __shared__ T sE[kMaxElemBox];
+    __shared__ T sK[576];  // Ke, read with wave-uniform (broadcast) LDS loads
+    for (int q = threadIdx.x; q < 576; q += kBlock) sK[q] = a.Ke[q];
     const Grid g = a.g;
     const Tile t = a.t;
     int b = blockIdx.x;
@@ -91,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
         su[c][(hi * HK + hk) * HJ + hj] = v;
     }
     const int EJ = t.TJ + 1, EK = t.TK + 1;
-    if (!STORED) {
+    {
         const int EI = t.TI + 1;
         for (int q = threadIdx.x; q < EJ * EK * EI; q += kBlock) {
             const int ej = q % EJ;
@@ -116,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
         if (gi < g.nnx && gj < g.nny && gk < g.nnz) {
             const int hb = (ti * HK + tk) * HJ + tj;  // halo index of neighbour offset (0,0,0)
             T y0 = T(0), y1 = T(0), y2 = T(0);
-            if (!STORED) {
+            {
                 // one incident element at a time (runtime loop keeps the live
                 // set small): t = Ke[3a..3a+2, :] u_e, y += E_e t, with the
                 // node's local index a and the Ke rows uniform across the wave
@@ -149,39 +131,6 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
                     y1 = fma(E, t1, y1);
                     y2 = fma(E, t2, y2);
                 }
-            } else {
-                const long nel = g.nel;
-                const T *__restrict__ A = a.A;
-#pragma unroll
-                for (int ei = 0; ei < 2; ++ei)
-#pragma unroll
-                    for (int ek = 0; ek < 2; ++ek)
-#pragma unroll
-                        for (int ej = 0; ej < 2; ++ej) {
-                            const int ex = gi - 1 + ei, ey = gj - 1 + ej, ez = gk - 1 + ek;
-                            if (ex < 0 || ex >= g.nx || ey < 0 || ey >= g.ny || ez < 0 || ez >= g.nz) continue;
-                            const T *__restrict__ Ae = A + ((long)g.ny * g.nz * ex + (long)g.ny * ez + ey);
-                            const int la = lidx(1 - ei, 1 - ej, 1 - ek);
-#pragma unroll
-                            for (int bx = 0; bx < 2; ++bx)
-#pragma unroll
-                                for (int bz = 0; bz < 2; ++bz)
-#pragma unroll
-                                    for (int by = 0; by < 2; ++by) {
-                                        const int h = hb + ((ei + bx) * HK + (ek + bz)) * HJ + (ej + by);
-                                        const T u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
-                                        const int lb = lidx(bx, by, bz);
-                                        y0 += Ae[pidx(3 * la + 0, 3 * lb + 0) * nel] * u0 +
-                                              Ae[pidx(3 * la + 0, 3 * lb + 1) * nel] * u1 +
-                                              Ae[pidx(3 * la + 0, 3 * lb + 2) * nel] * u2;
-                                        y1 += Ae[pidx(3 * la + 1, 3 * lb + 0) * nel] * u0 +
-                                              Ae[pidx(3 * la + 1, 3 * lb + 1) * nel] * u1 +
-                                              Ae[pidx(3 * la + 1, 3 * lb + 2) * nel] * u2;
-                                        y2 += Ae[pidx(3 * la + 2, 3 * lb + 0) * nel] * u0 +
-                                              Ae[pidx(3 * la + 2, 3 * lb + 1) * nel] * u1 +
-                                              Ae[pidx(3 * la + 2, 3 * lb + 2) * nel] * u2;
-                                    }
-                        }
             }
             const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
             const long dof = 3 * nod;
@@ -219,29 +168,132 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
     }
 }
 
-template <class T, bool STORED>
-static void launch_gather(hipStream_t s, int mode, const OpArgsT<T> &a)
+template <class T>
+void launch_op0(hipStream_t s, int mode, const OpArgsT<T> &a)
 {
     const dim3 grid((unsigned)a.t.blocks()), block(kBlock);
     switch (mode) {
-    case OP_MATVEC: k_gather<T, OP_MATVEC, STORED><<<grid, block, 0, s>>>(a); break;
-    case OP_DEFECT: k_gather<T, OP_DEFECT, STORED><<<grid, block, 0, s>>>(a); break;
-    case OP_JACOBI: k_gather<T, OP_JACOBI, STORED><<<grid, block, 0, s>>>(a); break;
-    case OP_PRE_JACOBI: k_gather<T, OP_PRE_JACOBI, STORED><<<grid, block, 0, s>>>(a); break;
-    case OP_PRE_DEFECT: k_gather<T, OP_PRE_DEFECT, STORED><<<grid, block, 0, s>>>(a); break;
+    case OP_MATVEC: k_gather<T, OP_MATVEC><<<grid, block, 0, s>>>(a); break;
+    case OP_DEFECT: k_gather<T, OP_DEFECT><<<grid, block, 0, s>>>(a); break;
+    case OP_JACOBI: k_gather<T, OP_JACOBI><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_JACOBI: k_gather<T, OP_PRE_JACOBI><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_DEFECT: k_gather<T, OP_PRE_DEFECT><<<grid, block, 0, s>>>(a); break;
     default: break;
     }
 }
 
-template <class T>
-void launch_op0(hipStream_t s, int mode, const OpArgsT<T> &a)
+// ---------------------------------------------------------------------------
+// Operator of a level with stored element matrices (levels >= 1; the
+// element-matrix products of defectc / multAV, Cantilever/TOP3D125MGCGCUDA/
+// mgcg1.cu:1266-1324, Cantilever/omp/mgcg1.c:667-715).  A block holds
+// kStNodes nodes x 8 element slots: thread (slot, node) forms the 3 rows of
+// the node's local index in the incident element at offset slot, from that
+// element's packed matrix.  The slot is uniform over each half-wave and nodes
+// run along j, so every packed entry is read as coalesced runs of
+// neighbouring elements; the 8 slot sums of a node are added in slot order
+// (deterministic).  Eight times the threads of a node-per-thread gather keeps
+// the small coarse levels, whose cost is load latency, occupied.
+// ---------------------------------------------------------------------------
+template <class T, int MODE>
+__device__ __forceinline__ T stored_u(const OpArgsT<T> &a, long dof)
 {
-    launch_gather<T, false>(s, mode, a);
+    if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT) return T(a.prerelax) * a.F[dof] / a.AD[dof];
+    return a.U[dof];
 }
+
+template <class T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_stored(OpArgsT<T> a)
+{
+    if (a.ctl && a.ctl[0]) return;
+    __shared__ T red[8][3][kStNodes];
+    const Grid g = a.g;
+    const Tile t = a.t;
+    int b = blockIdx.x;
+    const int bj = b % t.ntj;
+    b /= t.ntj;
+    const int bk = b % t.ntk;
+    const int bi = b / t.ntk;
+    const int slot = threadIdx.x / kStNodes, n = threadIdx.x % kStNodes;
+    const int tj = n % t.TJ, tk = (n / t.TJ) % t.TK, ti = n / (t.TJ * t.TK);
+    const int gi = bi * t.TI + ti, gj = bj * t.TJ + tj, gk = bk * t.TK + tk;
+    const bool node_ok = n < t.TJ * t.TK * t.TI && gi < g.nnx && gj < g.nny && gk < g.nnz;
+    T y0 = T(0), y1 = T(0), y2 = T(0);
+    {
+        const int ei = slot >> 2, ek = (slot >> 1) & 1, ej = slot & 1;
+        const int ex = gi - 1 + ei, ey = gj - 1 + ej, ez = gk - 1 + ek;
+        if (node_ok && ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz) {
+            const long nel = g.nel;
+            const T *__restrict__ Ae = a.A + ((long)g.ny * g.nz * ex + (long)g.ny * ez + ey);
+            const int la = lidx(1 - ei, 1 - ej, 1 - ek);
+#pragma unroll
+            for (int lb = 0; lb < 8; ++lb) {
+                const int bx = (lb == 1 || lb == 2 || lb == 5 || lb == 6) ? 1 : 0;
+                const int by = (lb == 2 || lb == 3 || lb == 6 || lb == 7) ? 1 : 0;
+                const int bz = lb >= 4 ? 1 : 0;
+                const long dof = 3 * ((long)g.nny * g.nnz * (ex + bx) + (long)g.nny * (ez + bz) + (ey + by));
+                const T u0 = stored_u<T, MODE>(a, dof), u1 = stored_u<T, MODE>(a, dof + 1),
+                        u2 = stored_u<T, MODE>(a, dof + 2);
+                y0 += Ae[pidx(3 * la + 0, 3 * lb + 0) * nel] * u0 + Ae[pidx(3 * la + 0, 3 * lb + 1) * nel] * u1 +
+                      Ae[pidx(3 * la + 0, 3 * lb + 2) * nel] * u2;
+                y1 += Ae[pidx(3 * la + 1, 3 * lb + 0) * nel] * u0 + Ae[pidx(3 * la + 1, 3 * lb + 1) * nel] * u1 +
+                      Ae[pidx(3 * la + 1, 3 * lb + 2) * nel] * u2;
+                y2 += Ae[pidx(3 * la + 2, 3 * lb + 0) * nel] * u0 + Ae[pidx(3 * la + 2, 3 * lb + 1) * nel] * u1 +
+                      Ae[pidx(3 * la + 2, 3 * lb + 2) * nel] * u2;
+            }
+        }
+    }
+    red[slot][0][n] = y0;
+    red[slot][1][n] = y1;
+    red[slot][2][n] = y2;
+    __syncthreads();
+    double part = 0.0;
+    if (slot == 0 && node_ok) {
+        const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
+        const long dof = 3 * nod;
+        const unsigned m = a.mask[nod];
+        const T om = T(a.omega), one = T(1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T yc = red[0][c][n];
+#pragma unroll
+            for (int q = 1; q < 8; ++q) yc += red[q][c][n];
+            const bool fixed = (m >> c) & 1u;
+            const T u = stored_u<T, MODE>(a, dof + c);
+            if (MODE == OP_MATVEC) {
+                const T v = fixed ? T(0) : yc;
+                a.out[dof + c] = v;
+                part += (double)u * (double)v;
+            } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
+                a.out[dof + c] = fixed ? T(0) : a.F[dof + c] - yc;
+                if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = u;
+            } else {
+                const T f = a.F[dof + c];
+                const T rv = fixed ? T(0) : f - yc;
+                const T ad = a.AD[dof + c];
+                const T un = (one - om) * u + om * (rv + ad * u) / ad;
+                a.out[dof + c] = un;
+                part += (double)f * (double)un;
+            }
+        }
+    }
+    if (a.partials) {
+        const double s = block_sum(part);
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+    }
+}
+
 template <class T>
 void launch_opA(hipStream_t s, int mode, const OpArgsT<T> &a)
 {
-    launch_gather<T, true>(s, mode, a);
+    const dim3 grid((unsigned)a.t.blocks()), block(kBlock);
+    switch (mode) {
+    case OP_MATVEC: k_stored<T, OP_MATVEC><<<grid, block, 0, s>>>(a); break;
+    case OP_DEFECT: k_stored<T, OP_DEFECT><<<grid, block, 0, s>>>(a); break;
+    case OP_JACOBI: k_stored<T, OP_JACOBI><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_JACOBI: k_stored<T, OP_PRE_JACOBI><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_DEFECT: k_stored<T, OP_PRE_DEFECT><<<grid, block, 0, s>>>(a); break;
+    default: break;
+    }
 }
 template void launch_op0<double>(hipStream_t, int, const OpArgsT<double> &);
 template void launch_op0<float>(hipStream_t, int, const OpArgsT<float> &);

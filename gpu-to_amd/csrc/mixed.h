@@ -28,7 +28,9 @@ struct MixedState {
 void mixed_create(gputo_handle *h);
 void mixed_destroy(gputo_handle *h);
 void mixed_stats(gputo_handle *h, gputo_to_stats *st);
-int mixed_solve(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit);
+void mixed_begin(gputo_handle *h);                                        // FP32 hierarchy copies
+void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl);  // z = V32(r)
+void mixed_end(gputo_handle *h, int nit, int converged);
 int mixed_apply_l0(gputo_handle *h, int kind, const double *u, double *y);
 int mixed_bench_operator(gputo_handle *h, int kind, int reps, double *avg_ms);
 HierView<float> mixed_level_view(gputo_handle *h);

@@ -67,6 +67,18 @@ Tile choose_tile(const Grid &g)
     return t;
 }
 
+Tile stored_tile(const Grid &g)
+{
+    Tile t;
+    t.TJ = balance(g.nny, kStNodes);
+    t.TK = balance(g.nnz, std::max(1, kStNodes / t.TJ));
+    t.TI = balance(g.nnx, std::max(1, kStNodes / (t.TJ * t.TK)));
+    t.ntj = (g.nny + t.TJ - 1) / t.TJ;
+    t.ntk = (g.nnz + t.TK - 1) / t.TK;
+    t.nti = (g.nnx + t.TI - 1) / t.TI;
+    return t;
+}
+
 // ---- Gauss-point element stiffness (h = 2, level-1 rediscretisation) ----
 // B^T D B |J| at one 2x2x2 Gauss point of a cube of side h in the element
 // order ie/je/ke; rows of B: exx, eyy, ezz, gxy, gyz, gzx (the formulation
@@ -224,6 +236,43 @@ bool all_finite(const double *v, long n)
 
 namespace gputo {
 
+// (Re)build the Galerkin tables of the current preset: once for a
+// rediscretised level 1 (they depend only on its Gauss-point matrices),
+// and whenever Ke changes when level 1 is itself Galerkin.
+static void ensure_tables(gputo_handle *h)
+{
+    const bool gal1 = preset_of(h->prm.preset).l1_galerkin;
+    if (h->gt_max < (gal1 ? 1 : 2)) return;
+    std::vector<double> ke;
+    if (gal1) {
+        ke.resize(576);
+        HIPCHK(hipMemcpyAsync(ke.data(), h->Ke, 576 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (h->gt_preset == h->prm.preset && ke == h->gt_ke) return;
+    } else if (h->gt_preset == h->prm.preset) {
+        return;
+    }
+    std::vector<std::vector<double>> t;
+    if (gal1) {
+        t = galerkin_tables(ke.data(), nullptr, h->gt_max);
+    } else {
+        std::vector<double> base(8 * 576);
+        const double g[2] = {-1.0 / std::sqrt(3.0), 1.0 / std::sqrt(3.0)};
+        for (int iP = 0; iP < 2; ++iP)
+            for (int kP = 0; kP < 2; ++kP)
+                for (int jP = 0; jP < 2; ++jP)
+                    gauss_point_matrix(kL1Nu, 2.0, g[iP], g[jP], g[kP], &base[576 * (4 * iP + 2 * kP + jP)]);
+        t = galerkin_tables(nullptr, base.data(), h->gt_max);
+    }
+    for (int l = 1; l <= h->gt_max; ++l)
+        if (!t[l].empty())
+            HIPCHK(hipMemcpyAsync(h->gt[l], t[l].data(), t[l].size() * sizeof(double), hipMemcpyHostToDevice,
+                                  h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));  // t is a temporary
+    h->gt_preset = h->prm.preset;
+    h->gt_ke = ke;
+}
+
 // Build the hierarchy for the current Ke, CX (device) and fixed list
 // (initVcycle, mgcg1.cu:326-466).
 int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
@@ -232,6 +281,7 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
     hipStream_t s = h->stream;
     Level &L0 = h->L[0];
     if (int rc = validate_fixed(fixed0, L0.g.ndof)) return rc;
+    ensure_tables(h);
     L0.fixed = fixed0;
     for (int l = 1; l < h->nl; ++l) h->L[l].fixed = restrict_fixed(h->L[l - 1].fixed, h->L[l].g);
     for (int l = 0; l < h->nl; ++l) {
@@ -247,6 +297,8 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
         Level &v = h->L[l];
         if (l == 1 && !p.l1_galerkin)
             launch_coarse_l1(s, v.g, h->Aelm, h->CX, v.A);
+        else if (l <= h->gt_max)
+            launch_galerkin_table(s, v.g, l, h->gt[l], h->CX, v.A);
         else
             launch_coarse_galerkin(s, v.g, l == 1 ? nullptr : h->L[l - 1].A, h->Ke, h->CX, v.A);
         if (l == h->nl - 1) {
@@ -263,9 +315,19 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
 }
 
 // Device-resident PCG (mgcg1.cu:235-280); U, F on the device.
+// z = M r: the FP64 V-cycle, or the FP32 V-cycle of the mixed precisions
+static void precond(gputo_handle *h, const double *r, double *z)
+{
+    if (h->prm.precision == GPUTO_PREC_F64)
+        vcycle(h, r, z);
+    else
+        mixed_vcycle(h, r, z, &h->sc->done);
+}
+
 int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit)
 {
     hipStream_t s = h->stream;
+    if (h->prm.precision != GPUTO_PREC_F64) mixed_begin(h);
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
@@ -274,7 +336,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     launch_copy(s, F, h->R, n, nullptr);
     launch_dot_partial(s, F, F, n, h->partials, nullptr);
     launch_cg_init(s, h->sc, h->partials, kRedBlocks);
-    vcycle(h, h->R, h->P);
+    precond(h, h->R, h->P);
     launch_copy(s, h->P, h->Z, n, ctl);
     launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
     launch_cg_rho0(s, h->sc, h->partials, kRedBlocks);
@@ -290,7 +352,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
             launch_cg_alpha(s, h->sc, h->partials, nmv);
             launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials);
             launch_cg_check(s, h->sc, h->partials, kRedBlocks, tol, maxit);
-            vcycle(h, h->R, h->Z);
+            precond(h, h->R, h->Z);
             launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
             launch_cg_beta(s, h->sc, h->partials, kRedBlocks, maxit);
             launch_cg_update_p(s, h->P, h->Z, n, h->sc);
@@ -304,6 +366,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     *relres = r.relres;
     *nit = r.done ? r.nit : maxit;
     if (maxit <= 0) *nit = 0;
+    if (h->prm.precision != GPUTO_PREC_F64) mixed_end(h, *nit, r.done);
     if (h->prm.print_level >= 1)
         std::fprintf(stderr, "gputo: MGCG nit=%d relres=%.6g res0=%.6g\n", *nit, *relres, r.res0);
     if (r.bad) {
@@ -326,7 +389,6 @@ void gputo_default_params(gputo_params *p)
     p->preset = GPUTO_PRESET_REF_C;
     p->precision = GPUTO_PREC_F64;
     p->learned = 0;
-    p->inner_tol = 1e-6;
     p->l0_kernel = 0;
     p->l1_kernel = 0;
     p->print_level = 0;
@@ -375,7 +437,7 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
         for (int l = 0; l < nl; ++l) {
             Level &v = h->L[l];
             v.g = make_grid(nelx >> l, nely >> l, nelz >> l);
-            v.t = choose_tile(v.g);
+            v.t = l == 0 ? choose_tile(v.g) : stored_tile(v.g);
             maxblocks = std::max<long>(maxblocks, v.t.blocks());
             if (l == 1) {
                 v.tmf = l1_tile(v.g);
@@ -410,6 +472,11 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
         HIPCHK(hipMemcpy(h->sc, h->sc_host, sizeof(CgScalars), hipMemcpyHostToDevice));
         Coarse &C = h->C;
         C.n = (int)h->L[nl - 1].g.ndof;
+        {
+            // DOFs of one element are at most 3(nny*nnz + nny + 1) + 2 apart
+            const Grid &gc = h->L[nl - 1].g;
+            C.bw = 3 * (gc.nny * gc.nnz + gc.nny + 1) + 2;
+        }
         const long N = (C.n + 31) / 32 * 32;
         C.AQ = dalloc<double>(h, N * N);
         C.Linv = dalloc<double>(h, N * N);
@@ -418,6 +485,8 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
         C.flag = dalloc<int>(h, 1);
         std::vector<double> tab = level1_gauss_table(0.3, 2.0);
         HIPCHK(hipMemcpy(h->Aelm, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+        h->gt_max = std::min(nl - 1, kGtMaxDepth);
+        for (int l = 1; l <= h->gt_max; ++l) h->gt[l] = dalloc<double>(h, (size_t)300 << (3 * l));
         if (h->prm.precision != GPUTO_PREC_F64) mixed_create(h);
     } catch (...) {
         int rc = map_exception();
@@ -494,10 +563,7 @@ int gputo_mgcg1(gputo_handle *h, const double *Ke0, const double *F, double *U, 
         HIPCHK(hipMemcpyAsync(h->Fv, F, g0.ndof * sizeof(double), hipMemcpyHostToDevice, h->stream));
         rc = setup_hierarchy(h, fx);
         if (rc == 0) {
-            if (h->prm.precision == GPUTO_PREC_F64)
-                rc = run_pcg(h, h->Fv, h->Uv, tol, maxit, relres, nit);
-            else
-                rc = mixed_solve(h, h->Fv, h->Uv, tol, maxit, relres, nit);
+            rc = run_pcg(h, h->Fv, h->Uv, tol, maxit, relres, nit);
         }
         if (rc) {
             *nit = rc;
@@ -527,10 +593,7 @@ int gputo_solve_d(gputo_handle *h, const double *Ke0, const double *d_F, double 
         HIPCHK(hipStreamSynchronize(h->stream));
         int rc = setup_hierarchy(h, fx);
         if (rc == 0) {
-            if (h->prm.precision == GPUTO_PREC_F64)
-                rc = run_pcg(h, d_F, d_U, tol, maxit, relres, nit);
-            else
-                rc = mixed_solve(h, d_F, d_U, tol, maxit, relres, nit);
+            rc = run_pcg(h, d_F, d_U, tol, maxit, relres, nit);
         }
         if (rc) *nit = rc;
         return rc;

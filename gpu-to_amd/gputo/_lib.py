@@ -43,7 +43,6 @@ class GputoParams(ctypes.Structure):
         ("preset", ctypes.c_int),
         ("precision", ctypes.c_int),
         ("learned", ctypes.c_int),
-        ("inner_tol", ctypes.c_double),
         ("print_level", ctypes.c_int),
         ("l0_kernel", ctypes.c_int),
         ("l1_kernel", ctypes.c_int),

@@ -27,7 +27,7 @@ class Solver:
     """One libgputo handle for an nelx x nely x nelz mesh with nl levels."""
 
     def __init__(self, nelx, nely, nelz, nl, preset=_lib.PRESET_REF_C, precision=_lib.PREC_F64, learned=False,
-                 inner_tol=1e-6, print_level=0, l0_kernel=0, l1_kernel=0):
+                 print_level=0, l0_kernel=0, l1_kernel=0):
         self.lib = _lib.load()
         self.nelx, self.nely, self.nelz, self.nl = int(nelx), int(nely), int(nelz), int(nl)
         self.nel = self.nelx * self.nely * self.nelz
@@ -37,7 +37,6 @@ class Solver:
         self.params.preset = preset
         self.params.precision = precision
         self.params.learned = int(bool(learned))
-        self.params.inner_tol = inner_tol
         self.params.print_level = print_level
         self.params.l0_kernel = l0_kernel
         self.params.l1_kernel = l1_kernel

@@ -48,21 +48,21 @@ enum gputo_preset { GPUTO_PRESET_REF_C = 0, GPUTO_PRESET_MATLAB_CFG0 = 1 };
 
 /* Arithmetic of the solve.
  *  GPUTO_PREC_F64: everything in FP64 (the reference's arithmetic).
- *  GPUTO_PREC_MIXED_F32: FP64 iterative refinement around an inner PCG
- *    whose matvec and smoother run in FP32 (FP32 accumulation); dot
- *    products, alpha/beta, the outer residual and the coarsest Cholesky
- *    stay in FP64.
- *  GPUTO_PREC_MIXED_BF16: as MIXED_F32 with the fine-level Ke x u
- *    contraction on BF16 MFMA (split-BF16 operands, FP32 accumulation). */
+ *  GPUTO_PREC_MIXED_F32: the PCG recurrences, the level-0 matvec, dot
+ *    products, alpha/beta and the coarsest Cholesky stay in FP64; the
+ *    preconditioner (the whole V-cycle) runs on FP32 copies of the hierarchy
+ *    with FP32 accumulation, so rounding changes only the iteration count.
+ *  GPUTO_PREC_MIXED_BF16: as MIXED_F32 with the V-cycle's fine-level
+ *    Ke x u contraction on BF16 MFMA (split-BF16 operands, FP32
+ *    accumulation). */
 enum gputo_precision { GPUTO_PREC_F64 = 0, GPUTO_PREC_MIXED_F32 = 1, GPUTO_PREC_MIXED_BF16 = 2 };
 
 typedef struct gputo_params {
     int preset;        /* enum gputo_preset */
     int precision;     /* enum gputo_precision */
     int learned;       /* 1: apply the learned level-2 correction inside flexible CG */
-    double inner_tol;  /* relative tolerance of one inner solve (mixed precision) */
     int print_level;   /* 0 silent; >=1 summary on stderr; >=2 per-iteration */
-    int l0_kernel;     /* level-0 operator of the inner solve: 0 by precision, 1 FP32
+    int l0_kernel;     /* level-0 operator of the FP32 V-cycle: 0 by precision, 1 FP32
                           node-gather, 2 FP32 MFMA, 3 split-BF16 MFMA (hi+lo),
                           4 split-BF16 MFMA (hi+mid+lo) */
     int l1_kernel;     /* level-1 operator in the V-cycle: 0 by preset (matrix-free
@@ -71,8 +71,8 @@ typedef struct gputo_params {
                           matrices, 2 matrix-free (rediscretised level 1 only) */
 } gputo_params;
 
-/* Fill *p with the defaults (REF_C, F64, no learned stage, inner_tol 1e-6,
- * level-0 kernel chosen by precision). */
+/* Fill *p with the defaults (REF_C, F64, no learned stage, operator
+ * kernels chosen by precision and preset). */
 void gputo_default_params(gputo_params *p);
 
 /* Create a solver for an nelx x nely x nelz hex mesh with nl multigrid

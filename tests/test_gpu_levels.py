@@ -22,7 +22,7 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-MESHES = [(8, 4, 4, 3), (16, 8, 8, 3), (20, 12, 8, 3), (24, 12, 12, 4), (56, 28, 28, 3)]
+MESHES = [(8, 4, 4, 3), (16, 8, 8, 3), (20, 12, 8, 3), (32, 16, 16, 4), (56, 28, 28, 3)]
 
 
 @pytest.mark.parametrize("mesh", MESHES)
@@ -54,6 +54,27 @@ def test_level1_matrix_free_vs_oracle(gputo, oracle, mesh, fp32):
     assert rel(j, j_or) < tol
     # the stored-matrix kernel computes the same operator
     assert rel(s.level_apply(1, 1, 0, u, fp32=fp32), y_or) < tol
+    s.close()
+
+
+@pytest.mark.parametrize("mesh", [(16, 8, 8, 3), (32, 16, 16, 4), (48, 24, 24, 4)])
+def test_fp32_level_operators_match_fp64(gputo, mesh):
+    """Every level's FP32 operator (matvec, defect, Jacobi sweep) against the
+    FP64 operator of the same hierarchy: 2e-6 of max|y|."""
+    nelx, nely, nelz, nl = mesh
+    print("case", mesh, flush=True)
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=37, kind="perturbed")
+    s = gputo.Solver(nelx, nely, nelz, nl, precision=1, l0_kernel=1)
+    s.setup(Ke, CX, fixed)
+    rng = np.random.default_rng(38)
+    for l in range(nl - 1):
+        n = 3 * ((nelx >> l) + 1) * ((nely >> l) + 1) * ((nelz >> l) + 1)
+        u, f = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+        for op in ((1, 2) if l == 1 else (1,)):
+            for mode in (0, 1, 2):
+                y64 = s.level_apply(l, op, mode, u, f)
+                y32 = s.level_apply(l, op, mode, u, f, fp32=True)
+                assert rel(y32, y64) < 2e-6, (l, op, mode, rel(y32, y64))
     s.close()
 
 

@@ -1,12 +1,14 @@
-"""Mixed-precision path: FP32 / split-BF16 MFMA level-0 operator and the FP64
-iterative-refinement solve, checked against the FP64 oracle.
+"""Mixed-precision path: FP32 / split-BF16 MFMA level-0 operators and the
+mixed PCG (FP32 operators, FP64 iterate, FP64 residual replacement), checked
+against the FP64 oracle.
 
 Tolerances: the FP32 node-gather and FP32 MFMA products agree with the FP64
 oracle to 1e-6 of max|y| (FP32 rounding of Ke, E, u and the accumulations);
 the three-way split-BF16 product (terms below 2^-24 dropped) to 1e-6; the
 two-way split (hi*hi + hi*lo + lo*hi, lo*lo dropped) to 1e-4.
-The refined solve reaches the FP64 residual tolerance, so its displacements
-and compliance agree with the plain FP64 solve to the tolerance.
+The mixed solve stops on an FP64 true residual below the tolerance, so its
+displacements and compliance agree with the plain FP64 solve to the
+tolerance, in about the FP64 iteration count.
 """
 import numpy as np
 import pytest
@@ -51,9 +53,12 @@ def test_mixed_solve_matches_f64(gputo, oracle, prec, kernel, mesh):
     s64 = gputo.Solver(nelx, nely, nelz, nl)
     U64, r64, n64, rc = s64.solve(Ke, F, CX, fixed, 1e-10, 300)
     assert rc == 0
-    sm = gputo.Solver(nelx, nely, nelz, nl, precision=prec, inner_tol=1e-6, l0_kernel=kernel)
+    sm = gputo.Solver(nelx, nely, nelz, nl, precision=prec, l0_kernel=kernel)
     Um, rm, nm, rc = sm.solve(Ke, F, CX, fixed, 1e-10, 300)
     assert rc == 0 and rm < 1e-10
+    # FP32 operators with residual replacement keep the FP64 iteration count
+    # (split-BF16 hi+lo operator: within a few iterations)
+    assert nm <= n64 + (2 if kernel != 3 else 40), (nm, n64)
     # the outer residual is the true FP64 residual: check it with the oracle
     Uc = Um.copy()
     Uc[1::3] *= -1

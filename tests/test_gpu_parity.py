@@ -66,7 +66,7 @@ def test_matvec_dense_bruteforce(solvers):
 
 
 @pytest.mark.parametrize("preset", [0, 1])
-@pytest.mark.parametrize("mesh", [(8, 4, 4, 3), (16, 8, 8, 3), (16, 8, 8, 4)])
+@pytest.mark.parametrize("mesh", [(8, 4, 4, 3), (16, 8, 8, 3), (16, 8, 8, 4), (32, 16, 16, 5)])
 def test_hierarchy_levels(solvers, oracle, mesh, preset):
     nelx, nely, nelz, nl = mesh
     print("case", mesh, preset, flush=True)
@@ -81,6 +81,20 @@ def test_hierarchy_levels(solvers, oracle, mesh, preset):
         if l > 0:
             assert rel(A, Ao) < 1e-12, "element matrices of level %d" % l
         assert rel(ad, ado) < 1e-12
+
+
+def test_galerkin_levels_follow_ke(solvers, oracle):
+    """With a Galerkin level 1 the coarse operators are rebuilt when Ke
+    changes between calls on one handle (and stay right when it does not)."""
+    nelx, nely, nelz, nl = 16, 8, 8, 3
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=8)
+    s = solvers(nelx, nely, nelz, nl, 1)
+    for K in (Ke, 2.0 * Ke, 2.0 * Ke, Ke):
+        s.setup(K, CX, fixed)
+        for l in (1, 2):
+            A, ad, fd = s.level_matrices(l)
+            Ao, ado, fdo = oracle.level_data(K, CX, fixed, nelx, nely, nelz, nl, l, preset=1)
+            assert rel(A, Ao) < 1e-12
 
 
 def test_edofmat_bit_exact(solvers, oracle):
