@@ -389,7 +389,7 @@ int gputo_sens_f64(gputo_handle *h, const double *Ke0, const double *U_cframe, d
     try {
         const Grid &g = h->L[0].g;
         hipStream_t s = h->stream;
-        check_hip(hipMemcpyAsync(h->Ke, Ke0, 576 * sizeof(double), hipMemcpyHostToDevice, s), "copy");
+        set_ke(h, Ke0);
         check_hip(hipMemcpyAsync(h->P, U_cframe, g.ndof * sizeof(double), hipMemcpyHostToDevice, s), "copy");
         k_sens<<<nb256(g.nel), 256, 0, s>>>(g, h->Ke, h->P, nullptr, nullptr, 0.0, nullptr, nullptr, h->Q);
         check_hip(hipMemcpyAsync(ce, h->Q, g.nel * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
@@ -505,7 +505,7 @@ int gputo_to_iterate(gputo_handle *h, gputo_to_stats *st)
         hipEventRecord(e0, s);
         double relres = 0.0;
         int nit = 0;
-        check_hip(hipMemcpyAsync(h->Ke, t->Ke.data(), 576 * sizeof(double), hipMemcpyHostToDevice, s), "copy");
+        set_ke(h, t->Ke.data());
         check_hip(hipMemcpyAsync(h->CX, t->CX, ne * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
         int rc = setup_hierarchy(h, t->fixed);
         if (rc == 0) {

@@ -22,6 +22,7 @@ constexpr int kStNodes = kBlock / 8; // nodes per block of the stored-matrix ope
 constexpr int kRedBlocks = 1024;     // blocks of the two-stage reductions
 constexpr int kMaxLevels = 12;
 constexpr int kMaxCoarseDofs = 16384;
+constexpr int kKeMaxClasses = 16;    // Ke0 entry classes passed by value to the level-0 kernels
 
 // One structured hex grid: elements nx*ny*nz, nodes (nx+1)(ny+1)(nz+1);
 // node id = nny*nnz*i + nny*k + j, element id = ny*nz*i + ny*k + j.
@@ -60,6 +61,8 @@ struct OpArgsT {
     double *partials;      // per-block partial sums (fused dot), may be null
     const int *ctl;        // ctl[0] != 0: skip (converged)
     double omega, prerelax;
+    int kc;                // Ke0 has the unit-H8 class structure: use kv
+    T kv[kKeMaxClasses];   // value of each Ke0 entry class (kKeClass)
 };
 using OpArgs = OpArgsT<double>;
 
@@ -103,6 +106,7 @@ struct Coarse {
 };
 
 struct MixedState;
+struct LearnState;
 
 }  // namespace gputo
 
@@ -114,6 +118,8 @@ struct gputo_handle {
     hipStream_t stream = nullptr;
     int device = 0;
     double *Ke = nullptr;         // fine element matrix (row-major, symmetric)
+    int ke_classes = 0;           // Ke has the unit-H8 class structure (values in kv)
+    double kv[gputo::kKeMaxClasses] = {};
     double *Aelm = nullptr;       // level-1 Gauss-point matrices, packed [300][8]
     double *CX = nullptr;         // element scales of level 0
     double *Fv = nullptr, *Uv = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr, *Z = nullptr;
@@ -126,6 +132,7 @@ struct gputo_handle {
     std::vector<void *> allocs;
     struct ToState *to = nullptr;
     gputo::MixedState *mx = nullptr;
+    gputo::LearnState *ln = nullptr;  // learned level-2 correction (params.learned)
     // per-fine-element Galerkin tables of levels 1..gt_max (device, packed
     // [8^l][300]; null where the level is built from the level above)
     double *gt[gputo::kMaxLevels] = {};
@@ -156,6 +163,12 @@ void launch_coarse_galerkin(hipStream_t s, const Grid &gc, const double *Af, con
 void launch_galerkin_table(hipStream_t s, const Grid &gc, int depth, const double *T, const double *CX0, double *A);
 std::vector<std::vector<double>> galerkin_tables(const double *Ke, const double *base1, int maxdepth);
 constexpr int kGtMaxDepth = 4;  // deepest level built from tables (8^4 x 300 doubles = 9.8 MB)
+// A table product costs 300 FMA per level-0 element at every level, spread
+// over nel_l/16 blocks that each stream 8^l table rows; below this many
+// elements the child-by-child Galerkin of the level above (about 1.7e5 FMA
+// per element, one block each) is faster (measured at 192x96x96: level 3
+// with 3456 elements took 263 us from its table).
+constexpr long kGtMinElems = 8192;
 void launch_corrcsa(hipStream_t s, const Grid &g, const int *fixed_d, int nfd, double *A, int mirrored);
 void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ);
 int coarse_factor(hipStream_t s, Coarse &C);
@@ -171,8 +184,12 @@ void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long
 void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit);
 void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit);
 template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc);
+// flexible beta (partials needs 2*kRedBlocks entries)
+void launch_cg_beta_flex(hipStream_t s, const double *r, const double *z, const double *q, long n, CgScalars *sc,
+                         double *partials, int maxit);
 
 void check_hip(hipError_t e, const char *what);
+void set_ke(gputo_handle *h, const double *Ke0);  // upload Ke0 and detect its class structure
 
 // Deterministic sum over a kBlock-thread block: fixed butterfly per wave,
 // then the wave sums in order.  Every thread of the block must call it.

@@ -12,6 +12,7 @@
 // mgcg1.cu:1313-1316, stored entry-major so that neighbouring elements are
 // neighbouring in memory).
 #include "gputo_internal.h"
+#include "ke_classes.h"
 
 namespace gputo {
 
@@ -31,20 +32,20 @@ __host__ __device__ constexpr int pidx(int m, int n) { return n <= m ? m * (m + 
 // TOP3D125MGCGCUDA/mgcg1.cu:941-997,1208-1264).  One block owns a
 // TJ x TK x TI brick of nodes; it stages the nodal values of the brick plus a
 // one-node halo in LDS (structure of arrays, one plane per component), the
-// element scales of the (TJ+1)(TK+1)(TI+1) element box and Ke.  Each thread
+// element scales of the (TJ+1)(TK+1)(TI+1) element box.  Each thread
 // then forms the 3 rows of K U at its node from the 27 neighbouring nodes:
-//   y_c = sum_e E_e sum_b Ke[3a_e + c][3b + d] u_{b,d}
-// one incident element at a time (Ke rows wave-uniform LDS broadcasts).
+//   y_c = sum_e E_e sum_b Ke[3a_e + c][3b + d] u_{b,d}.
 // No atomics: each output DOF is written by exactly one thread.
 // ---------------------------------------------------------------------------
-template <class T, int MODE>
+template <class T, int MODE, bool KC>
 __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
 {
     if (a.ctl && a.ctl[0]) return;
     __shared__ T su[3][kMaxHalo];
     __shared__ T sE[kMaxElemBox];
-    __shared__ T sK[576];  // Ke, read with wave-uniform (broadcast) LDS loads
-    for (int q = threadIdx.x; q < 576; q += kBlock) sK[q] = a.Ke[q];
+    __shared__ T sK[KC ? 1 : 576];  // general Ke: wave-uniform (broadcast) LDS reads
+    if (!KC)
+        for (int q = threadIdx.x; q < 576; q += kBlock) sK[q] = a.Ke[q];
     const Grid g = a.g;
     const Tile t = a.t;
     int b = blockIdx.x;
@@ -97,39 +98,95 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
         const int gi = i0 + ti, gj = j0 + tj, gk = k0 + tk;
         if (gi < g.nnx && gj < g.nny && gk < g.nnz) {
             const int hb = (ti * HK + tk) * HJ + tj;  // halo index of neighbour offset (0,0,0)
-            T y0 = T(0), y1 = T(0), y2 = T(0);
-            {
-                // one incident element at a time (runtime loop keeps the live
-                // set small): t = Ke[3a..3a+2, :] u_e, y += E_e t, with the
-                // node's local index a and the Ke rows uniform across the wave
-                const int eb = (ti * EK + tk) * EJ + tj;
-#pragma unroll 1
-                for (int e = 0; e < 8; ++e) {
-                    const int ei = e >> 2, ek = (e >> 1) & 1, ej = e & 1;
-                    const T E = sE[eb + (ei * EK + ek) * EJ + ej];
-                    const T *kr = sK + 72 * (4 * (1 - ek) + ((1 - ej) ? ((1 - ei) ? 2 : 3) : ((1 - ei) ? 1 : 0)));
-                    const int he = hb + (ei * HK + ek) * HJ + ej;
-                    T t0 = T(0), t1 = T(0), t2 = T(0);
+            T y0, y1, y2;
+            if (KC) {
+                // Neighbour form: for each of the 27 neighbour nodes m (u_m read
+                // from LDS once) and each incident element s holding both this
+                // node and m, z_s += Ke[3a_s.., 3b..] u_m; then y = sum_s E_s z_s.
+                // Every Ke index is a compile-time constant after unrolling and
+                // Ke0 has the unit-H8 class structure, so each entry is one of
+                // the kKeClasses values passed as kernel arguments (scalar
+                // registers): LDS carries only the 81 neighbour values.
+                T z[8][3];
 #pragma unroll
-                    for (int lb = 0; lb < 8; ++lb) {
-                        const int bx = (lb == 1 || lb == 2 || lb == 5 || lb == 6) ? 1 : 0;
-                        const int by = (lb == 2 || lb == 3 || lb == 6 || lb == 7) ? 1 : 0;
-                        const int bz = lb >= 4 ? 1 : 0;
-                        const int h = he + (bx * HK + bz) * HJ + by;
-                        const T u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
-                        t0 = fma(kr[3 * lb + 0], u0, t0);
-                        t0 = fma(kr[3 * lb + 1], u1, t0);
-                        t0 = fma(kr[3 * lb + 2], u2, t0);
-                        t1 = fma(kr[24 + 3 * lb + 0], u0, t1);
-                        t1 = fma(kr[24 + 3 * lb + 1], u1, t1);
-                        t1 = fma(kr[24 + 3 * lb + 2], u2, t1);
-                        t2 = fma(kr[48 + 3 * lb + 0], u0, t2);
-                        t2 = fma(kr[48 + 3 * lb + 1], u1, t2);
-                        t2 = fma(kr[48 + 3 * lb + 2], u2, t2);
+                for (int q = 0; q < 8; ++q) z[q][0] = z[q][1] = z[q][2] = T(0);
+#pragma unroll
+                for (int di = -1; di <= 1; ++di)
+#pragma unroll
+                    for (int dk = -1; dk <= 1; ++dk)
+#pragma unroll
+                        for (int dj = -1; dj <= 1; ++dj) {
+                            const int h = hb + ((1 + di) * HK + (1 + dk)) * HJ + (1 + dj);
+                            const T u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
+#pragma unroll
+                            for (int si = 0; si < 2; ++si)
+#pragma unroll
+                                for (int sk = 0; sk < 2; ++sk)
+#pragma unroll
+                                    for (int sj = 0; sj < 2; ++sj) {
+                                        // element s has its first node at n + (s - 1)
+                                        const int bx = di - si + 1, by = dj - sj + 1, bz = dk - sk + 1;
+                                        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+                                        const int la = lidx(1 - si, 1 - sj, 1 - sk), lb = lidx(bx, by, bz);
+                                        T *zs = z[4 * si + 2 * sk + sj];
+#pragma unroll
+                                        for (int c = 0; c < 3; ++c) {
+                                            const int r = 24 * (3 * la + c) + 3 * lb;
+                                            zs[c] = fma(a.kv[kKeClass[r + 0]], u0, zs[c]);
+                                            zs[c] = fma(a.kv[kKeClass[r + 1]], u1, zs[c]);
+                                            zs[c] = fma(a.kv[kKeClass[r + 2]], u2, zs[c]);
+                                        }
+                                    }
+                        }
+                const int eb = (ti * EK + tk) * EJ + tj;
+                y0 = T(0), y1 = T(0), y2 = T(0);
+#pragma unroll
+                for (int si = 0; si < 2; ++si)
+#pragma unroll
+                    for (int sk = 0; sk < 2; ++sk)
+#pragma unroll
+                        for (int sj = 0; sj < 2; ++sj) {
+                            const T E = sE[eb + (si * EK + sk) * EJ + sj];
+                            const T *zs = z[4 * si + 2 * sk + sj];
+                            y0 = fma(E, zs[0], y0);
+                            y1 = fma(E, zs[1], y1);
+                            y2 = fma(E, zs[2], y2);
+                        }
+            } else {
+                y0 = T(0), y1 = T(0), y2 = T(0);
+                {
+                    // one incident element at a time (runtime loop keeps the live
+                    // set small): t = Ke[3a..3a+2, :] u_e, y += E_e t, with the
+                    // node's local index a and the Ke rows uniform across the wave
+                    const int eb = (ti * EK + tk) * EJ + tj;
+#pragma unroll 1
+                    for (int e = 0; e < 8; ++e) {
+                        const int ei = e >> 2, ek = (e >> 1) & 1, ej = e & 1;
+                        const T E = sE[eb + (ei * EK + ek) * EJ + ej];
+                        const T *kr = sK + 72 * (4 * (1 - ek) + ((1 - ej) ? ((1 - ei) ? 2 : 3) : ((1 - ei) ? 1 : 0)));
+                        const int he = hb + (ei * HK + ek) * HJ + ej;
+                        T t0 = T(0), t1 = T(0), t2 = T(0);
+#pragma unroll
+                        for (int lb = 0; lb < 8; ++lb) {
+                            const int bx = (lb == 1 || lb == 2 || lb == 5 || lb == 6) ? 1 : 0;
+                            const int by = (lb == 2 || lb == 3 || lb == 6 || lb == 7) ? 1 : 0;
+                            const int bz = lb >= 4 ? 1 : 0;
+                            const int h = he + (bx * HK + bz) * HJ + by;
+                            const T u0 = su[0][h], u1 = su[1][h], u2 = su[2][h];
+                            t0 = fma(kr[3 * lb + 0], u0, t0);
+                            t0 = fma(kr[3 * lb + 1], u1, t0);
+                            t0 = fma(kr[3 * lb + 2], u2, t0);
+                            t1 = fma(kr[24 + 3 * lb + 0], u0, t1);
+                            t1 = fma(kr[24 + 3 * lb + 1], u1, t1);
+                            t1 = fma(kr[24 + 3 * lb + 2], u2, t1);
+                            t2 = fma(kr[48 + 3 * lb + 0], u0, t2);
+                            t2 = fma(kr[48 + 3 * lb + 1], u1, t2);
+                            t2 = fma(kr[48 + 3 * lb + 2], u2, t2);
+                        }
+                        y0 = fma(E, t0, y0);
+                        y1 = fma(E, t1, y1);
+                        y2 = fma(E, t2, y2);
                     }
-                    y0 = fma(E, t0, y0);
-                    y1 = fma(E, t1, y1);
-                    y2 = fma(E, t2, y2);
                 }
             }
             const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
@@ -172,12 +229,23 @@ template <class T>
 void launch_op0(hipStream_t s, int mode, const OpArgsT<T> &a)
 {
     const dim3 grid((unsigned)a.t.blocks()), block(kBlock);
+    if (a.kc) {
+        switch (mode) {
+        case OP_MATVEC: k_gather<T, OP_MATVEC, true><<<grid, block, 0, s>>>(a); break;
+        case OP_DEFECT: k_gather<T, OP_DEFECT, true><<<grid, block, 0, s>>>(a); break;
+        case OP_JACOBI: k_gather<T, OP_JACOBI, true><<<grid, block, 0, s>>>(a); break;
+        case OP_PRE_JACOBI: k_gather<T, OP_PRE_JACOBI, true><<<grid, block, 0, s>>>(a); break;
+        case OP_PRE_DEFECT: k_gather<T, OP_PRE_DEFECT, true><<<grid, block, 0, s>>>(a); break;
+        default: break;
+        }
+        return;
+    }
     switch (mode) {
-    case OP_MATVEC: k_gather<T, OP_MATVEC><<<grid, block, 0, s>>>(a); break;
-    case OP_DEFECT: k_gather<T, OP_DEFECT><<<grid, block, 0, s>>>(a); break;
-    case OP_JACOBI: k_gather<T, OP_JACOBI><<<grid, block, 0, s>>>(a); break;
-    case OP_PRE_JACOBI: k_gather<T, OP_PRE_JACOBI><<<grid, block, 0, s>>>(a); break;
-    case OP_PRE_DEFECT: k_gather<T, OP_PRE_DEFECT><<<grid, block, 0, s>>>(a); break;
+    case OP_MATVEC: k_gather<T, OP_MATVEC, false><<<grid, block, 0, s>>>(a); break;
+    case OP_DEFECT: k_gather<T, OP_DEFECT, false><<<grid, block, 0, s>>>(a); break;
+    case OP_JACOBI: k_gather<T, OP_JACOBI, false><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_JACOBI: k_gather<T, OP_PRE_JACOBI, false><<<grid, block, 0, s>>>(a); break;
+    case OP_PRE_DEFECT: k_gather<T, OP_PRE_DEFECT, false><<<grid, block, 0, s>>>(a); break;
     default: break;
     }
 }
@@ -618,6 +686,51 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(CgScalars *sc, const double 
             sc->nit = maxit;
         }
     }
+}
+
+// Flexible (Polak-Ribiere) beta for a preconditioner that varies between
+// iterations: beta = <z, r - r_old>/rho_old with r - r_old = -alpha q, so
+// beta = -alpha <z,q>/rho_old; rho = <r,z> is kept for the next alpha.
+__global__ __launch_bounds__(kBlock) void k_dot2_partial(const double *__restrict__ r, const double *__restrict__ z,
+                                                       const double *__restrict__ q, long n, double *partials,
+                                                       const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    double a = 0.0, b = 0.0;
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) {
+        a += r[i] * z[i];
+        b += q[i] * z[i];
+    }
+    a = block_sum(a);
+    b = block_sum(b);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = a;
+        partials[kRedBlocks + blockIdx.x] = b;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_beta_flex(CgScalars *sc, const double *p, int maxit)
+{
+    if (sc->done) return;
+    const double rz = sum_partials(p, kRedBlocks);
+    const double qz = sum_partials(p + kRedBlocks, kRedBlocks);
+    if (threadIdx.x == 0) {
+        sc->rho = rz;
+        sc->beta = -sc->alpha * qz / sc->rho_p;
+        sc->rho_p = rz;
+        sc->it += 1;
+        if (sc->it >= maxit) {
+            sc->done = 1;
+            sc->nit = maxit;
+        }
+    }
+}
+
+void launch_cg_beta_flex(hipStream_t s, const double *r, const double *z, const double *q, long n, CgScalars *sc,
+                         double *partials, int maxit)
+{
+    k_dot2_partial<<<kRedBlocks, kBlock, 0, s>>>(r, z, q, n, partials, &sc->done);
+    k_cg_beta_flex<<<1, kBlock, 0, s>>>(sc, partials, maxit);
 }
 
 template <class T>

@@ -111,6 +111,7 @@ static void mixed_prepare(gputo_handle *h)
     }
     m->H.l0 = l0_kind(h->prm, h->prm.precision);
     m->H.l0_ctx = m;
+    m->H.learn = h->ln != nullptr;
     if (m->H.l0 == L0_MFMA_BF16) mfma_prepare(h, m);
 }
 

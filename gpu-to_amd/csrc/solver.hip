@@ -7,6 +7,8 @@
 #include <stdexcept>
 
 #include "gputo_internal.h"
+#include "ke_classes.h"
+#include "learn.h"
 #include "mixed.h"
 #include "vcycle.h"
 
@@ -22,6 +24,21 @@ void check_hip(hipError_t e, const char *what)
 }
 
 #define HIPCHK(x) check_hip((x), #x)
+
+static_assert(kKeClasses <= kKeMaxClasses, "Ke class table larger than the kernel argument array");
+
+// Upload Ke0 (read row-major, as the reference reads MATLAB's column-major
+// symmetric array, mgcg1.cu:131-133) and record whether it has the unit-H8
+// class structure bit for bit: then the level-0 kernels take its class
+// values as arguments instead of reading the 576 entries.
+void set_ke(gputo_handle *h, const double *Ke0)
+{
+    HIPCHK(hipMemcpyAsync(h->Ke, Ke0, 576 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    for (int k = 0; k < kKeClasses; ++k) h->kv[k] = Ke0[kKeFirst[k]];
+    int ok = 1;
+    for (int q = 0; q < 576 && ok; ++q) ok = std::memcmp(&Ke0[q], &h->kv[kKeClass[q]], sizeof(double)) == 0;
+    h->ke_classes = ok;
+}
 
 Grid make_grid(int nx, int ny, int nz)
 {
@@ -297,7 +314,7 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
         Level &v = h->L[l];
         if (l == 1 && !p.l1_galerkin)
             launch_coarse_l1(s, v.g, h->Aelm, h->CX, v.A);
-        else if (l <= h->gt_max)
+        else if (l <= h->gt_max && (v.g.nel >= kGtMinElems || l == 1))
             launch_galerkin_table(s, v.g, l, h->gt[l], h->CX, v.A);
         else
             launch_coarse_galerkin(s, v.g, l == 1 ? nullptr : h->L[l - 1].A, h->Ke, h->CX, v.A);
@@ -328,6 +345,9 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
 {
     hipStream_t s = h->stream;
     if (h->prm.precision != GPUTO_PREC_F64) mixed_begin(h);
+    if (h->ln) learn_begin_solve(h);
+    // a trained learned stage makes the preconditioner vary: flexible beta
+    const bool flex = h->ln && h->ln->trained;
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
@@ -353,8 +373,12 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
             launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials);
             launch_cg_check(s, h->sc, h->partials, kRedBlocks, tol, maxit);
             precond(h, h->R, h->Z);
-            launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
-            launch_cg_beta(s, h->sc, h->partials, kRedBlocks, maxit);
+            if (flex) {
+                launch_cg_beta_flex(s, h->R, h->Z, h->Q, n, h->sc, h->partials, maxit);
+            } else {
+                launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
+                launch_cg_beta(s, h->sc, h->partials, kRedBlocks, maxit);
+            }
             launch_cg_update_p(s, h->P, h->Z, n, h->sc);
         }
         HIPCHK(hipGetLastError());
@@ -367,6 +391,14 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     *nit = r.done ? r.nit : maxit;
     if (maxit <= 0) *nit = 0;
     if (h->prm.precision != GPUTO_PREC_F64) mixed_end(h, *nit, r.done);
+    if (h->ln) {
+        int acc = 0, rej = 0;
+        learn_end_solve(h, &acc, &rej);
+        if (h->mx) {
+            h->mx->last_learned_ok = acc;
+            h->mx->last_learned_bad = rej;
+        }
+    }
     if (h->prm.print_level >= 1)
         std::fprintf(stderr, "gputo: MGCG nit=%d relres=%.6g res0=%.6g\n", *nit, *relres, r.res0);
     if (r.bad) {
@@ -424,6 +456,8 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
     if (nelx % f || nely % f || nelz % f) return GPUTO_ERR_ARGUMENT;
     const Grid gc = make_grid(nelx / f, nely / f, nelz / f);
     if (gc.ndof > kMaxCoarseDofs) return GPUTO_ERR_WORKSPACE;
+    // the learned stage works on level 2 of the FP32 V-cycle
+    if (params && params->learned && (params->precision == GPUTO_PREC_F64 || nl < 4)) return GPUTO_ERR_ARGUMENT;
     gputo_handle *h = new gputo_handle();
     if (params)
         h->prm = *params;
@@ -433,7 +467,7 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
     try {
         HIPCHK(hipGetDevice(&h->device));
         HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-        int maxblocks = kRedBlocks;
+        int maxblocks = 2 * kRedBlocks;  // two reductions share the partials (flexible beta)
         for (int l = 0; l < nl; ++l) {
             Level &v = h->L[l];
             v.g = make_grid(nelx >> l, nely >> l, nelz >> l);
@@ -488,6 +522,7 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
         h->gt_max = std::min(nl - 1, kGtMaxDepth);
         for (int l = 1; l <= h->gt_max; ++l) h->gt[l] = dalloc<double>(h, (size_t)300 << (3 * l));
         if (h->prm.precision != GPUTO_PREC_F64) mixed_create(h);
+        if (h->prm.learned) learn_create(h);
     } catch (...) {
         int rc = map_exception();
         gputo_destroy(h);
@@ -501,6 +536,7 @@ int gputo_destroy(gputo_handle *h)
 {
     if (!h) return 0;
     to_release(h);
+    learn_destroy(h);
     mixed_destroy(h);
     if (h->stream) hipStreamSynchronize(h->stream);
     for (void *p : h->allocs) hipFree(p);
@@ -513,8 +549,16 @@ int gputo_destroy(gputo_handle *h)
 int gputo_set_params(gputo_handle *h, const gputo_params *params)
 {
     if (!h || !params) return GPUTO_ERR_ARGUMENT;
+    if (!!params->learned != !!h->prm.learned) return GPUTO_ERR_ARGUMENT;  // fixed at creation
+    if (params->precision != h->prm.precision && params->precision == GPUTO_PREC_F64 && h->ln)
+        return GPUTO_ERR_ARGUMENT;  // the learned stage runs in the FP32 V-cycle
     if (params->preset != h->prm.preset) h->hier_ready = false;
     h->prm = *params;
+    try {
+        if (h->prm.precision != GPUTO_PREC_F64) mixed_create(h);
+    } catch (...) {
+        return map_exception();
+    }
     return 0;
 }
 
@@ -526,9 +570,7 @@ static int upload_problem(gputo_handle *h, const double *Ke0, const double *CX, 
     if (!all_finite(Ke0, 576) || !all_finite(CX, g0.nel)) return GPUTO_ERR_NONFINITE;
     fx.resize(nfd);
     for (int n = 0; n < nfd; ++n) fx[n] = (int)fixed[n] - 1;  // (int)FDofs[n]-1, mgcg1.c:663
-    // Ke is read row-major from the MATLAB column-major array (a transpose;
-    // Ke0 is symmetric), as mgcg1.cu:131-133 does.
-    HIPCHK(hipMemcpyAsync(h->Ke, Ke0, 576 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    set_ke(h, Ke0);
     HIPCHK(hipMemcpyAsync(h->CX, CX, g0.nel * sizeof(double), hipMemcpyHostToDevice, h->stream));
     return 0;
 }
@@ -588,7 +630,7 @@ int gputo_solve_d(gputo_handle *h, const double *Ke0, const double *d_F, double 
         const Grid &g0 = h->L[0].g;
         std::vector<int> fx(nfd);
         if (nfd) HIPCHK(hipMemcpyAsync(fx.data(), d_fixed, nfd * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipMemcpyAsync(h->Ke, Ke0, 576 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        set_ke(h, Ke0);
         HIPCHK(hipMemcpyAsync(h->CX, d_CX, g0.nel * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         int rc = setup_hierarchy(h, fx);

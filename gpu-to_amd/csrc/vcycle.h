@@ -41,7 +41,12 @@ struct HierView {
     int l0 = L0_GATHER;
     const void *l0_ctx = nullptr;  // state of a non-gather level-0 kernel
     bool l1_mf = false;            // level 1 applied matrix-free from CX
+    bool learn = false;            // learned level-2 stage (FP32 views only)
 };
+
+constexpr int kLearnLevel = 2;
+// learned correction after the level-2 post-smoothing (learn.hip)
+void learn_stage(gputo_handle *h, const HierView<float> &H, int l, const float *F, float *U, const int *ctl);
 
 // level-0 BF16-MFMA operator (mfma.hip); ctx is the MixedState
 void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const void *ctx, int flavor);
@@ -76,6 +81,8 @@ OpArgsT<T> view_args(const gputo_handle *h, const HierView<T> &H, int l, const i
     const Preset p = preset_of(h->prm.preset);
     a.omega = p.omega;
     a.prerelax = p.prerelax;
+    a.kc = h->ke_classes;
+    for (int k = 0; k < kKeMaxClasses; ++k) a.kv[k] = (T)h->kv[k];
     return a;
 }
 
@@ -155,6 +162,9 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
         T *U = l == 0 ? U0 : v.U;
         launch_interp<T>(h->stream, h->L[l].g, H.L[l + 1].U, U, h->L[l].mask, p.zero_after_interp, ctl);
         sweeps_t(h, H, l, F, U, p.npost, false, ctl);
+        if constexpr (std::is_same<T, float>::value) {
+            if (H.learn && l == kLearnLevel) learn_stage(const_cast<gputo_handle *>(h), H, l, F, U, ctl);
+        }
     }
 }
 

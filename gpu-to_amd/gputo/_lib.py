@@ -118,6 +118,12 @@ SIGNATURES = {
     "gputo_bench_operator": (_I, [_P, _I, _I, _pD]),
     "gputo_level_apply": (_I, [_P, _I, _I, _I, _I, _P, _P, _P]),
     "gputo_bench_level": (_I, [_P, _I, _I, _I, _I, _pD]),
+    "gputo_learn_info": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _pI, _pI, _pI, _pI]),
+    "gputo_learn_params": (_I, [_P, _I, _P]),
+    "gputo_learn_reset": (_I, [_P, ctypes.c_uint]),
+    "gputo_learn_forward": (_I, [_P, _P, _P]),
+    "gputo_learn_grad": (_I, [_P, _P, _P, _P]),
+    "gputo_learn_train": (_I, [_P, _I, _D, _pD, _pD]),
     "gputo_edofmat": (_I, [_P, _P]),
     "gputo_fixed_list": (_I, [_P, _I, _P, _pI]),
     "gputo_to_begin": (_I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(ToConfig)]),

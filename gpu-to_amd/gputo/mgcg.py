@@ -143,6 +143,43 @@ class Solver:
               "gputo_bench_level")
         return avg.value
 
+    # ---- learned level-2 correction -------------------------------------
+    def learn_info(self):
+        n = ctypes.c_longlong()
+        d = [ctypes.c_int() for _ in range(4)]
+        check(self.lib.gputo_learn_info(self.h, ctypes.byref(n), *[ctypes.byref(v) for v in d]), "gputo_learn_info")
+        return {"nparam": n.value, "grid": (d[0].value, d[1].value, d[2].value), "nsamples": d[3].value}
+
+    def learn_params(self, values=None):
+        info = self.learn_info()
+        if values is None:
+            buf = np.zeros(info["nparam"])
+            check(self.lib.gputo_learn_params(self.h, 0, ptr(buf)), "gputo_learn_params")
+            return buf
+        buf = f64(values).ravel()
+        assert buf.size == info["nparam"]
+        check(self.lib.gputo_learn_params(self.h, 1, ptr(buf)), "gputo_learn_params")
+
+    def learn_reset(self, seed=1234):
+        check(self.lib.gputo_learn_reset(self.h, seed), "gputo_learn_reset")
+
+    def learn_forward(self, x):
+        n0, n1, n2 = self.learn_info()["grid"]
+        y = np.zeros(3 * n0 * n1 * n2)
+        check(self.lib.gputo_learn_forward(self.h, ptr(f64(x).ravel()), ptr(y)), "gputo_learn_forward")
+        return y.reshape(3, n0, n1, n2)
+
+    def learn_grad(self, x, gy):
+        g = np.zeros(self.learn_info()["nparam"])
+        check(self.lib.gputo_learn_grad(self.h, ptr(f64(x).ravel()), ptr(f64(gy).ravel()), ptr(g)),
+              "gputo_learn_grad")
+        return g
+
+    def learn_train(self, steps, lr=1e-3):
+        l0, l1 = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.gputo_learn_train(self.h, steps, lr, ctypes.byref(l0), ctypes.byref(l1)), "gputo_learn_train")
+        return l0.value, l1.value
+
     def fixed_list(self, level):
         n = ctypes.c_int()
         check(self.lib.gputo_fixed_list(self.h, level, None, ctypes.byref(n)), "gputo_fixed_list")

@@ -6,11 +6,10 @@ All arrays use the reference's numbering: node n = (nely+1)(nelz+1) i +
 import numpy as np
 
 
-def ke0_h8(nu=0.3, h=1.0):
+def ke_h8_gauss(nu=0.3, h=1.0):
     """24x24 trilinear hex stiffness (E = 1), 2x2x2 Gauss, in the element
     order ie={0,1,1,0,0,1,1,0}, je={0,0,1,1,0,0,1,1}, ke={0,0,0,0,1,1,1,1}
-    (Cantilever/omp/mgcg1.c:621-624).  Numerically equal to the Ke0 the
-    MATLAB drivers build from their integer table (top3D125mod.m:50-76)."""
+    (Cantilever/omp/mgcg1.c:621-624)."""
     ie = np.array([0, 1, 1, 0, 0, 1, 1, 0])
     je = np.array([0, 0, 1, 1, 0, 0, 1, 1])
     kk = np.array([0, 0, 0, 0, 1, 1, 1, 1])
@@ -35,6 +34,27 @@ def ke0_h8(nu=0.3, h=1.0):
                 B[5, 0::3], B[5, 2::3] = dz, dx
                 K += B.T @ D @ B * (h ** 3 / 8.0)
     return K
+
+
+def ke_integer_tables():
+    """Integer matrices (A, B) with Ke(nu) = (A + nu B) / ((1+nu)(2nu-1) 144)
+    for the unit H8 -- the form in which the MATLAB drivers write Ke0
+    (top3D125mod.m:50-76) -- derived from the Gauss-point stiffness:
+    (1+nu)(1-2nu) Ke(nu) is affine in nu."""
+    def khat(nu):
+        return (1 + nu) * (1 - 2 * nu) * ke_h8_gauss(nu)
+    A = np.rint(-144 * khat(0.0))
+    B = np.rint(-144 * (khat(1.0) - khat(0.0)))
+    return A, B
+
+
+def ke0_h8(nu=0.3, h=1.0):
+    """Ke0 exactly as the MATLAB drivers build it (top3D125mod.m:50-76):
+    1/(1+nu)/(2nu-1)/144 * (A + nu B), evaluated in that order, so entries
+    with the same integer pair are bitwise equal.  Times h for an h-cube."""
+    A, B = ke_integer_tables()
+    K = 1.0 / (1 + nu) / (2 * nu - 1) / 144 * (A + nu * B)
+    return K if h == 1.0 else h * K
 
 
 def node_id(nelx, nely, nelz, i, j, k):

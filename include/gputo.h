@@ -159,6 +159,31 @@ int gputo_level_apply(gputo_handle *h, int level, int op, int mode, int fp32, co
 /* Mean duration (ms, HIP events on the handle's stream) of `reps`
  * back-to-back matvecs of gputo_level_apply's operator. */
 int gputo_bench_level(gputo_handle *h, int level, int op, int fp32, int reps, double *avg_ms);
+/* ---- learned level-2 correction (params.learned = 1, mixed precision,
+ * nl >= 4).  No counterpart in the reference: an extra stage of the FP32
+ * V-cycle after the level-2 post-smoothing.  A 3-D convolutional
+ * encoder-decoder (4 -> 16 -> 32 -> 64 -> 32 -> 16 -> 3 channels, 3x3x3
+ * kernels, ReLU, strides 1/2/2, nearest upsampling with skip additions) maps
+ * the Jacobi-scaled level-2 defect d and a stiffness channel to a correction
+ * e; U += gamma e with gamma = <d,e>/<e,Ae> when gamma > 0, else the plain
+ * V-cycle result stands (the fallback).  The PCG then uses the flexible
+ * (Polak-Ribiere) beta.  Every solve logs level-2 defects as training
+ * samples; gputo_learn_train runs Adam on the level-2 energy
+ * (1/2 e^T A e - e^T d)/sigma^2 of those samples on the device.
+ * Parameters are a flat FP32 array: per layer W[cout][cin][27] then b[cout].
+ * Activations/inputs are channel-major on the level-2 node grid
+ * n0 x n1 x n2 = (nnx, nnz, nny), position (i*n1 + k)*n2 + j. */
+int gputo_learn_info(gputo_handle *h, long long *nparam, int *n0, int *n1, int *n2, int *nsamples);
+/* set != 0: load params from buf (marks the network trained); 0: read them. */
+int gputo_learn_params(gputo_handle *h, int set, double *buf);
+int gputo_learn_reset(gputo_handle *h, unsigned seed);
+/* y[3][n] = network(x[4][n]) */
+int gputo_learn_forward(gputo_handle *h, const double *x, double *y);
+/* g = d<gy, network(x)>/dparams */
+int gputo_learn_grad(gputo_handle *h, const double *x, const double *gy, double *g);
+/* `steps` Adam steps (learning rate lr) on the samples of the last solve. */
+int gputo_learn_train(gputo_handle *h, int steps, double lr, double *loss_first, double *loss_last);
+
 /* Coarsest-level solve x = A_c^{-1} b (cholsl1, mgcg1.cu:2175-2232). */
 int gputo_coarse_solve_f64(gputo_handle *h, const double *b, double *x);
 

@@ -1,0 +1,85 @@
+"""Learned level-2 correction: the HIP network (forward and parameter
+gradients) against the torch-CPU restatement in oracle/learn_oracle.py,
+training on the device, and the solve it accelerates.
+
+Tolerances: forward 1e-5 and gradients 1e-4 relative to the largest entry
+(FP32 activations and accumulation against float64).  The learned solve
+converges to the FP64 tolerance (the CG decides), so its displacements
+agree with the plain solve to that tolerance.
+"""
+import numpy as np
+import pytest
+
+import learn_oracle
+from conftest import cantilever_case
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def net_solver(gputo):
+    s = gputo.Solver(32, 16, 16, 4, precision=1, learned=True)
+    yield s
+    s.close()
+
+
+def test_network_layout(net_solver):
+    info = net_solver.learn_info()
+    assert info["nparam"] == learn_oracle.nparams()
+    assert info["grid"] == (9, 5, 5)  # level-2 nodes of 32x16x16: (nnx, nnz, nny)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_network_forward_and_gradient_match_cpu_oracle(net_solver, seed):
+    rng = np.random.default_rng(40 + seed)
+    p = rng.normal(size=learn_oracle.nparams()) * 0.05
+    net_solver.learn_params(p)
+    n0, n1, n2 = net_solver.learn_info()["grid"]
+    x = rng.normal(size=(4, n0, n1, n2))
+    gy = rng.normal(size=(3, n0, n1, n2))
+    p32 = net_solver.learn_params()  # the FP32 parameters the device holds
+    y_ref, g_ref = learn_oracle.run(p32, x.astype(np.float32).astype(np.float64), gy.astype(np.float32))
+    y = net_solver.learn_forward(x)
+    assert rel(y, y_ref) < 1e-5
+    g = net_solver.learn_grad(x, gy)
+    assert rel(g, g_ref) < 1e-4
+    # per layer as well, so that a small layer's error is not hidden
+    off = 0
+    for ci, co, _, _ in learn_oracle.LAYERS:
+        n = co * ci * 27 + co
+        assert rel(g[off:off + n], g_ref[off:off + n]) < 1e-3
+        off += n
+
+
+def test_training_lowers_the_level2_energy(gputo):
+    nelx, nely, nelz, nl = 48, 24, 24, 4
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=41, kind="smooth")
+    s = gputo.Solver(nelx, nely, nelz, nl, precision=1, learned=True)
+    U, r, n, rc = s.solve(Ke, F, CX, fixed, 1e-8, 200)
+    assert rc == 0 and s.learn_info()["nsamples"] > 0
+    l0, l1 = s.learn_train(200, 1e-3)
+    print("energy loss", l0, "->", l1)
+    assert l1 < l0 and l1 < 0.0  # below zero: the correction lowers the energy
+    s.close()
+
+
+def test_learned_solve_matches_plain(gputo):
+    nelx, nely, nelz, nl = 48, 24, 24, 4
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=42, kind="smooth")
+    s64 = gputo.Solver(nelx, nely, nelz, nl)
+    U64, r64, n64, rc = s64.solve(Ke, F, CX, fixed, 1e-10, 300)
+    s = gputo.Solver(nelx, nely, nelz, nl, precision=1, learned=True)
+    U0, r0, n0, rc0 = s.solve(Ke, F, CX, fixed, 1e-10, 300)  # untrained: logs samples only
+    s.learn_train(300, 1e-3)
+    U1, r1, n1, rc1 = s.solve(Ke, F, CX, fixed, 1e-10, 300)
+    print("iterations: fp64 %d, mixed %d, mixed+learned %d" % (n64, n0, n1))
+    assert rc0 == 0 and rc1 == 0 and r1 < 1e-10
+    assert rel(U1, U64) < 1e-7
+    C64, C1 = F @ U64, F @ U1
+    assert abs(C1 - C64) <= 1e-9 * abs(C64)
+    s.close()
+    s64.close()

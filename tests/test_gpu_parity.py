@@ -35,12 +35,21 @@ def solvers(gputo):
         s.close()
 
 
+def general_ke(Ke, seed=3):
+    """A symmetric Ke0 without the unit-H8 entry classes (exercises the
+    level-0 kernel that reads all 576 entries)."""
+    E = np.random.default_rng(seed).uniform(-1, 1, (24, 24)) * 1e-3
+    return Ke + (E + E.T)
+
+
 @pytest.mark.parametrize("mesh", MESHES)
-@pytest.mark.parametrize("kind", ["perturbed", "binary"])
+@pytest.mark.parametrize("kind", ["perturbed", "binary", "general_ke"])
 def test_matvec_defect_diag(solvers, oracle, mesh, kind):
     nelx, nely, nelz, nl = mesh
     print("case", mesh, kind, flush=True)
-    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=1, kind=kind)
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=1, kind="perturbed" if kind == "general_ke" else kind)
+    if kind == "general_ke":
+        Ke = general_ke(Ke)
     s = solvers(nelx, nely, nelz, nl)
     u = np.random.default_rng(2).uniform(-1, 1, s.ndof)
     y = s.matvec(Ke, CX, u, fixed)
@@ -52,6 +61,21 @@ def test_matvec_defect_diag(solvers, oracle, mesh, kind):
     assert rel(r, ro) < 1e-13
     ad = s.diag(Ke, CX)
     assert rel(ad, oracle.prepad0(Ke, CX, nelx, nely, nelz)) < 1e-14
+
+
+def test_class_and_general_ke_kernels_agree(solvers):
+    """The class-valued level-0 kernel and the general one compute the same
+    product for the unit-H8 Ke0 (the general one is forced by a Ke0 whose one
+    entry differs in the last bit, which the class check rejects)."""
+    nelx, nely, nelz, nl = 16, 8, 8, 3
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=9, kind="perturbed")
+    K2 = Ke.copy()
+    K2[0, 0] = np.nextafter(K2[0, 0], 1.0)
+    s = solvers(nelx, nely, nelz, nl)
+    u = np.random.default_rng(10).uniform(-1, 1, s.ndof)
+    y1 = s.matvec(Ke, CX, u, fixed)
+    y2 = s.matvec(K2, CX, u, fixed)
+    assert rel(y1, y2) < 1e-13
 
 
 def test_matvec_dense_bruteforce(solvers):
@@ -80,6 +104,20 @@ def test_hierarchy_levels(solvers, oracle, mesh, preset):
         assert np.array_equal(s.fixed_list(l), fdo)
         if l > 0:
             assert rel(A, Ao) < 1e-12, "element matrices of level %d" % l
+        assert rel(ad, ado) < 1e-12
+
+
+def test_hierarchy_levels_table_galerkin(solvers, oracle):
+    """Levels with at least 8192 elements are built from the per-fine-element
+    tables (level 2, 32x16x16, here)."""
+    nelx, nely, nelz, nl = 128, 64, 64, 4
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=7, kind="perturbed")
+    s = solvers(nelx, nely, nelz, nl, 0)
+    s.setup(Ke, CX, fixed)
+    for l in (2,):
+        A, ad, fd = s.level_matrices(l)
+        Ao, ado, fdo = oracle.level_data(Ke, CX, fixed, nelx, nely, nelz, nl, l)
+        assert rel(A, Ao) < 1e-12, "element matrices of level %d" % l
         assert rel(ad, ado) < 1e-12
 
 

@@ -37,7 +37,8 @@ GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_tables.js
 
 
 def ke_from_table(nu=0.3):
-    v = (np.array(GOLD["ke_a"], float) + nu * np.array(GOLD["ke_nu"], float)) / ((1 + nu) * (2 * nu - 1) * 144)
+    # evaluated as the MATLAB line does: 1/(1+nu)/(2*nu-1)/144 * (a + nu*b)
+    v = 1.0 / (1 + nu) / (2 * nu - 1) / 144 * (np.array(GOLD["ke_a"], float) + nu * np.array(GOLD["ke_nu"], float))
     flat = np.zeros(576)
     lower_colmajor = np.array([(p % 24) >= (p // 24) for p in range(576)])
     flat[lower_colmajor] = v
@@ -76,8 +77,11 @@ def test_ke0_matches_reference_table(oracle):
 
 
 def test_ke0_python_helper_matches(oracle):
-    from gputo.problems import ke0_h8
-    assert np.abs(ke0_h8() - ke_from_table()).max() < 1e-15
+    from gputo.problems import ke0_h8, ke_h8_gauss
+    # the integer-table form reproduces the reference's Ke0 bit for bit
+    assert np.array_equal(ke0_h8(), ke_from_table())
+    assert np.array_equal(ke0_h8(0.2), ke_from_table(0.2))
+    assert np.abs(ke_h8_gauss() - ke_from_table()).max() < 1e-15
 
 
 def test_prolongation_table(oracle):
@@ -308,3 +312,24 @@ def test_sensitivity_frame_invariance(oracle):
     for e in range(cmat.shape[0]):
         ue = np.concatenate([Uc[3 * n:3 * n + 3] for n in en[e]])
         assert abs(ue @ Ke @ ue - ce[e]) < 1e-12 * abs(ce[e])
+
+
+def test_learn_oracle_gradient_finite_differences():
+    """The torch restatement of the network: autograd against central
+    differences of <gy, y> on a tiny grid."""
+    import learn_oracle
+    rng = np.random.default_rng(5)
+    p = rng.normal(size=learn_oracle.nparams()) * 0.05
+    x = rng.normal(size=(4, 5, 3, 3))
+    gy = rng.normal(size=(3, 5, 3, 3))
+    y, g = learn_oracle.run(p, x, gy)
+    assert y.shape == (3, 5, 3, 3)
+    eps = 1e-6
+    for idx in rng.choice(learn_oracle.nparams(), 12, replace=False):
+        pp = p.copy()
+        pp[idx] += eps
+        y1, _ = learn_oracle.run(pp, x)
+        pp[idx] -= 2 * eps
+        y2, _ = learn_oracle.run(pp, x)
+        fd = ((y1 - y2) * gy).sum() / (2 * eps)
+        assert abs(fd - g[idx]) <= 1e-6 * max(1.0, abs(g[idx]))
