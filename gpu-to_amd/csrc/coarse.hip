@@ -285,6 +285,71 @@ std::vector<std::vector<double>> galerkin_tables(const double *Ke, const double 
 }
 
 // ---------------------------------------------------------------------------
+// mgcg9 coarse levels (Homogenization/QNC/mgcg9.cu).
+// formCXc (:753-789): CX_c = 2 * mean of the 8 children, summed in the
+// reference's child order.
+__global__ void k_coarse_cx(Grid gc, const double *__restrict__ CXf, double *__restrict__ CXc)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= gc.nel) return;
+    const int j = (int)(e % gc.ny), k = (int)((e / gc.ny) % gc.nz), i = (int)(e / ((long)gc.ny * gc.nz));
+    const long fny = 2L * gc.ny, fnz = 2L * gc.nz;
+    double s = 0.0;
+    for (int iP = 0; iP < 2; ++iP)
+        for (int kP = 0; kP < 2; ++kP)
+            for (int jP = 0; jP < 2; ++jP) s += CXf[fnz * fny * (2 * i + iP) + fny * (2 * k + kP) + 2 * j + jP];
+    CXc[e] = 2.0 * (s / 8);
+}
+
+void launch_coarse_cx(hipStream_t s, const Grid &gc, const double *CXf, double *CXc)
+{
+    k_coarse_cx<<<(unsigned)((gc.nel + 255) / 256), 256, 0, s>>>(gc, CXf, CXc);
+}
+
+// packed element matrices CX_e Ke (assembAQc's element products, :1747-...)
+__global__ void k_ke_elements(Grid g, const double *__restrict__ CX, const double *__restrict__ Ke,
+                              double *__restrict__ A)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 300 * g.nel) return;
+    const int p = (int)(q / g.nel);
+    const long e = q % g.nel;
+    int m = 0;
+    while ((m + 1) * (m + 2) / 2 <= p) ++m;
+    const int n = p - m * (m + 1) / 2;
+    A[q] = CX[e] * Ke[24 * m + n];
+}
+
+void launch_ke_elements(hipStream_t s, const Grid &g, const double *CX, const double *Ke, double *A)
+{
+    k_ke_elements<<<(unsigned)((300 * g.nel + 255) / 256), 256, 0, s>>>(g, CX, Ke, A);
+}
+
+// CorrCsAQc (:1005-1028): zero the rows and columns of the fixed DOFs of
+// the dense coarsest matrix (N x N, row-major), then 1e10 on their diagonal
+__global__ void k_zero_rowcol(int N, const int *__restrict__ fixed, int nfd, double *__restrict__ AQ)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long)nfd * N) return;
+    const long d = fixed[q / N], i = q % N;
+    AQ[d * N + i] = 0.0;
+    AQ[i * N + d] = 0.0;
+}
+
+__global__ void k_fixed_diag(int N, const int *__restrict__ fixed, int nfd, double *__restrict__ AQ)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nfd) AQ[(long)fixed[f] * N + fixed[f]] = 1e10;
+}
+
+void launch_zero_rowcol(hipStream_t s, int N, const int *fixed_d, int nfd, double *AQ)
+{
+    if (nfd <= 0) return;
+    k_zero_rowcol<<<(unsigned)(((long)nfd * N + 255) / 256), 256, 0, s>>>(N, fixed_d, nfd, AQ);
+    k_fixed_diag<<<(unsigned)((nfd + 255) / 256), 256, 0, s>>>(N, fixed_d, nfd, AQ);
+}
+
+// ---------------------------------------------------------------------------
 // Coarsest-level penalty (CorrCsAc, mgcg1.cu:1368-1402): the diagonal of
 // every incident element matrix is set to 1e10 for each fixed DOF.  The
 // reference's table {5,6,1,2,4,7,0,3} is the y-mirror of the node's own

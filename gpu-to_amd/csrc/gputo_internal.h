@@ -21,7 +21,7 @@ constexpr int kBlock = 256;          // threads per block of the gather kernels
 constexpr int kStNodes = kBlock / 8; // nodes per block of the stored-matrix operator
 constexpr int kRedBlocks = 1024;     // blocks of the two-stage reductions
 constexpr int kMaxLevels = 12;
-constexpr int kMaxCoarseDofs = 16384;
+constexpr int kMaxCoarseDofs = 20000;  // dense coarsest solve (mgcg9.cu:74, maxDofs0)
 constexpr int kKeMaxClasses = 16;    // Ke0 entry classes passed by value to the level-0 kernels
 
 // One structured hex grid: elements nx*ny*nz, nodes (nx+1)(ny+1)(nz+1);
@@ -38,8 +38,20 @@ Grid make_grid(int nx, int ny, int nz);
 struct Tile {
     int TJ, TK, TI;
     int ntj, ntk, nti;
-    long blocks() const { return (long)ntj * ntk * nti; }
+    __host__ __device__ long blocks() const { return (long)ntj * ntk * nti; }
 };
+
+// XCD-aware block order.  Consecutive blocks are dealt round-robin over the
+// 8 XCDs, each with its own L2; launching 8*ceil(n/8) blocks and giving XCD x
+// the contiguous tile range [x*m, (x+1)*m), m = ceil(n/8), keeps
+// neighbouring tiles, which share halo nodes, on one L2.
+constexpr int kXcds = 8;
+inline long xcd_launch(long ntiles) { return (ntiles + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ long xcd_tile(long b, long ntiles)
+{
+    const long m = (ntiles + kXcds - 1) / kXcds;
+    return (b % kXcds) * m + b / kXcds;
+}
 
 Tile choose_tile(const Grid &g);   // level-0 node-gather bricks
 Tile stored_tile(const Grid &g);   // stored-matrix operator (at most kStNodes nodes)
@@ -61,6 +73,7 @@ struct OpArgsT {
     double *partials;      // per-block partial sums (fused dot), may be null
     const int *ctl;        // ctl[0] != 0: skip (converged)
     double omega, prerelax;
+    double cx_floor;       // > 0: element scales below it count as 0 (mgcg9 smoother defect)
     int kc;                // Ke0 has the unit-H8 class structure: use kv
     T kv[kKeMaxClasses];   // value of each Ke0 entry class (kKeClass)
 };
@@ -88,6 +101,8 @@ struct Level {
     Grid g;
     Tile t;
     Tile tmf;                     // tiling of the matrix-free level-1 operator
+    Tile tg;                      // node-gather tiling of this level (matrix-free levels)
+    double *CXl = nullptr;        // element scales of a matrix-free coarse level (mgcg9)
     double *U = nullptr, *F = nullptr, *R = nullptr, *AD = nullptr, *T = nullptr;
     double *A = nullptr;          // packed element matrices (levels >= 1)
     uint8_t *mask = nullptr;      // fixed-DOF node mask
@@ -97,6 +112,7 @@ struct Level {
 
 struct Coarse {
     int n = 0;
+    bool direct = true;      // factored (else smoothed: mgcg9 above 20000 DOFs)
     int bw = 0;              // lower bandwidth of the assembled matrix (node-major DOFs)
     double *AQ = nullptr;    // dense matrix, then its Cholesky factor (lower)
     double *Linv = nullptr;  // inverse of the factor (lower, row-major)
@@ -170,6 +186,11 @@ constexpr int kGtMaxDepth = 4;  // deepest level built from tables (8^4 x 300 do
 // with 3456 elements took 263 us from its table).
 constexpr long kGtMinElems = 8192;
 void launch_corrcsa(hipStream_t s, const Grid &g, const int *fixed_d, int nfd, double *A, int mirrored);
+// mgcg9 coarse levels: CX_c = 2 mean(children), element matrices CX Ke, and
+// the zeroing of fixed rows/columns of the dense coarsest matrix
+void launch_coarse_cx(hipStream_t s, const Grid &gc, const double *CXf, double *CXc);
+void launch_ke_elements(hipStream_t s, const Grid &g, const double *CX, const double *Ke, double *A);
+void launch_zero_rowcol(hipStream_t s, int N, const int *fixed_d, int nfd, double *AQ);
 void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ);
 int coarse_factor(hipStream_t s, Coarse &C);
 template <class T>

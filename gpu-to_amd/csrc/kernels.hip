@@ -44,11 +44,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
     __shared__ T su[3][kMaxHalo];
     __shared__ T sE[kMaxElemBox];
     __shared__ T sK[KC ? 1 : 576];  // general Ke: wave-uniform (broadcast) LDS reads
-    if (!KC)
-        for (int q = threadIdx.x; q < 576; q += kBlock) sK[q] = a.Ke[q];
     const Grid g = a.g;
     const Tile t = a.t;
-    int b = blockIdx.x;
+    const long tb = xcd_tile(blockIdx.x, t.blocks());
+    if (tb >= t.blocks()) {  // padding block of the XCD order
+        if (a.partials && threadIdx.x == 0) a.partials[blockIdx.x] = 0.0;
+        return;
+    }
+    if (!KC)
+        for (int q = threadIdx.x; q < 576; q += kBlock) sK[q] = a.Ke[q];
+    int b = (int)tb;
     const int bj = b % t.ntj;
     b /= t.ntj;
     const int bk = b % t.ntk;
@@ -56,35 +61,45 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
     const int j0 = bj * t.TJ, k0 = bk * t.TK, i0 = bi * t.TI;
     const int HJ = t.TJ + 2, HK = t.TK + 2, HI = t.TI + 2;
 
+    // Stage the halo brick one j-row per wave (rows are contiguous in the DOF
+    // vector, so the loads coalesce) and the element box the same way; the
+    // row split is wave-uniform, so no per-lane integer division is needed.
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rowlen = 3 * HJ;
-    for (int q = threadIdx.x; q < rowlen * HK * HI; q += kBlock) {
-        const int row = q / rowlen;
-        const int col = q - row * rowlen;
-        const int hk = row % HK, hi = row / HK;
-        const int hj = col / 3, c = col - 3 * hj;
-        const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
-        T v = T(0);
-        if (gi >= 0 && gi < g.nnx && gj >= 0 && gj < g.nny && gk >= 0 && gk < g.nnz) {
-            const long dof = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + gj) + c;
-            if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
-                v = T(a.prerelax) * a.F[dof] / a.AD[dof];
-            else
-                v = a.U[dof];
+    for (int row = wave; row < HK * HI; row += kBlock / 64) {
+        const int hi = row / HK, hk = row - hi * HK;
+        const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
+        const bool rowok = gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
+        const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
+        for (int col = lane; col < rowlen; col += 64) {
+            const int hj = col / 3, c = col - 3 * hj;
+            const int gj = j0 - 1 + hj;
+            T v = T(0);
+            if (rowok && gj >= 0 && gj < g.nny) {
+                const long dof = base + col;
+                if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
+                    v = T(a.prerelax) * a.F[dof] / a.AD[dof];
+                else
+                    v = a.U[dof];
+            }
+            su[c][row * HJ + hj] = v;
         }
-        su[c][(hi * HK + hk) * HJ + hj] = v;
     }
     const int EJ = t.TJ + 1, EK = t.TK + 1;
     {
         const int EI = t.TI + 1;
-        for (int q = threadIdx.x; q < EJ * EK * EI; q += kBlock) {
-            const int ej = q % EJ;
-            const int r = q / EJ;
-            const int ek = r % EK, ei = r / EK;
-            const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = k0 - 1 + ek;
-            T v = T(0);
-            if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz)
-                v = a.CX[(long)g.ny * g.nz * gi + (long)g.ny * gk + gj];
-            sE[q] = v;
+        for (int row = wave; row < EK * EI; row += kBlock / 64) {
+            const int ei = row / EK, ek = row - ei * EK;
+            const int gi = i0 - 1 + ei, gk = k0 - 1 + ek;
+            const bool rowok = gi >= 0 && gi < g.nx && gk >= 0 && gk < g.nz;
+            for (int ej = lane; ej < EJ; ej += 64) {
+                const int gj = j0 - 1 + ej;
+                T v = T(0);
+                if (rowok && gj >= 0 && gj < g.ny) v = a.CX[(long)g.ny * g.nz * gi + (long)g.ny * gk + gj];
+                if (a.cx_floor > 0.0 && v < T(a.cx_floor)) v = T(0);  // mgcg9 defect0c skip
+                sE[row * EJ + ej] = v;
+            }
         }
     }
     __syncthreads();
@@ -228,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
 template <class T>
 void launch_op0(hipStream_t s, int mode, const OpArgsT<T> &a)
 {
-    const dim3 grid((unsigned)a.t.blocks()), block(kBlock);
+    const dim3 grid((unsigned)xcd_launch(a.t.blocks())), block(kBlock);
     if (a.kc) {
         switch (mode) {
         case OP_MATVEC: k_gather<T, OP_MATVEC, true><<<grid, block, 0, s>>>(a); break;

@@ -22,6 +22,16 @@ namespace gputo {
 constexpr int kL1Halo = 1024;  // halo nodes per block
 constexpr int kL1Own = 512;    // owned nodes per block
 
+// q / d for 0 <= q < 2^22, d >= 1, with inv = 1/d: the float quotient is
+// within one of the true one and is corrected both ways
+__device__ __forceinline__ int udiv_small(int q, int d, float inv)
+{
+    int r = (int)((float)q * inv);
+    r -= (r * d > q) ? 1 : 0;
+    r += ((r + 1) * d <= q) ? 1 : 0;
+    return r;
+}
+
 template <class T>
 __device__ __forceinline__ T lane_w(int x, T xi)
 {
@@ -47,22 +57,27 @@ __global__ __launch_bounds__(256) void k_l1_mf(OpArgsT<T> a, Tile t, const T *__
     const int EJ = t.TJ + 1, EK = t.TK + 1, EI = t.TI + 1;
     const int nown = t.TJ * t.TK * t.TI;
 
+    // halo brick, one j-row per wave (coalesced rows, wave-uniform row split)
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
     const int rowlen = 3 * HJ;
-    for (int q = threadIdx.x; q < rowlen * HK * HI; q += 256) {
-        const int row = q / rowlen;
-        const int col = q - row * rowlen;
-        const int hk = row % HK, hi = row / HK;
-        const int hj = col / 3, c = col - 3 * hj;
-        const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
-        T v = T(0);
-        if (gi >= 0 && gi < g.nnx && gj >= 0 && gj < g.nny && gk >= 0 && gk < g.nnz) {
-            const long dof = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + gj) + c;
-            if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
-                v = T(a.prerelax) * a.F[dof] / a.AD[dof];
-            else
-                v = a.U[dof];
+    for (int row = wv; row < HK * HI; row += 4) {
+        const int hi = row / HK, hk = row - hi * HK;
+        const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
+        const bool rowok = gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
+        const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
+        for (int col = ln; col < rowlen; col += 64) {
+            const int hj = col / 3;
+            const int gj = j0 - 1 + hj;
+            T v = T(0);
+            if (rowok && gj >= 0 && gj < g.nny) {
+                const long dof = base + col;
+                if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
+                    v = T(a.prerelax) * a.F[dof] / a.AD[dof];
+                else
+                    v = a.U[dof];
+            }
+            su[3 * row * HJ + col] = v;
         }
-        su[3 * ((hi * HK + hk) * HJ + hj) + c] = v;
     }
     for (int q = threadIdx.x; q < 3 * nown; q += 256) acc[q] = T(0);
     __syncthreads();
@@ -82,15 +97,17 @@ __global__ __launch_bounds__(256) void k_l1_mf(OpArgsT<T> a, Tile t, const T *__
         const int pj = c & 1, pk = (c >> 1) & 1, pi = (c >> 2) & 1;
         const int nj = (EJ - pj + 1) / 2, nk = (EK - pk + 1) / 2, ni = (EI - pi + 1) / 2;
         const int cnt = nj * nk * ni;
+        const float inv_nj = 1.0f / (float)nj, inv_nk = 1.0f / (float)nk;
         for (int q0 = 0; q0 < cnt; q0 += 32) {
             const int q = q0 + grp;
             int ej = 0, ek = 0, ei = 0;
             bool valid = q < cnt;
             if (valid) {
-                ej = pj + 2 * (q % nj);
-                const int r = q / nj;
-                ek = pk + 2 * (r % nk);
-                ei = pi + 2 * (r / nk);
+                const int r = udiv_small(q, nj, inv_nj);
+                ej = pj + 2 * (q - r * nj);
+                const int r2 = udiv_small(r, nk, inv_nk);
+                ek = pk + 2 * (r - r2 * nk);
+                ei = pi + 2 * r2;
                 const int ex = i0 - 1 + ei, ey = j0 - 1 + ej, ez = k0 - 1 + ek;
                 valid = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz;
             }

@@ -46,6 +46,7 @@ void mixed_create(gputo_handle *h)
             float *A = malloc_dev<float>(m, 300 * g.nel);
             m->A32[l] = A;
             v.A = A;
+            v.CX = m->CXl32[l] = malloc_dev<float>(m, g.nel);
         }
     }
     const Grid &g0 = h->L[0].g;
@@ -53,6 +54,7 @@ void mixed_create(gputo_handle *h)
     m->CX32 = malloc_dev<float>(m, g0.nel);
     m->H.Ke = m->Ke32;
     m->H.CX = m->CX32;
+    m->H.L[0].CX = m->CX32;
     m->X = malloc_dev<float>(m, g0.ndof);
     m->R = malloc_dev<float>(m, g0.ndof);
     m->P = malloc_dev<float>(m, g0.ndof);
@@ -104,10 +106,16 @@ static void mixed_prepare(gputo_handle *h)
     launch_copy<double, float>(s, h->Ke, m->Ke32, 576, nullptr);
     launch_copy<double, float>(s, h->CX, m->CX32, g0.nel, nullptr);
     m->H.l1_mf = use_l1_mf(h->prm, h->nl);
+    m->H.mg9 = preset_of(h->prm).mg9;
+    if (m->H.mg9 && !h->C.direct) launch_fill<float>(s, m->H.L[h->nl - 1].U, h->L[h->nl - 1].g.ndof, 0.0);
     for (int l = 0; l < h->nl; ++l) {
         const Level &v = h->L[l];
         launch_copy<double, float>(s, v.AD, m->H.L[l].AD, v.g.ndof, nullptr);
-        if (l > 0 && !(l == 1 && m->H.l1_mf)) launch_copy<double, float>(s, v.A, m->A32[l], 300 * v.g.nel, nullptr);
+        if (m->H.mg9) {
+            if (l > 0) launch_copy<double, float>(s, v.CXl, m->CXl32[l], v.g.nel, nullptr);
+        } else if (l > 0 && !(l == 1 && m->H.l1_mf)) {
+            launch_copy<double, float>(s, v.A, m->A32[l], 300 * v.g.nel, nullptr);
+        }
     }
     m->H.l0 = l0_kind(h->prm, h->prm.precision);
     m->H.l0_ctx = m;

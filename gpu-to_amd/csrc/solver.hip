@@ -224,10 +224,12 @@ HierView<double> f64_view(const gputo_handle *h)
         H.L[l].AD = v.AD;
         H.L[l].Tmp = v.T;
         H.L[l].A = v.A;
+        H.L[l].CX = l == 0 ? h->CX : v.CXl;
     }
     H.Ke = h->Ke;
     H.CX = h->CX;
     H.l1_mf = use_l1_mf(h->prm, h->nl);
+    H.mg9 = preset_of(h->prm).mg9;
     return H;
 }
 
@@ -258,7 +260,8 @@ namespace gputo {
 // and whenever Ke changes when level 1 is itself Galerkin.
 static void ensure_tables(gputo_handle *h)
 {
-    const bool gal1 = preset_of(h->prm.preset).l1_galerkin;
+    if (preset_of(h->prm).mg9) return;  // no Galerkin levels
+    const bool gal1 = preset_of(h->prm).l1_galerkin;
     if (h->gt_max < (gal1 ? 1 : 2)) return;
     std::vector<double> ke;
     if (gal1) {
@@ -294,7 +297,7 @@ static void ensure_tables(gputo_handle *h)
 // (initVcycle, mgcg1.cu:326-466).
 int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
 {
-    const Preset p = preset_of(h->prm.preset);
+    const Preset p = preset_of(h->prm);
     hipStream_t s = h->stream;
     Level &L0 = h->L[0];
     if (int rc = validate_fixed(fixed0, L0.g.ndof)) return rc;
@@ -310,6 +313,33 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
         HIPCHK(hipStreamSynchronize(s));  // host vectors above are temporaries
     }
     launch_diag0(s, L0.g, h->Ke, h->CX, L0.AD);
+    if (p.mg9) {
+        // mgcg9 (initVcycle, mgcg9.cu:342-414): coarse element scales and
+        // diagonals from the unit Ke; the coarsest matrix, when factored,
+        // is CX*Ke assembled with its fixed rows and columns decoupled
+        for (int l = 1; l < h->nl; ++l) {
+            Level &v = h->L[l];
+            launch_coarse_cx(s, v.g, l == 1 ? h->CX : h->L[l - 1].CXl, v.CXl);
+            launch_diag0(s, v.g, h->Ke, v.CXl, v.AD);
+        }
+        Level &vc = h->L[h->nl - 1];
+        h->C.direct = vc.g.ndof < kMaxCoarseDofs;
+        // a smoothed coarsest level starts each solve from zero and then
+        // keeps its iterate from one V-cycle to the next (mgcg9.cu:553-563)
+        if (!h->C.direct) launch_fill<double>(s, vc.U, vc.g.ndof, 0.0);
+        int rc = 0;
+        if (h->C.direct) {
+            launch_ke_elements(s, vc.g, vc.CXl, h->Ke, vc.A);
+            launch_assemble(s, vc.g, vc.A, h->C.AQ);
+            launch_zero_rowcol(s, (h->C.n + 31) / 32 * 32, vc.fixed_d, (int)vc.fixed.size(), h->C.AQ);
+            HIPCHK(hipGetLastError());
+            rc = coarse_factor(s, h->C);
+        }
+        HIPCHK(hipGetLastError());
+        h->hier_ready = (rc == 0);
+        return rc;
+    }
+    h->C.direct = true;
     for (int l = 1; l < h->nl; ++l) {
         Level &v = h->L[l];
         if (l == 1 && !p.l1_galerkin)
@@ -351,7 +381,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
-    const int nmv = (int)L0.t.blocks();
+    const int nmv = (int)xcd_launch(L0.t.blocks());  // partials of the level-0 matvec
     launch_fill(s, U, n, 0.0);
     launch_copy(s, F, h->R, n, nullptr);
     launch_dot_partial(s, F, F, n, h->partials, nullptr);
@@ -365,6 +395,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     for (;;) {
         for (int c = 0; c < chunk && enq < maxit; ++c, ++enq) {
             OpArgs a = op_args(h, 0);
+            a.cx_floor = 0.0;  // the CG matvec keeps every element (multAV0c, mgcg9.cu:791-848)
             a.U = h->P;
             a.out = h->Q;
             a.partials = h->partials;
@@ -455,7 +486,10 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
     const int f = 1 << (nl - 1);
     if (nelx % f || nely % f || nelz % f) return GPUTO_ERR_ARGUMENT;
     const Grid gc = make_grid(nelx / f, nely / f, nelz / f);
-    if (gc.ndof > kMaxCoarseDofs) return GPUTO_ERR_WORKSPACE;
+    // a coarsest level too large for the dense solve is only allowed for the
+    // mgcg9 engine, which smooths it instead
+    const bool dense_ok = gc.ndof <= kMaxCoarseDofs;
+    if (!dense_ok && !(params && params->preset == GPUTO_PRESET_MGCG9)) return GPUTO_ERR_WORKSPACE;
     // the learned stage works on level 2 of the FP32 V-cycle
     if (params && params->learned && (params->precision == GPUTO_PREC_F64 || nl < 4)) return GPUTO_ERR_ARGUMENT;
     gputo_handle *h = new gputo_handle();
@@ -472,7 +506,9 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
             Level &v = h->L[l];
             v.g = make_grid(nelx >> l, nely >> l, nelz >> l);
             v.t = l == 0 ? choose_tile(v.g) : stored_tile(v.g);
-            maxblocks = std::max<long>(maxblocks, v.t.blocks());
+            v.tg = choose_tile(v.g);
+            maxblocks = std::max<long>(maxblocks, xcd_launch(v.t.blocks()));
+            maxblocks = std::max<long>(maxblocks, xcd_launch(v.tg.blocks()));
             if (l == 1) {
                 v.tmf = l1_tile(v.g);
                 maxblocks = std::max<long>(maxblocks, v.tmf.blocks());
@@ -486,6 +522,7 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
                 v.U = dalloc<double>(h, v.g.ndof);
                 v.F = dalloc<double>(h, v.g.ndof);
                 v.A = dalloc<double>(h, 300 * v.g.nel);
+                v.CXl = dalloc<double>(h, v.g.nel);
             }
         }
         const Grid &g0 = h->L[0].g;
@@ -512,10 +549,13 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
             C.bw = 3 * (gc.nny * gc.nnz + gc.nny + 1) + 2;
         }
         const long N = (C.n + 31) / 32 * 32;
-        C.AQ = dalloc<double>(h, N * N);
-        C.Linv = dalloc<double>(h, N * N);
-        C.LinvT = dalloc<double>(h, N * N);
-        C.y = dalloc<double>(h, N);
+        if (dense_ok) {
+            C.AQ = dalloc<double>(h, N * N);
+            C.Linv = dalloc<double>(h, N * N);
+            C.LinvT = dalloc<double>(h, N * N);
+            C.y = dalloc<double>(h, N);
+        }
+        C.direct = dense_ok;
         C.flag = dalloc<int>(h, 1);
         std::vector<double> tab = level1_gauss_table(0.3, 2.0);
         HIPCHK(hipMemcpy(h->Aelm, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -552,6 +592,7 @@ int gputo_set_params(gputo_handle *h, const gputo_params *params)
     if (!!params->learned != !!h->prm.learned) return GPUTO_ERR_ARGUMENT;  // fixed at creation
     if (params->precision != h->prm.precision && params->precision == GPUTO_PREC_F64 && h->ln)
         return GPUTO_ERR_ARGUMENT;  // the learned stage runs in the FP32 V-cycle
+    if (!h->C.AQ && params->preset != GPUTO_PRESET_MGCG9) return GPUTO_ERR_WORKSPACE;  // no dense coarsest
     if (params->preset != h->prm.preset) h->hier_ready = false;
     h->prm = *params;
     try {
@@ -667,6 +708,7 @@ static int host_op(gputo_handle *h, int mode, const double *Ke0, const double *C
             if (F) HIPCHK(hipMemcpyAsync(h->Fv, F, g0.ndof * sizeof(double), hipMemcpyHostToDevice, s));
             OpArgs a = op_args(h, 0);
             a.ctl = nullptr;
+            a.cx_floor = 0.0;  // multAV0c / defect0c of mgcg1 keep every element
             a.U = h->P;
             a.F = h->Fv;
             a.out = h->Q;
@@ -783,6 +825,21 @@ int gputo_level_matrices(gputo_handle *h, int level, double *A, double *ad, doub
     }
 }
 
+int gputo_level_cx(gputo_handle *h, int level, double *cx)
+{
+    if (!h || level < 0 || level >= h->nl || !h->hier_ready || !cx || !preset_of(h->prm).mg9)
+        return GPUTO_ERR_ARGUMENT;
+    try {
+        const Level &v = h->L[level];
+        HIPCHK(hipMemcpyAsync(cx, level == 0 ? h->CX : v.CXl, v.g.nel * sizeof(double), hipMemcpyDeviceToHost,
+                              h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
 int gputo_fixed_list(const gputo_handle *h, int level, double *fixed, int *nfd)
 {
     if (!h || level < 0 || level >= h->nl) return GPUTO_ERR_ARGUMENT;
@@ -881,7 +938,7 @@ void level_apply_t(gputo_handle *h, HierView<T> H, int level, int op, int mode, 
 int level_args_ok(const gputo_handle *h, int level, int op, int mode)
 {
     if (!h || !h->hier_ready || level < 0 || level >= h->nl - 1 || mode < 0 || mode > 2) return 0;
-    if (op == 2) return level == 1 && !preset_of(h->prm.preset).l1_galerkin;
+    if (op == 2) return level == 1 && !preset_of(h->prm).l1_galerkin && !preset_of(h->prm).mg9;
     return op == 1;
 }
 

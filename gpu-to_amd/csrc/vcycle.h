@@ -13,14 +13,21 @@ struct Preset {
     int npre, npost;
     double omega, prerelax;
     int l1_galerkin, zero_after_interp, corr_mirrored;
+    int mg9;           // every level matrix-free with coarse element scales
+    double cx_floor;   // smoother defects skip elements with CX below this
 };
 
-inline Preset preset_of(int p)
+inline Preset preset_of(const gputo_params &prm)
 {
     // REF_C: mgcg1.c:110,308,353,363-364, mgcg1.cu:188,631,1196,1205;
-    // MATLAB_CFG0: top3D125mgcg.m:147,194,243 (omega 0.6, one sweep from zero)
-    if (p == GPUTO_PRESET_MATLAB_CFG0) return {0, 1, 0.6, 0.6, 1, 1, 0};
-    return {3, 3, 0.48, 0.7, 0, 1, 1};
+    // MATLAB_CFG0: top3D125mgcg.m:147,194,243 (omega 0.6, one sweep from zero);
+    // MGCG9: mgcg9.cu:920-937,939-996 (0.7 prerelax, omega 0.33, nswp sweeps)
+    if (prm.preset == GPUTO_PRESET_MATLAB_CFG0) return {0, 1, 0.6, 0.6, 1, 1, 0, 0, 0.0};
+    if (prm.preset == GPUTO_PRESET_MGCG9) {
+        const int n = prm.nswp > 0 ? prm.nswp : 2;
+        return {n, n, 0.33, 0.7, 0, 1, 0, 1, 1e-8};
+    }
+    return {3, 3, 0.48, 0.7, 0, 1, 1, 0, 0.0};
 }
 
 // Level-0 operator implementation of a view.
@@ -30,7 +37,8 @@ enum L0Kind { L0_AUTO = 0, L0_GATHER = 1, L0_MFMA_F32 = 2, L0_MFMA_BF16 = 3, L0_
 template <class T>
 struct LevelView {
     T *U = nullptr, *F = nullptr, *R = nullptr, *AD = nullptr, *Tmp = nullptr;
-    const T *A = nullptr;  // packed element matrices (levels >= 1)
+    const T *A = nullptr;   // packed element matrices (levels >= 1)
+    const T *CX = nullptr;  // element scales of a matrix-free level (mgcg9)
 };
 
 template <class T>
@@ -42,6 +50,7 @@ struct HierView {
     const void *l0_ctx = nullptr;  // state of a non-gather level-0 kernel
     bool l1_mf = false;            // level 1 applied matrix-free from CX
     bool learn = false;            // learned level-2 stage (FP32 views only)
+    bool mg9 = false;              // every level matrix-free (mgcg9 engine)
 };
 
 constexpr int kLearnLevel = 2;
@@ -61,7 +70,8 @@ constexpr double kL1Nu = 0.3;  // Poisson ratio of the level-1 rediscretisation 
 inline bool use_l1_mf(const gputo_params &prm, int nl)
 {
     if (nl < 3) return false;  // level 1 is the coarsest: its stored matrix feeds the Cholesky
-    const bool rediscretised = !preset_of(prm.preset).l1_galerkin;
+    if (preset_of(prm).mg9) return false;
+    const bool rediscretised = !preset_of(prm).l1_galerkin;
     if (prm.l1_kernel == 1) return false;
     return rediscretised;
 }
@@ -78,9 +88,14 @@ OpArgsT<T> view_args(const gputo_handle *h, const HierView<T> &H, int l, const i
     a.mask = h->L[l].mask;
     a.AD = H.L[l].AD;
     a.ctl = ctl;
-    const Preset p = preset_of(h->prm.preset);
+    const Preset p = preset_of(h->prm);
     a.omega = p.omega;
     a.prerelax = p.prerelax;
+    if (p.mg9) {
+        a.CX = H.L[l].CX;
+        a.t = h->L[l].tg;
+        a.cx_floor = p.cx_floor;
+    }
     a.kc = h->ke_classes;
     for (int k = 0; k < kKeMaxClasses; ++k) a.kv[k] = (T)h->kv[k];
     return a;
@@ -89,7 +104,9 @@ OpArgsT<T> view_args(const gputo_handle *h, const HierView<T> &H, int l, const i
 template <class T>
 void level_op(const gputo_handle *h, const HierView<T> &H, int l, int mode, const OpArgsT<T> &a)
 {
-    if (l == 1 && H.l1_mf) {
+    if (H.mg9) {
+        launch_op0<T>(h->stream, mode, a);  // level scales and the CX floor are in a
+    } else if (l == 1 && H.l1_mf) {
         launch_op1_mf<T>(h->stream, mode, a, H.CX, h->L[1].tmf, kL1Nu);
     } else if (l > 0) {
         launch_opA<T>(h->stream, mode, a);
@@ -129,7 +146,7 @@ void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T 
 template <class T>
 void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, const int *ctl)
 {
-    const Preset p = preset_of(h->prm.preset);
+    const Preset p = preset_of(h->prm);
     const int nl = h->nl;
     for (int l = 0; l < nl; ++l) {
         const LevelView<T> &v = H.L[l];
@@ -152,6 +169,10 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
                 a.out = v.R;
                 level_op(h, H, l, OP_DEFECT, a);
             }
+        } else if (!h->C.direct) {
+            // coarsest level too large to factor: 10*npre sweeps from the
+            // level's previous iterate (mgcg9.cu:553-563)
+            sweeps_t(h, H, l, F, U, 10 * p.npre, false, ctl);
         } else {
             launch_coarse_solve<T>(h->stream, h->C, F, U, lv.mask, ctl);
         }

@@ -20,6 +20,7 @@ GPUTO_ERR_ARGUMENT = -5
 
 PRESET_REF_C = 0
 PRESET_MATLAB_CFG0 = 1
+PRESET_MGCG9 = 2
 
 PREC_F64 = 0
 PREC_MIXED_F32 = 1
@@ -45,6 +46,7 @@ class GputoParams(ctypes.Structure):
         ("learned", ctypes.c_int),
         ("print_level", ctypes.c_int),
         ("l0_kernel", ctypes.c_int),
+        ("nswp", ctypes.c_int),
         ("l1_kernel", ctypes.c_int),
     ]
 
@@ -112,6 +114,7 @@ SIGNATURES = {
     "gputo_setup": (_I, [_P, _P, _P, _P, _I]),
     "gputo_level_info": (_I, [_P, _I, _pI, _pI, _pI, _pI]),
     "gputo_level_matrices": (_I, [_P, _I, _P, _P, _P]),
+    "gputo_level_cx": (_I, [_P, _I, _P]),
     "gputo_restrict_f64": (_I, [_P, _I, _P, _P]),
     "gputo_interp_f64": (_I, [_P, _I, _P, _P]),
     "gputo_coarse_solve_f64": (_I, [_P, _P, _P]),

@@ -27,7 +27,7 @@ class Solver:
     """One libgputo handle for an nelx x nely x nelz mesh with nl levels."""
 
     def __init__(self, nelx, nely, nelz, nl, preset=_lib.PRESET_REF_C, precision=_lib.PREC_F64, learned=False,
-                 print_level=0, l0_kernel=0, l1_kernel=0):
+                 print_level=0, l0_kernel=0, l1_kernel=0, nswp=0):
         self.lib = _lib.load()
         self.nelx, self.nely, self.nelz, self.nl = int(nelx), int(nely), int(nelz), int(nl)
         self.nel = self.nelx * self.nely * self.nelz
@@ -40,6 +40,7 @@ class Solver:
         self.params.print_level = print_level
         self.params.l0_kernel = l0_kernel
         self.params.l1_kernel = l1_kernel
+        self.params.nswp = nswp
         h = ctypes.c_void_p()
         check(self.lib.gputo_create(self.nelx, self.nely, self.nelz, self.nl, ctypes.byref(self.params),
                                     ctypes.byref(h)), "gputo_create")
@@ -127,6 +128,12 @@ class Solver:
         fd = np.zeros(max(nfd, 1))
         check(self.lib.gputo_level_matrices(self.h, level, ptr(A), ptr(ad), ptr(fd)), "gputo_level_matrices")
         return (None if A is None else A.reshape(nel, 24, 24)), ad, fd[:nfd]
+
+    def level_cx(self, level):
+        nx, ny, nz, _ = self.level_info(level)
+        cx = np.zeros(nx * ny * nz)
+        check(self.lib.gputo_level_cx(self.h, level, ptr(cx)), "gputo_level_cx")
+        return cx
 
     def level_apply(self, level, op, mode, u, f=None, fp32=False):
         """One level-`level` operator application (gputo_level_apply): mode 0
@@ -222,13 +229,15 @@ class Solver:
         return self.lib.gputo_device_bytes(self.h)
 
 
-def _handle(nelx, nely, nelz, nl, preset=_lib.PRESET_REF_C, print_level=0):
+def _handle(nelx, nely, nelz, nl, preset=_lib.PRESET_REF_C, print_level=0, nswp=0):
     # one persistent handle per mesh, as a MEX would keep it between calls
     key = (int(nelx), int(nely), int(nelz), int(nl), preset)
     s = _HANDLES.get(key)
     if s is None:
-        s = Solver(nelx, nely, nelz, nl, preset=preset, print_level=print_level)
+        s = Solver(nelx, nely, nelz, nl, preset=preset, print_level=print_level, nswp=nswp)
         _HANDLES[key] = s
+    elif s.params.nswp != nswp or s.params.print_level != print_level:
+        s.set_params(nswp=nswp, print_level=print_level)
     return s
 
 
@@ -274,7 +283,16 @@ def mgcg1(*args):
 
 def mgcg9(Ke0, F, U, CX, fixed, nfd, nelx, nely, nelz, nl, tol, maxit, nswp, printLev, nu, hx, hy, hz):
     """[cgres, cgiters] = mgcg9(...) with the argument list of
-    Homogenization/QNC/mgcg9.cu:124-182 (nu and hx..hz are parsed and unused
-    there too).  Solved by the mgcg1 engine (REF_C preset); the converged
-    displacements agree to the CG tolerance."""
-    return mgcg1(Ke0, F, U, CX, fixed, nfd, nelx, nely, nelz, nl, tol, maxit, printLev)
+    Homogenization/QNC/mgcg9.cu:124-182, solved by the mgcg9 engine
+    (GPUTO_PRESET_MGCG9 with its nswp; nu and hx..hz are parsed and unused
+    there too).  U (3rd argument) is overwritten in place."""
+    if not (isinstance(U, np.ndarray) and U.dtype == np.float64 and U.flags["C_CONTIGUOUS"]):
+        raise TypeError("U must be a contiguous float64 array (it is overwritten in place)")
+    nfd = int(_scalar(nfd))
+    fixed = f64(fixed).ravel()[:nfd]
+    s = _handle(_scalar(nelx), _scalar(nely), _scalar(nelz), _scalar(nl), preset=_lib.PRESET_MGCG9,
+                print_level=int(_scalar(printLev)), nswp=int(_scalar(nswp)))
+    _, relres, nit, rc = s.solve(Ke0, F, CX, fixed, _scalar(tol), int(_scalar(maxit)), U=U.reshape(-1))
+    if rc != 0:
+        return 0.0, float(rc)
+    return relres, float(nit)

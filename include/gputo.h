@@ -43,8 +43,17 @@ enum gputo_status {
  *  GPUTO_PRESET_MATLAB_CFG0: the MATLAB top3D125mgcg smoother (omega 0.6,
  *    one sweep down from zero, one sweep up; Cantilever/TOP3D125MGCG/
  *    top3D125mgcg.m:190-246) on element-Galerkin operators at every
- *    coarse level. */
-enum gputo_preset { GPUTO_PRESET_REF_C = 0, GPUTO_PRESET_MATLAB_CFG0 = 1 };
+ *    coarse level.
+ *  GPUTO_PRESET_MGCG9: the mgcg9 engine of the CrossWalk and QNC drivers
+ *    (Homogenization/QNC/mgcg9.cu): every level matrix-free with the unit
+ *    Ke and element scales CX_l = 2 * mean of the children (formCXc,
+ *    :753-789); the smoother's defect skips elements with CX < 1e-8
+ *    (defect0c, :939-996) while the CG matvec keeps them; pre-relaxation
+ *    U = 0.7 F/AD and `nswp` damped-Jacobi sweeps (omega 0.33) each way
+ *    (:416-635, Jacobc :929-937); a coarsest level below 20000 DOFs is
+ *    factored with its fixed rows and columns zeroed and 1e10 on their
+ *    diagonal (CorrCsAQc, :1005-1028), a larger one gets 10*nswp sweeps. */
+enum gputo_preset { GPUTO_PRESET_REF_C = 0, GPUTO_PRESET_MATLAB_CFG0 = 1, GPUTO_PRESET_MGCG9 = 2 };
 
 /* Arithmetic of the solve.
  *  GPUTO_PREC_F64: everything in FP64 (the reference's arithmetic).
@@ -65,6 +74,7 @@ typedef struct gputo_params {
     int l0_kernel;     /* level-0 operator of the FP32 V-cycle: 0 by precision, 1 FP32
                           node-gather, 2 FP32 MFMA, 3 split-BF16 MFMA (hi+lo),
                           4 split-BF16 MFMA (hi+mid+lo) */
+    int nswp;          /* GPUTO_PRESET_MGCG9: Jacobi sweeps each way (0: 2, the MEX default) */
     int l1_kernel;     /* level-1 operator in the V-cycle: 0 by preset (matrix-free
                           where level 1 is rediscretised from the child densities,
                           stored element matrices otherwise), 1 stored element
@@ -136,6 +146,9 @@ int gputo_vcycle_f64(gputo_handle *h, const double *Ke0, const double *CX, const
 int gputo_setup(gputo_handle *h, const double *Ke0, const double *CX, const double *fixed, int nfd);
 int gputo_level_info(const gputo_handle *h, int level, int *nelx, int *nely, int *nelz, int *nfd);
 int gputo_level_matrices(gputo_handle *h, int level, double *A, double *ad, double *fixed);
+/* Element scales of level `level` of a GPUTO_PRESET_MGCG9 hierarchy
+ * (formCXc, Homogenization/QNC/mgcg9.cu:753-789; level 0: the input CX). */
+int gputo_level_cx(gputo_handle *h, int level, double *cx);
 /* Grid transfers on level l (fine) <-> l+1 (coarse) of the current
  * hierarchy: restriction F_c = P^T r_f with F_c[fixed_c] = 0 (rstrFc,
  * mgcg1.cu:792-831) and prolongation u_f += P u_c (interpc, 1587-1641). */

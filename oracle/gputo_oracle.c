@@ -528,6 +528,40 @@ void or_cholsl(int n, const double *a, const double *p, const double *b, double 
 /* MGCG                                                                */
 /* ------------------------------------------------------------------ */
 
+/* formCXc (Homogenization/QNC/mgcg9.cu:753-789): CX_c = 2 * mean of the 8
+ * children (nelx.. are the coarse sizes). */
+void or_coarse_cx(const double *CXP, double *CX, int nelx, int nely, int nelz)
+{
+    int nelyP = 2 * nely, nelzP = 2 * nelz;
+    for (int i = 0; i < nelx; ++i)
+        for (int k = 0; k < nelz; ++k)
+            for (int j = 0; j < nely; ++j) {
+                double s = 0.0;
+                for (int iP = 0; iP < 2; ++iP)
+                    for (int kP = 0; kP < 2; ++kP)
+                        for (int jP = 0; jP < 2; ++jP)
+                            s += CXP[(long)nelzP * nelyP * (2 * i + iP) + (long)nelyP * (2 * k + kP) + 2 * j + jP];
+                CX[(long)nelz * nely * i + (long)nely * k + j] = 2.0 * (s / 8);
+            }
+}
+
+/* CorrCsAQc (Homogenization/QNC/mgcg9.cu:1005-1028): rows and columns of
+ * the fixed DOFs zeroed, 1e10 on their diagonal. */
+void or_zero_rowcol(double *AQ, long n, const double *FDofs, int nFD)
+{
+    for (int f = 0; f < nFD; ++f) {
+        long d = (long)FDofs[f] - 1;
+        for (long i = 0; i < n; ++i) {
+            AQ[d * n + i] = 0.0;
+            AQ[i * n + d] = 0.0;
+        }
+    }
+    for (int f = 0; f < nFD; ++f) {
+        long d = (long)FDofs[f] - 1;
+        AQ[d * n + d] = 1e10;
+    }
+}
+
 /* Smoother / hierarchy preset.  ref_c follows mgcg1.c + mgcg1.cu
  * (mgcg1.c:110,308,353,363-364; mgcg1.cu:188,1196,1205,631);
  * matlab_cfg0 follows top3D125mgcg.m:190-246 (omega 0.6, one sweep,
@@ -540,10 +574,23 @@ typedef struct {
     int zero_after_interp;
     int corr_mirrored;    /* CorrCsA table choice */
     double nu;            /* Poisson ratio of the level-1 rediscretisation */
+    int mg9;              /* mgcg9 engine: every level matrix-free with a coarse CX */
 } or_params;
+
+/* nswp of the mgcg9 preset (its MEX argument, mgcg9.cu:153) */
+static int s_mg9_nswp = 2;
+void or_set_nswp(int n) { s_mg9_nswp = n; }
 
 void or_params_preset(int preset, or_params *p)
 {
+    p->mg9 = 0;
+    if (preset == 2) { /* mgcg9 (Homogenization/QNC/mgcg9.cu): omega 0.33 (Jacobc :929-937),
+                          preRelc 0.7 (:920-927), nswp sweeps each way (set by the caller) */
+        p->npre = s_mg9_nswp; p->npost = s_mg9_nswp; p->omega = 0.33; p->prerelax = 0.7;
+        p->l1_galerkin = 0; p->zero_after_interp = 1; p->corr_mirrored = 0; p->mg9 = 1;
+        p->nu = 0.3;
+        return;
+    }
     if (preset == 1) { /* matlab_cfg0 */
         p->npre = 0; p->npost = 1; p->omega = 0.6; p->prerelax = 0.6;
         p->l1_galerkin = 1; p->zero_after_interp = 1; p->corr_mirrored = 0;
@@ -560,6 +607,9 @@ typedef struct {
     double *U, *F, *R, *AD, *A, *FD;
     int nFD;
     double *AQ, *pdiag; /* coarsest only */
+    double *CX, *CXd;   /* mgcg9: level element scales, and the same with
+                           scales below 1e-8 dropped (its defect operator) */
+    int direct;         /* coarsest level solved by Cholesky */
 } or_level;
 
 typedef struct {
@@ -580,7 +630,9 @@ static double dot(const double *a, const double *b, long n)
 static void lev_defect(or_hier *H, int l, const double *F, const double *U, double *R)
 {
     or_level *v = &H->L[l];
-    if (l == 0)
+    if (H->prm.mg9)
+        or_defect0(H->Ke, F, U, R, v->CXd, v->FD, v->nFD, v->nelx, v->nely, v->nelz);
+    else if (l == 0)
         or_defect0(H->Ke, F, U, R, H->CX, v->FD, v->nFD, v->nelx, v->nely, v->nelz);
     else
         or_defect_el(v->A, F, U, R, v->FD, v->nFD, v->nelx, v->nely, v->nelz);
@@ -590,6 +642,59 @@ static void jacobi(double *U, const double *R, const double *AD, double om, long
 {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) U[i] = (1.0 - om) * U[i] + om * (R[i] + AD[i] * U[i]) / AD[i];
+}
+
+/* mgcg9 hierarchy (initVcycle, Homogenization/QNC/mgcg9.cu:342-414): every
+ * level keeps element scales, CX_l = 2 * mean of the 8 children (formCXc,
+ * :753-789, the factor 2 standing for Ke(2h) = 2 Ke(h)), the unit Ke and
+ * the diagonal from them (formADc, :1030-1070); the defect skips elements
+ * with CX < 1e-8 (defect0c, :939-996) while the CG matvec does not
+ * (multAV0c, :791-848).  The coarsest level (not level 0) is factored when
+ * it has fewer than 20000 DOFs (maxDofs0, :74): CX*Ke assembled
+ * (assembAQc), the rows and columns of its fixed DOFs zeroed and 1e10 put
+ * on their diagonal (CorrCsAQc, :1005-1028). */
+static int hier_init_mg9(or_hier *H, const double *Ke, const double *CX, const double *FDofs, int nFD,
+                         int nelx, int nely, int nelz, int nl)
+{
+    for (int l = 0; l < nl; ++l) {
+        or_level *v = &H->L[l];
+        v->nelx = nelx >> l;
+        v->nely = nely >> l;
+        v->nelz = nelz >> l;
+        v->nel = (long)v->nelx * v->nely * v->nelz;
+        v->ndof = 3L * (v->nelx + 1) * (v->nely + 1) * (v->nelz + 1);
+        v->R = (double *)calloc(v->ndof, sizeof(double));
+        v->AD = (double *)calloc(v->ndof, sizeof(double));
+        v->CX = (double *)malloc(v->nel * sizeof(double));
+        v->CXd = (double *)malloc(v->nel * sizeof(double));
+        if (l == 0) {
+            memcpy(v->CX, CX, v->nel * sizeof(double));
+            v->FD = (double *)malloc((nFD + 1) * sizeof(double));
+            memcpy(v->FD, FDofs, nFD * sizeof(double));
+            v->nFD = nFD;
+        } else {
+            or_level *p = &H->L[l - 1];
+            v->U = (double *)calloc(v->ndof, sizeof(double));
+            v->F = (double *)calloc(v->ndof, sizeof(double));
+            v->FD = (double *)malloc((p->nFD + 1) * sizeof(double));
+            v->nFD = or_rstrfd(p->FD, v->FD, v->nelx, v->nely, v->nelz, p->nFD);
+            or_coarse_cx(p->CX, v->CX, v->nelx, v->nely, v->nelz);
+        }
+        for (long e = 0; e < v->nel; ++e) v->CXd[e] = v->CX[e] < 1e-8 ? 0.0 : v->CX[e];
+        or_prepad(Ke, NULL, v->CX, v->AD, v->nelx, v->nely, v->nelz);
+        if (l == nl - 1 && l > 0 && v->ndof < 20000) {
+            v->A = (double *)malloc(576 * v->nel * sizeof(double));
+            for (long e = 0; e < v->nel; ++e)
+                for (int m = 0; m < 576; ++m) v->A[576 * e + m] = v->CX[e] * Ke[m];
+            v->AQ = (double *)malloc(v->ndof * v->ndof * sizeof(double));
+            v->pdiag = (double *)malloc(v->ndof * sizeof(double));
+            or_assemble(v->A, v->AQ, v->nelx, v->nely, v->nelz);
+            or_zero_rowcol(v->AQ, v->ndof, v->FD, v->nFD);
+            if (or_choldc((int)v->ndof, v->AQ, v->pdiag)) return -2;
+            v->direct = 1;
+        }
+    }
+    return 0;
 }
 
 /* initVcycle (mgcg1.c:209-281; mgcg1.cu:326-466).  Returns 0, or -2 when
@@ -603,6 +708,7 @@ static int hier_init(or_hier *H, const double *Ke, const double *CX, const doubl
     H->Ke = Ke;
     H->CX = CX;
     H->prm = *prm;
+    if (prm->mg9) return hier_init_mg9(H, Ke, CX, FDofs, nFD, nelx, nely, nelz, nl);
     for (int l = 0; l < nl; ++l) {
         or_level *v = &H->L[l];
         v->nelx = nelx >> l;
@@ -657,6 +763,7 @@ static void hier_free(or_hier *H)
         free(v->R); free(v->AD); free(v->FD);
         if (l > 0) { free(v->U); free(v->F); }
         free(v->A); free(v->AQ); free(v->pdiag);
+        free(v->CX); free(v->CXd);
     }
 }
 
@@ -670,7 +777,14 @@ static void vcycle(or_hier *H, const double *F0, double *U0)
         const double *F = l == 0 ? F0 : v->F;
         double *U = l == 0 ? U0 : v->U;
         if (l > 0) or_restrict(H->L[l - 1].R, v->F, v->FD, v->nFD, v->nelx, v->nely, v->nelz);
-        if (l < nl - 1) {
+        if (p->mg9 && l == nl - 1 && l > 0 && !v->direct) {
+            /* coarsest level too large to factor: 10 nswp sweeps from the
+               level's previous iterate (mgcg9.cu:553-563) */
+            for (int it = 0; it < 10 * p->npre; ++it) {
+                lev_defect(H, l, F, U, v->R);
+                jacobi(U, v->R, v->AD, p->omega, v->ndof);
+            }
+        } else if (l < nl - 1 || (p->mg9 && l == 0)) {
 #pragma omp parallel for schedule(static)
             for (long n = 0; n < v->ndof; ++n) U[n] = p->prerelax * F[n] / v->AD[n];
             for (int it = 0; it < p->npre; ++it) {
@@ -680,7 +794,7 @@ static void vcycle(or_hier *H, const double *F0, double *U0)
             lev_defect(H, l, F, U, v->R);
         } else {
             or_cholsl((int)v->ndof, v->AQ, v->pdiag, F, U);
-            zero_bc(U, v->FD, v->nFD);
+            if (!p->mg9) zero_bc(U, v->FD, v->nFD);
         }
     }
     for (int l = nl - 2; l >= 0; --l) {
@@ -785,6 +899,19 @@ int or_level_data(int preset, const double *Ke, const double *CX, const double *
     int n = v->nFD;
     hier_free(&H);
     return n;
+}
+
+/* Element scales of level l of a fresh mgcg9 hierarchy (test entry). */
+int or_level_cx(const double *Ke, const double *CX, const double *FDofs, int nFD, int nelx, int nely, int nelz,
+                int nl, int l, double *CXl)
+{
+    or_params prm;
+    or_params_preset(2, &prm);
+    or_hier H;
+    int rc = hier_init(&H, Ke, CX, FDofs, nFD, nelx, nely, nelz, nl, &prm);
+    if (!rc) memcpy(CXl, H.L[l].CX, H.L[l].nel * sizeof(double));
+    hier_free(&H);
+    return rc;
 }
 
 /* ------------------------------------------------------------------ */

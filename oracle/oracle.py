@@ -14,6 +14,7 @@ _LIB = None
 
 PRESET_REF_C = 0
 PRESET_MATLAB_CFG0 = 1
+PRESET_MGCG9 = 2
 
 _d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -58,6 +59,9 @@ def lib():
             "or_design_iter": (_c_int, [_c_int, _d, _d, _d, _d, _c_int, _c_int, _c_int, _c_int, _c_int,
                                         _c_dbl, _c_int, _d, _d, _d, _c_dbl, _c_dbl, _c_dbl, _c_int,
                                         ctypes.POINTER(_c_dbl), _c_dbl, _c_dbl, _c_dbl, _c_dbl, _d]),
+            "or_set_nswp": (None, [_c_int]),
+            "or_level_cx": (_c_int, [_d, _d, _d, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _d]),
+            "or_coarse_cx": (None, [_d, _d, _c_int, _c_int, _c_int]),
             "or_num_threads": (_c_int, []),
             "or_set_num_threads": (None, [_c_int]),
         }
@@ -212,9 +216,11 @@ def cholesky_solve(AQ, b):
     return x, fail
 
 
-def mgcg(Ke, F, CX, fixed, nelx, nely, nelz, nl, tol, maxit, preset=PRESET_REF_C, flip_y=True):
-    """mgcg1 semantics: returns (U, relres, nit, rc); U is in the MATLAB
-    frame (y negated) when flip_y."""
+def mgcg(Ke, F, CX, fixed, nelx, nely, nelz, nl, tol, maxit, preset=PRESET_REF_C, flip_y=True, nswp=2):
+    """mgcg1 semantics (mgcg9 with preset PRESET_MGCG9 and its nswp):
+    returns (U, relres, nit, rc); U is in the MATLAB frame (y negated) when
+    flip_y."""
+    lib().or_set_nswp(nswp)
     U = np.zeros(ndof_of(nelx, nely, nelz))
     res = ctypes.c_double(0.0)
     nit = ctypes.c_int(0)
@@ -234,6 +240,17 @@ def vcycle_once(Ke, CX, fixed, nelx, nely, nelz, nl, r, preset=PRESET_REF_C):
     if rc:
         raise RuntimeError("oracle hierarchy setup failed: %d" % rc)
     return z
+
+
+def level_cx(Ke, CX, fixed, nelx, nely, nelz, nl, level):
+    """Element scales of level `level` of the mgcg9 hierarchy."""
+    out = np.zeros((nelx >> level) * (nely >> level) * (nelz >> level))
+    fixed = np.ascontiguousarray(fixed, dtype=np.float64)
+    rc = lib().or_level_cx(np.ascontiguousarray(Ke, dtype=np.float64).ravel(), np.ascontiguousarray(CX, float),
+                           fixed if len(fixed) else np.zeros(1), len(fixed), nelx, nely, nelz, nl, level, out)
+    if rc:
+        raise RuntimeError("oracle hierarchy setup failed: %d" % rc)
+    return out
 
 
 def level_data(Ke, CX, fixed, nelx, nely, nelz, nl, level, preset=PRESET_REF_C):
