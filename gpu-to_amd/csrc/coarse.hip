@@ -430,20 +430,30 @@ void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ)
 // (lower triangle used), NB = 32: factor the diagonal tile in LDS, solve
 // the panel below it, update the trailing lower triangle.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_potrf_diag(double *A, int N, int k0, int *flag)
+__device__ __forceinline__ double readlane_f64(double v, int lane)
 {
-    // one wave: lane r < NB holds row r of the diagonal tile in registers;
-    // column jj's pivot and multipliers are exchanged by lane shuffles, so
-    // the 32 elimination steps need no barrier
-    const int lane = threadIdx.x;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// One full wave factors the 32x32 diagonal tile at T (row stride ld, lower
+// triangle): lane r < NB holds row r in registers; column jj's pivot and
+// multipliers are read across lanes into scalar registers (v_readlane), so
+// the 32 elimination steps need neither LDS nor a barrier.  Returns true
+// on a non-positive pivot.
+__device__ __forceinline__ bool potrf_tile(double *T, long ld)
+{
+    const int lane = threadIdx.x & 63;
     const int r = lane < NB ? lane : NB - 1;  // lanes >= NB shadow row NB-1, results dropped
     double L[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) L[c] = c <= r ? A[(long)(k0 + r) * N + k0 + c] : 0.0;
+    for (int c = 0; c < NB; ++c) L[c] = c <= r ? T[(long)r * ld + c] : 0.0;
     bool bad = false;
 #pragma unroll
     for (int jj = 0; jj < NB; ++jj) {
-        double d = __shfl(L[jj], jj, 64);  // updated A[jj][jj]
+        double d = readlane_f64(L[jj], jj);  // updated A[jj][jj]
         if (!(d > 0.0)) {  // not positive definite (or NaN)
             bad = true;
             d = 1.0;
@@ -453,16 +463,22 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double *A, int N, int k0, int
         L[jj] = lj;
 #pragma unroll
         for (int c = jj + 1; c < NB; ++c) {
-            const double lc = __shfl(lj, c, 64);  // L[c][jj]
+            const double lc = readlane_f64(lj, c);  // L[c][jj]
             if (c <= r) L[c] -= lj * lc;
         }
     }
     if (lane < NB) {
 #pragma unroll
         for (int c = 0; c < NB; ++c)
-            if (c <= r) A[(long)(k0 + r) * N + k0 + c] = L[c];
+            if (c <= r) T[(long)r * ld + c] = L[c];
     }
-    if (bad && lane == 0) flag[0] = 1;
+    return bad;
+}
+
+__global__ __launch_bounds__(64) void k_potrf_diag(double *A, int N, int k0, int *flag)
+{
+    const bool bad = potrf_tile(A + (long)k0 * N + k0, N);
+    if (bad && threadIdx.x == 0) flag[0] = 1;
 }
 
 __global__ __launch_bounds__(256) void k_trsm_panel(double *A, int N, int k0, int rows)
@@ -545,55 +561,6 @@ __global__ __launch_bounds__(64) void k_trtri_diag(const double *A, double *Linv
     for (int r = 0; r < NB; ++r) Linv[(long)(k0 + r) * N + k0 + col] = x[r];
 }
 
-// Block row ib of Linv: Linv[ib][jb] = -inv(L[ib][ib]) sum_{kb=jb}^{ib-1}
-// L[ib][kb] Linv[kb][jb], one block per jb < ib; the sum runs over the
-// factor's band only.
-__global__ __launch_bounds__(256) void k_linv_row(const double *A, double *Linv, int N, int ib, int nbt)
-{
-    __shared__ double Ta[NB][NB + 1], Tb[NB][NB + 1];
-    const int jb = blockIdx.x;
-    double acc[4] = {0, 0, 0, 0};
-    // tiles L[ib][kb] with kb < ib - nbt lie outside the band (exact zeros)
-    for (int kb = max(jb, ib - nbt); kb < ib; ++kb) {
-        __syncthreads();
-        for (int q = threadIdx.x; q < NB * NB; q += 256) {
-            const int r = q / NB, c = q % NB;
-            Ta[r][c] = A[(long)(ib * NB + r) * N + kb * NB + c];
-            Tb[r][c] = Linv[(long)(kb * NB + r) * N + jb * NB + c];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int q = threadIdx.x + 256 * u;
-            const int r = q / NB, c = q % NB;
-            double s = 0.0;
-#pragma unroll
-            for (int m = 0; m < NB; ++m) s += Ta[r][m] * Tb[m][c];
-            acc[u] += s;
-        }
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < NB * NB; q += 256) {
-        const int r = q / NB, c = q % NB;
-        Ta[r][c] = Linv[(long)(ib * NB + r) * N + ib * NB + c];  // inv(L[ib][ib])
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int q = threadIdx.x + 256 * u;
-        Tb[q / NB][q % NB] = acc[u];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int q = threadIdx.x + 256 * u;
-        const int r = q / NB, c = q % NB;
-        double s = 0.0;
-#pragma unroll
-        for (int m = 0; m < NB; ++m) s += Ta[r][m] * Tb[m][c];
-        Linv[(long)(ib * NB + r) * N + jb * NB + c] = -s;
-    }
-}
-
 __global__ void k_transpose(const double *__restrict__ In, double *__restrict__ Out, int N)
 {
     __shared__ double T[NB][NB + 1];
@@ -601,6 +568,151 @@ __global__ void k_transpose(const double *__restrict__ In, double *__restrict__ 
     for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) T[q / NB][q % NB] = In[(long)(bi + q / NB) * N + bj + q % NB];
     __syncthreads();
     for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) Out[(long)(bj + q / NB) * N + bi + q % NB] = T[q % NB][q / NB];
+}
+
+// ---------------------------------------------------------------------------
+// The same factorisation in one workgroup when the band spans at most
+// kBandRowsMax rows below the diagonal tile (the coarsest grids of the
+// usual hierarchies): every step stages the diagonal tile and the panel
+// below it in LDS, one wave factors the tile, one thread per panel row
+// solves the panel, and the trailing band is updated from LDS in 4x4
+// register blocks.  One launch instead of three per block column, with the
+// same operations in the same order as the tile-parallel kernels.
+// ---------------------------------------------------------------------------
+constexpr int kBandRowsMax = 256;
+constexpr int kCholThreads = 512;  // 2 waves per SIMD: 256 VGPRs for the panel row and 4x4 blocks
+
+__global__ __launch_bounds__(kCholThreads) void k_chol_band(double *A, int N, int band_rows, int *flag)
+{
+    __shared__ double T[NB][NB + 1];
+    __shared__ double P[kBandRowsMax][NB + 1];
+    const int tid = threadIdx.x;
+    const int nb = N / NB;
+    bool bad = false;
+    for (int kb = 0; kb < nb; ++kb) {
+        const int k0 = kb * NB;
+        const int rows = min(N - k0 - NB, band_rows);  // multiple of NB
+        for (int q = tid; q < NB * NB; q += kCholThreads) T[q / NB][q % NB] = A[(long)(k0 + q / NB) * N + k0 + q % NB];
+        for (int q = tid; q < rows * NB; q += kCholThreads) P[q / NB][q % NB] = A[(long)(k0 + NB + q / NB) * N + k0 + q % NB];
+        __syncthreads();
+        if (tid < 64) {
+            bad |= potrf_tile(&T[0][0], NB + 1);
+            if (tid < NB)
+                for (int c = 0; c <= tid; ++c) A[(long)(k0 + tid) * N + k0 + c] = T[tid][c];
+        }
+        __syncthreads();
+        if (tid < rows) {
+            double x[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) x[c] = P[tid][c];
+            // column-oriented substitution: x[m] receives its updates in the
+            // same order (and roundings) as the row-oriented panel kernel
+#pragma unroll
+            for (int jj = 0; jj < NB; ++jj) {
+                x[jj] = x[jj] / T[jj][jj];
+#pragma unroll
+                for (int m = jj + 1; m < NB; ++m) x[m] -= x[jj] * T[m][jj];
+            }
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                P[tid][c] = x[c];
+                A[(long)(k0 + NB + tid) * N + k0 + c] = x[c];
+            }
+        }
+        __syncthreads();
+        const int nt = rows / NB;
+        const int nsub = nt * (nt + 1) / 2 * 64;  // 64 4x4 blocks per tile
+        for (int w = tid; w < nsub; w += kCholThreads) {
+            int ti, tj;
+            tri_index(w >> 6, ti, tj);
+            const int r0 = ti * NB + ((w >> 3) & 7) * 4, c0 = tj * NB + (w & 7) * 4;
+            double acc[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+#pragma unroll 8
+            for (int m = 0; m < NB; ++m) {
+                double a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = P[r0 + i][m];
+                    b[i] = P[c0 + i][m];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) A[(long)(k0 + NB + r0 + i) * N + k0 + NB + c0 + j] -= acc[i][j];
+        }
+        __syncthreads();
+    }
+    if (bad && tid == 0) flag[0] = 1;
+}
+
+// Column block jb of L^-1 in one workgroup, marching down the block rows:
+// X[ib][jb] = -inv(L[ib][ib]) sum_{kb=jb}^{ib-1} L[ib][kb] X[kb][jb] with
+// the sum over the factor's band and X[jb][jb] = inv(L[jb][jb]) written by
+// k_trtri_diag.  The column blocks are independent: one launch of nb
+// workgroups replaces nb - 1 launches of the row-block kernel, with the same
+// operations in the same order.  256 threads, 2x2 outputs each.
+__global__ __launch_bounds__(256) void k_linv_cols(const double *A, double *Linv, int N, int nbt)
+{
+    __shared__ double Ta[NB][NB + 1], Tb[NB][NB + 1];
+    const int jb = blockIdx.x, nb = N / NB, tid = threadIdx.x;
+    const int r0 = (tid >> 4) * 2, c0 = (tid & 15) * 2;
+    for (int ib = jb + 1; ib < nb; ++ib) {
+        double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+        for (int kb = max(jb, ib - nbt); kb < ib; ++kb) {
+            __syncthreads();
+            for (int q = tid; q < NB * NB; q += 256) {
+                const int r = q / NB, c = q % NB;
+                Ta[r][c] = A[(long)(ib * NB + r) * N + kb * NB + c];
+                Tb[r][c] = Linv[(long)(kb * NB + r) * N + jb * NB + c];
+            }
+            __syncthreads();
+            double sm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll 8
+            for (int m = 0; m < NB; ++m) {
+                const double a0 = Ta[r0][m], a1 = Ta[r0 + 1][m], b0 = Tb[m][c0], b1 = Tb[m][c0 + 1];
+                sm[0][0] += a0 * b0;
+                sm[0][1] += a0 * b1;
+                sm[1][0] += a1 * b0;
+                sm[1][1] += a1 * b1;
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] += sm[i][j];
+        }
+        __syncthreads();
+        for (int q = tid; q < NB * NB; q += 256) {
+            const int r = q / NB, c = q % NB;
+            Ta[r][c] = Linv[(long)(ib * NB + r) * N + ib * NB + c];  // inv(L[ib][ib])
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) Tb[r0 + i][c0 + j] = acc[i][j];
+        __syncthreads();
+        double sm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll 8
+        for (int m = 0; m < NB; ++m) {
+            const double a0 = Ta[r0][m], a1 = Ta[r0 + 1][m], b0 = Tb[m][c0], b1 = Tb[m][c0 + 1];
+            sm[0][0] += a0 * b0;
+            sm[0][1] += a0 * b1;
+            sm[1][0] += a1 * b0;
+            sm[1][1] += a1 * b1;
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) Linv[(long)(ib * NB + r0 + i) * N + jb * NB + c0 + j] = -sm[i][j];
+    }
 }
 
 int coarse_factor(hipStream_t s, Coarse &C)
@@ -612,20 +724,24 @@ int coarse_factor(hipStream_t s, Coarse &C)
     const int bw = std::max(1, std::min(C.bw, N));
     const int band_rows = (bw + NB - 1) / NB * NB;
     hipMemsetAsync(C.flag, 0, sizeof(int), s);
-    for (int kb = 0; kb < nb; ++kb) {
-        const int k0 = kb * NB;
-        k_potrf_diag<<<1, 64, 0, s>>>(C.AQ, N, k0, C.flag);
-        const int rows = std::min(N - k0 - NB, band_rows);
-        if (rows > 0) {
-            k_trsm_panel<<<(rows + 255) / 256, 256, 0, s>>>(C.AQ, N, k0, rows);
-            const int nt = rows / NB;
-            k_syrk_update<<<nt * (nt + 1) / 2, 256, 0, s>>>(C.AQ, N, k0);
+    if (band_rows <= kBandRowsMax) {
+        k_chol_band<<<1, kCholThreads, 0, s>>>(C.AQ, N, band_rows, C.flag);
+    } else {
+        for (int kb = 0; kb < nb; ++kb) {
+            const int k0 = kb * NB;
+            k_potrf_diag<<<1, 64, 0, s>>>(C.AQ, N, k0, C.flag);
+            const int rows = std::min(N - k0 - NB, band_rows);
+            if (rows > 0) {
+                k_trsm_panel<<<(rows + 255) / 256, 256, 0, s>>>(C.AQ, N, k0, rows);
+                const int nt = rows / NB;
+                k_syrk_update<<<nt * (nt + 1) / 2, 256, 0, s>>>(C.AQ, N, k0);
+            }
         }
     }
     launch_fill(s, C.Linv, (long)N * N, 0.0);
     k_trtri_diag<<<nb, 64, 0, s>>>(C.AQ, C.Linv, N);
     const int nbt = (bw + 2 * NB - 2) / NB;
-    for (int ib = 1; ib < nb; ++ib) k_linv_row<<<ib, 256, 0, s>>>(C.AQ, C.Linv, N, ib, nbt);
+    if (nb > 1) k_linv_cols<<<nb - 1, 256, 0, s>>>(C.AQ, C.Linv, N, nbt);
     k_transpose<<<dim3(nb, nb), 256, 0, s>>>(C.Linv, C.LinvT, N);
     int flag = 0;
     hipMemcpyAsync(&flag, C.flag, sizeof(int), hipMemcpyDeviceToHost, s);

@@ -1,21 +1,23 @@
-// Level-0 operator of the mixed-precision inner solve on the matrix cores:
-// the shared 24x24 element matrix runs against 24 x 16 tiles of element
-// vectors (16 elements per MFMA column tile, Ke rows padded 24 -> 32) with
-// FP32 accumulation.  Three flavours of the same kernel:
-//   MF_F32   v_mfma_f32_16x16x4_f32, exact FP32 products (K = 24 in 6 steps)
-//   MF_BF16  v_mfma_f32_16x16x32_bf16 on split operands x = hi + lo,
-//            products hi*hi + hi*lo + lo*hi (K padded 24 -> 32)
+// Level-0 operator of the FP32 V-cycle on the matrix cores: the shared
+// 24x24 element matrix runs against 24 x 16 tiles of element vectors (16
+// elements per MFMA column tile, Ke rows padded 24 -> 32) with FP32
+// accumulation.  Flavours of the same kernel:
+//   MF_F32    v_mfma_f32_16x16x4_f32, exact FP32 products (K = 24 in 6 steps)
+//   MF_BF16X1 v_mfma_f32_16x16x32_bf16 on bf16(Ke) and bf16(u), one product
+//   MF_BF16   split operands x = hi + lo, products hi*hi + hi*lo + lo*hi
 //   MF_BF16X3 three-way split x = hi + mid + lo (exact for FP32 inputs) and
-//            the six products whose order is >= 2^-24
+//             the six products whose order is >= 2^-24
 //
-// One block owns a TJ x TK x TI brick of nodes.  It stages the nodal values
-// of the brick plus a one-node halo and builds a colour-sorted table of the
-// (TJ+1)(TK+1)(TI+1) elements of its box (origin offset, E) in LDS.  Each
-// wave multiplies 16 elements of one colour at a time, scales each column by
-// E_e and adds the 24 rows into LDS node accumulators with ds_add_f32.
-// Elements of one colour share no node, so a colour pass never adds twice to
-// one address, and every node receives its 8 element contributions in the
-// fixed colour order: no global atomics, bitwise deterministic.
+// One block owns a TJ x TK x TI brick of nodes (bricks dealt in XCD order).
+// It stages the nodal values of the brick plus a one-node halo and the
+// element scales of its (TJ+1)(TK+1)(TI+1) element box in LDS, one j-row per
+// wave.  Each wave multiplies 16 elements of one colour at a time (element
+// coordinates from the colour index by reciprocal multiplication), scales
+// each column by E_e and adds the 24 rows into LDS node accumulators with
+// ds_add_f32.  Elements of one colour share no node, so a colour pass never
+// adds twice to one address, and every node receives its 8 element
+// contributions in the fixed colour order: no global atomics, bitwise
+// deterministic.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -27,7 +29,7 @@ namespace gputo {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-enum MfFlavor { MF_F32 = 0, MF_BF16 = 1, MF_BF16X3 = 2 };
+enum MfFlavor { MF_F32 = 0, MF_BF16 = 1, MF_BF16X3 = 2, MF_BF16X1 = 3 };
 
 constexpr int kMfHalo = 2048;  // halo nodes per block
 constexpr int kMfElem = 1280;  // element box per block
@@ -41,16 +43,6 @@ __device__ constexpr int mIE[8] = {0, 1, 1, 0, 0, 1, 1, 0};
 __device__ constexpr int mJE[8] = {0, 0, 1, 1, 0, 0, 1, 1};
 __device__ constexpr int mKE[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
-__device__ __forceinline__ double mf_block_sum(double v)
-{
-    __shared__ double ws[4];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-    __syncthreads();
-    return ws[0] + ws[1] + ws[2] + ws[3];
-}
 
 // halo-relative offset (in floats) of element DOF k for an element whose
 // origin node has halo index 0
@@ -74,13 +66,27 @@ __device__ __forceinline__ void split3(float v, __bf16 &h, __bf16 &m, __bf16 &l)
     l = (__bf16)(r - (float)m);
 }
 
+// q / d for 0 <= q < 2^22 with inv = 1/d (float quotient corrected both ways)
+__device__ __forceinline__ int mf_udiv(int q, int d, float inv)
+{
+    int r = (int)((float)q * inv);
+    r -= (r * d > q) ? 1 : 0;
+    r += ((r + 1) * d <= q) ? 1 : 0;
+    return r;
+}
+
 template <int FL, int MODE>
 __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
 {
     if (a.ctl && a.ctl[0]) return;
     extern __shared__ float smem[];
     const Grid g = a.g;
-    int b = blockIdx.x;
+    const long tb = xcd_tile(blockIdx.x, t.blocks());
+    if (tb >= t.blocks()) {  // padding block of the XCD order
+        if (a.partials && threadIdx.x == 0) a.partials[blockIdx.x] = 0.0;
+        return;
+    }
+    int b = (int)tb;
     const int bj = b % t.ntj;
     b /= t.ntj;
     const int bk = b % t.ntk;
@@ -88,54 +94,46 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
     const int j0 = bj * t.TJ, k0 = bk * t.TK, i0 = bi * t.TI;
     const int HJ = t.TJ + 2, HK = t.TK + 2, HI = t.TI + 2;
     const int EJ = t.TJ + 1, EK = t.TK + 1, EI = t.TI + 1;
-    const int NH = HJ * HK * HI, NE = EJ * EK * EI;
-    float *su = smem;            // [3*NH] nodal values, AoS
-    float *acc = su + 3 * NH;    // [3*NH] node accumulators
-    float *cE = acc + 3 * NH;    // [NE]   element scale, colour order
-    int *ch = (int *)(cE + NE);  // [NE]   3 * halo index of element origin, colour order
+    const int NH = HJ * HK * HI;
+    float *su = smem;          // [3*NH] nodal values, AoS
+    float *acc = su + 3 * NH;  // [3*NH] node accumulators
+    float *cE = acc + 3 * NH;  // [EI*EK*EJ] element scales
 
-    const int rowlen = 3 * HJ;
-    for (int q = threadIdx.x; q < rowlen * HK * HI; q += 256) {
-        const int row = q / rowlen;
-        const int col = q - row * rowlen;
-        const int hk = row % HK, hi = row / HK;
-        const int hj = col / 3, c = col - 3 * hj;
-        const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
-        float v = 0.0f;
-        if (gi >= 0 && gi < g.nnx && gj >= 0 && gj < g.nny && gk >= 0 && gk < g.nnz) {
-            const long dof = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + gj) + c;
-            if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
-                v = (float)a.prerelax * a.F[dof] / a.AD[dof];
-            else
-                v = a.U[dof];
+    // stage rows of the halo brick and of the element box, one j-row per
+    // wave (coalesced, no per-lane integer division)
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int row = wave; row < HK * HI; row += 4) {
+        const int hi = row / HK, hk = row - hi * HK;
+        const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
+        const bool rowok = gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
+        const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
+        for (int col = lane; col < 3 * HJ; col += 64) {
+            const int gj = j0 - 1 + col / 3;
+            float v = 0.0f;
+            if (rowok && gj >= 0 && gj < g.nny) {
+                const long dof = base + col;
+                if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
+                    v = (float)a.prerelax * a.F[dof] / a.AD[dof];
+                else
+                    v = a.U[dof];
+            }
+            su[3 * row * HJ + col] = v;
+            acc[3 * row * HJ + col] = 0.0f;
         }
-        const int s = 3 * ((hi * HK + hk) * HJ + hj) + c;
-        su[s] = v;
-        acc[s] = 0.0f;
     }
-    // colour-sorted element table: colour c = (ej&1) + 2(ek&1) + 4(ei&1)
-    for (int q = threadIdx.x; q < NE; q += 256) {
-        const int ej = q % EJ;
-        const int r = q / EJ;
-        const int ek = r % EK, ei = r / EK;
-        const int pj = ej & 1, pk = ek & 1, pi = ei & 1;
-        const int c = pj + 2 * pk + 4 * pi;
-        int off = 0;
-        for (int cc = 0; cc < c; ++cc) {
-            const int qj = cc & 1, qk = (cc >> 1) & 1, qi = (cc >> 2) & 1;
-            off += ((EJ - qj + 1) / 2) * ((EK - qk + 1) / 2) * ((EI - qi + 1) / 2);
+    for (int row = wave; row < EK * EI; row += 4) {
+        const int ei = row / EK, ek = row - ei * EK;
+        const int gi = i0 - 1 + ei, gk = k0 - 1 + ek;
+        const bool rowok = gi >= 0 && gi < g.nx && gk >= 0 && gk < g.nz;
+        for (int ej = lane; ej < EJ; ej += 64) {
+            const int gj = j0 - 1 + ej;
+            float E = 0.0f;
+            if (rowok && gj >= 0 && gj < g.ny) E = a.CX[(long)g.ny * g.nz * gi + (long)g.ny * gk + gj];
+            cE[row * EJ + ej] = E;
         }
-        const int nj = (EJ - pj + 1) / 2, nk = (EK - pk + 1) / 2;
-        const int pos = off + ((ei >> 1) * nk + (ek >> 1)) * nj + (ej >> 1);
-        const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = k0 - 1 + ek;
-        float E = 0.0f;
-        if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz)
-            E = a.CX[(long)g.ny * g.nz * gi + (long)g.ny * gk + gj];
-        cE[pos] = E;
-        ch[pos] = 3 * ((ei * HK + ek) * HJ + ej);
     }
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lg = lane >> 4;
     // A fragments (Ke rows 16*tile + lr) and per-lane DOF offsets
     float Af[2][6];
@@ -183,16 +181,23 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
         }
     __syncthreads();
 
-    int off = 0;
+    // colour c = (ej&1) + 2(ek&1) + 4(ei&1): elements of one colour share no
+    // node, so a pass adds at most once to any accumulator
     for (int c = 0; c < 8; ++c) {
         const int pj = c & 1, pk = (c >> 1) & 1, pi = (c >> 2) & 1;
-        const int cnt = ((EJ - pj + 1) / 2) * ((EK - pk + 1) / 2) * ((EI - pi + 1) / 2);
+        const int nj = (EJ - pj + 1) / 2, nk = (EK - pk + 1) / 2, ni = (EI - pi + 1) / 2;
+        const int cnt = nj * nk * ni;
+        const float inv_nj = 1.0f / (float)nj, inv_nk = 1.0f / (float)nk;
         const int nb = (cnt + 15) / 16;
         for (int bb = wave; bb < nb; bb += 4) {
-            const int idx = 16 * bb + lr;
-            const bool valid = idx < cnt;
-            const int base = valid ? ch[off + idx] : 0;
-            const float E = valid ? cE[off + idx] : 0.0f;
+            const int q = 16 * bb + lr;
+            const bool valid = q < cnt;
+            const int qq = valid ? q : 0;
+            const int r1 = mf_udiv(qq, nj, inv_nj);
+            const int r2 = mf_udiv(r1, nk, inv_nk);
+            const int ej = pj + 2 * (qq - r1 * nj), ek = pk + 2 * (r1 - r2 * nk), ei = pi + 2 * r2;
+            const int base = 3 * ((ei * HK + ek) * HJ + ej);
+            const float E = valid ? cE[(ei * EK + ek) * EJ + ej] : 0.0f;
             f32x4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
             if (FL == MF_F32) {
 #pragma unroll
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
                         d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[tt], Bl, d, 0, 0, 0);
                         d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[tt], Bh, d, 0, 0, 0);
                         d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[tt], Bm, d, 0, 0, 0);
-                    } else {
+                    } else if (FL == MF_BF16) {
                         d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[tt], Bh, d, 0, 0, 0);
                         d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[tt], Bl, d, 0, 0, 0);
                     }
@@ -245,17 +250,18 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
                 }
             }
         }
-        off += cnt;
         __syncthreads();
     }
 
     double part = 0.0;
     const float om = (float)a.omega;
     const int nown = t.TJ * t.TK * t.TI;
+    const float inv_tj = 1.0f / (float)t.TJ, inv_tk = 1.0f / (float)t.TK;
     for (int own = threadIdx.x; own < nown; own += 256) {
-        const int tj = own % t.TJ;
-        const int r = own / t.TJ;
-        const int tk = r % t.TK, ti = r / t.TK;
+        const int r = mf_udiv(own, t.TJ, inv_tj);
+        const int tj = own - r * t.TJ;
+        const int ti = mf_udiv(r, t.TK, inv_tk);
+        const int tk = r - ti * t.TK;
         const int gi = i0 + ti, gj = j0 + tj, gk = k0 + tk;
         if (gi >= g.nnx || gj >= g.nny || gk >= g.nnz) continue;
         const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
@@ -263,30 +269,30 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, Tile t)
         const unsigned msk = a.mask[nod];
         const int hc = 3 * (((ti + 1) * HK + (tk + 1)) * HJ + (tj + 1));
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const bool fixed = (msk >> c) & 1u;
-            const float y = acc[hc + c];
-            const float u = su[hc + c];
+        for (int cc = 0; cc < 3; ++cc) {
+            const bool fixed = (msk >> cc) & 1u;
+            const float y = acc[hc + cc];
+            const float u = su[hc + cc];
             if (MODE == OP_MATVEC) {
                 const float v = fixed ? 0.0f : y;
-                a.out[dof + c] = v;
+                a.out[dof + cc] = v;
                 part += (double)u * (double)v;
             } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
-                const float rv = fixed ? 0.0f : a.F[dof + c] - y;
-                a.out[dof + c] = rv;
-                if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = u;
+                const float rv = fixed ? 0.0f : a.F[dof + cc] - y;
+                a.out[dof + cc] = rv;
+                if (MODE == OP_PRE_DEFECT) a.out2[dof + cc] = u;
             } else {
-                const float f = a.F[dof + c];
+                const float f = a.F[dof + cc];
                 const float rv = fixed ? 0.0f : f - y;
-                const float ad = a.AD[dof + c];
+                const float ad = a.AD[dof + cc];
                 const float un = (1.0f - om) * u + om * (rv + ad * u) / ad;
-                a.out[dof + c] = un;
+                a.out[dof + cc] = un;
                 part += (double)f * (double)un;
             }
         }
     }
     if (a.partials) {
-        const double s = mf_block_sum(part);
+        const double s = block_sum(part);
         if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
     }
 }
@@ -323,13 +329,13 @@ static size_t mf_lds_bytes(const Tile &t)
 {
     const size_t NH = (size_t)(t.TJ + 2) * (t.TK + 2) * (t.TI + 2);
     const size_t NE = (size_t)(t.TJ + 1) * (t.TK + 1) * (t.TI + 1);
-    return (6 * NH + 2 * NE) * sizeof(float);
+    return (6 * NH + NE) * sizeof(float);
 }
 
 template <int FL>
 static void launch_flavor(hipStream_t s, int mode, const OpArgsT<float> &a, const Tile &t)
 {
-    const dim3 grid((unsigned)t.blocks()), block(256);
+    const dim3 grid((unsigned)xcd_launch(t.blocks())), block(256);
     const size_t lds = mf_lds_bytes(t);
     switch (mode) {
     case OP_MATVEC: k_mfma_op<FL, OP_MATVEC><<<grid, block, lds, s>>>(a, t); break;
@@ -349,11 +355,13 @@ void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const voi
         launch_flavor<MF_F32>(s, mode, a, t);
     else if (flavor == MF_BF16X3)
         launch_flavor<MF_BF16X3>(s, mode, a, t);
+    else if (flavor == MF_BF16X1)
+        launch_flavor<MF_BF16X1>(s, mode, a, t);
     else
         launch_flavor<MF_BF16>(s, mode, a, t);
 }
 
-long mfma_tile_blocks(const Grid &g) { return mfma_tile(g).blocks(); }
+long mfma_tile_blocks(const Grid &g) { return xcd_launch(mfma_tile(g).blocks()); }
 
 void mfma_create(gputo_handle *h, MixedState *m)
 {
@@ -367,9 +375,7 @@ void mfma_destroy(gputo_handle *, MixedState *m)
     m->mf = nullptr;
 }
 
-void mfma_prepare(gputo_handle *, MixedState *) {}
-
-int mfma_blocks(const gputo_handle *h) { return (int)h->mx->mf->t.blocks(); }
+int mfma_blocks(const gputo_handle *h) { return (int)xcd_launch(h->mx->mf->t.blocks()); }
 
 // ---------------------------------------------------------------------------
 // Lane-layout self-checks of the two MFMA instructions the level-0 kernel

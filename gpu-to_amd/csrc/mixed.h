@@ -39,7 +39,6 @@ HierView<float> mixed_level_view(gputo_handle *h);
 // split-BF16 MFMA level-0 operator (mfma.hip)
 void mfma_create(gputo_handle *h, MixedState *m);
 void mfma_destroy(gputo_handle *h, MixedState *m);
-void mfma_prepare(gputo_handle *h, MixedState *m);  // Ke splits, E in FP32
 int mfma_blocks(const gputo_handle *h);             // blocks (= partials) of one launch
 long mfma_tile_blocks(const Grid &g0);              // the same, before the state exists
 

@@ -92,7 +92,7 @@ void mixed_stats(gputo_handle *h, gputo_to_stats *st)
 // node-gather for MIXED_F32 and the three-way split-BF16 MFMA for MIXED_BF16.
 static int l0_kind(const gputo_params &p, int precision)
 {
-    if (p.l0_kernel >= L0_GATHER && p.l0_kernel <= L0_MFMA_BF16X3) return p.l0_kernel;
+    if (p.l0_kernel >= L0_GATHER && p.l0_kernel <= L0_MFMA_BF16X1) return p.l0_kernel;
     return precision == GPUTO_PREC_MIXED_BF16 ? L0_MFMA_BF16X3 : L0_GATHER;
 }
 
@@ -120,7 +120,6 @@ static void mixed_prepare(gputo_handle *h)
     m->H.l0 = l0_kind(h->prm, h->prm.precision);
     m->H.l0_ctx = m;
     m->H.learn = h->ln != nullptr;
-    if (m->H.l0 == L0_MFMA_BF16) mfma_prepare(h, m);
 }
 
 

@@ -733,7 +733,7 @@ int gputo_matvec_f64(gputo_handle *h, const double *Ke0, const double *CX, const
 int gputo_matvec_mixed(gputo_handle *h, int kernel, const double *Ke0, const double *CX, const double *u,
                        const double *fixed, int nfd, double *y)
 {
-    if (!h || !u || !y || kernel < 1 || kernel > 4) return GPUTO_ERR_ARGUMENT;
+    if (!h || !u || !y || kernel < 1 || kernel > 5) return GPUTO_ERR_ARGUMENT;
     try {
         const Grid &g0 = h->L[0].g;
         std::vector<int> fx;
@@ -887,7 +887,7 @@ int gputo_interp_f64(gputo_handle *h, int level, const double *u_coarse, double 
 
 int gputo_bench_operator(gputo_handle *h, int kernel, int reps, double *avg_ms)
 {
-    if (!h || !h->hier_ready || reps < 1 || !avg_ms || kernel < 0 || kernel > 4) return GPUTO_ERR_ARGUMENT;
+    if (!h || !h->hier_ready || reps < 1 || !avg_ms || kernel < 0 || kernel > 5) return GPUTO_ERR_ARGUMENT;
     try {
         hipStream_t s = h->stream;
         const Grid &g0 = h->L[0].g;

@@ -32,7 +32,7 @@ inline Preset preset_of(const gputo_params &prm)
 
 // Level-0 operator implementation of a view.
 // (values match gputo_params.l0_kernel; MFMA flavour = kind - L0_MFMA_F32)
-enum L0Kind { L0_AUTO = 0, L0_GATHER = 1, L0_MFMA_F32 = 2, L0_MFMA_BF16 = 3, L0_MFMA_BF16X3 = 4 };
+enum L0Kind { L0_AUTO = 0, L0_GATHER = 1, L0_MFMA_F32 = 2, L0_MFMA_BF16 = 3, L0_MFMA_BF16X3 = 4, L0_MFMA_BF16X1 = 5 };
 
 template <class T>
 struct LevelView {

@@ -32,6 +32,7 @@ L0_GATHER = 1
 L0_MFMA_F32 = 2
 L0_MFMA_BF16 = 3
 L0_MFMA_BF16X3 = 4
+L0_MFMA_BF16X1 = 5
 
 # level-1 operator (gputo_params.l1_kernel)
 L1_AUTO = 0

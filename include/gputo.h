@@ -73,7 +73,8 @@ typedef struct gputo_params {
     int print_level;   /* 0 silent; >=1 summary on stderr; >=2 per-iteration */
     int l0_kernel;     /* level-0 operator of the FP32 V-cycle: 0 by precision, 1 FP32
                           node-gather, 2 FP32 MFMA, 3 split-BF16 MFMA (hi+lo),
-                          4 split-BF16 MFMA (hi+mid+lo) */
+                          4 split-BF16 MFMA (hi+mid+lo), 5 single-product
+                          BF16 MFMA (smoother-grade accuracy only) */
     int nswp;          /* GPUTO_PRESET_MGCG9: Jacobi sweeps each way (0: 2, the MEX default) */
     int l1_kernel;     /* level-1 operator in the V-cycle: 0 by preset (matrix-free
                           where level 1 is rediscretised from the child densities,
@@ -124,7 +125,8 @@ int gputo_matvec_f64(gputo_handle *h, const double *Ke0, const double *CX, const
                      const double *fixed, int nfd, double *y);
 /* The same product with a reduced-precision level-0 kernel of the
  * mixed-precision solve, FP64 in and out: kernel 1 FP32 node-gather, 2 FP32
- * MFMA, 3 split-BF16 MFMA (hi+lo), 4 split-BF16 MFMA (hi+mid+lo). */
+ * MFMA, 3 split-BF16 MFMA (hi+lo), 4 split-BF16 MFMA (hi+mid+lo), 5 single
+ * BF16 MFMA product bf16(Ke) bf16(u). */
 int gputo_matvec_mixed(gputo_handle *h, int kernel, const double *Ke0, const double *CX, const double *u,
                        const double *fixed, int nfd, double *y);
 /* Lane-layout check of the BF16 and FP32 MFMA instructions the level-0
@@ -157,7 +159,7 @@ int gputo_interp_f64(gputo_handle *h, int level, const double *u_coarse, double 
 /* Mean duration (ms, HIP events on the handle's stream) of `reps`
  * back-to-back launches of the level-0 operator y = K(CX) u on the
  * hierarchy of the last solve / setup (microbenchmark of multAV0c,
- * mgcg1.cu:941-997): kernel 0 FP64 node-gather, 1-4 as gputo_matvec_mixed. */
+ * mgcg1.cu:941-997): kernel 0 FP64 node-gather, 1-5 as gputo_matvec_mixed. */
 int gputo_bench_operator(gputo_handle *h, int kernel, int reps, double *avg_ms);
 /* One application of the operator of level `level` (0 <= level < nl-1) of
  * the hierarchy of the last solve / setup: y = A u (mode 0, masked; multAV,

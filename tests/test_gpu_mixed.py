@@ -30,7 +30,7 @@ def test_mfma_lane_layout(gputo):
 
 
 @pytest.mark.parametrize("mesh", [(8, 4, 4, 3), (16, 8, 8, 3), (24, 12, 12, 3), (48, 24, 24, 4)])
-@pytest.mark.parametrize("kernel,tol", [(1, 1e-6), (2, 1e-6), (3, 1e-4), (4, 1e-6)])
+@pytest.mark.parametrize("kernel,tol", [(1, 1e-6), (2, 1e-6), (3, 1e-4), (4, 1e-6), (5, 3e-2)])
 def test_reduced_precision_operator(gputo, oracle, mesh, kernel, tol):
     nelx, nely, nelz, nl = mesh
     print("case", mesh, kernel, flush=True)
@@ -44,7 +44,7 @@ def test_reduced_precision_operator(gputo, oracle, mesh, kernel, tol):
     s.close()
 
 
-@pytest.mark.parametrize("prec,kernel", [(1, 1), (1, 2), (2, 3), (2, 4)])
+@pytest.mark.parametrize("prec,kernel", [(1, 1), (1, 2), (2, 3), (2, 4), (2, 5)])
 @pytest.mark.parametrize("mesh", [(16, 8, 8, 3), (48, 24, 24, 4)])
 def test_mixed_solve_matches_f64(gputo, oracle, prec, kernel, mesh):
     nelx, nely, nelz, nl = mesh
@@ -56,9 +56,11 @@ def test_mixed_solve_matches_f64(gputo, oracle, prec, kernel, mesh):
     sm = gputo.Solver(nelx, nely, nelz, nl, precision=prec, l0_kernel=kernel)
     Um, rm, nm, rc = sm.solve(Ke, F, CX, fixed, 1e-10, 300)
     assert rc == 0 and rm < 1e-10
-    # FP32 operators with residual replacement keep the FP64 iteration count
-    # (split-BF16 hi+lo operator: within a few iterations)
-    assert nm <= n64 + (2 if kernel != 3 else 40), (nm, n64)
+    # FP32 level-0 operators in the preconditioner keep the FP64 iteration
+    # count; the BF16 ones perturb the preconditioner (hi+lo: relative error
+    # ~1e-5, single product: ~1e-2) and cost iterations, never accuracy
+    print("iterations", n64, nm, flush=True)
+    assert nm <= n64 + {3: 40, 5: 2 * n64}.get(kernel, 2), (nm, n64)
     # the outer residual is the true FP64 residual: check it with the oracle
     Uc = Um.copy()
     Uc[1::3] *= -1

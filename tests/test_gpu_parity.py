@@ -236,6 +236,26 @@ def test_mgcg1_dense_direct(solvers):
     assert abs(F @ U - F @ Ud) <= 1e-10 * abs(F @ Ud)
 
 
+def test_wide_band_coarsest(gputo):
+    """Coarsest factorisations on both sides of the single-workgroup band
+    limit (256 rows below the diagonal tile): a 2-level hierarchy at
+    32x16x16 factors a 17x9x9-node coarsest grid (band 275 rows, tile-parallel
+    kernels), a 4-level one a 5x3x3-node grid (band 62 rows, one workgroup).
+    Both solves reach the same displacements."""
+    nelx, nely, nelz = 32, 16, 16
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=18)
+    out = {}
+    for nl in (2, 4):
+        print("case", nl, flush=True)
+        s = gputo.Solver(nelx, nely, nelz, nl)
+        U, relres, nit, rc = s.solve(Ke, F, CX, fixed, 1e-11, 300)
+        assert rc == 0 and relres < 1e-11
+        out[nl] = (U, nit)
+        s.close()
+    assert out[2][1] <= out[4][1]
+    assert rel(out[2][0], out[4][0]) < 1e-8
+
+
 def test_mgcg1_matlab_interface(gputo, oracle):
     """The reference call forms: CUDA 12/13 args and omp 13 args with WRK."""
     nelx, nely, nelz, nl = 16, 8, 8, 3
