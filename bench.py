@@ -20,15 +20,34 @@ sys.path[:0] = [os.path.join(ROOT, "gpu-to_amd"), os.path.join(ROOT, "oracle")]
 METRIC = "wall-s/TO iter @1.77M hex, mixed+learned vs FP64 MGCG; matvec HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "level0_operator_cantilever_192x96x96_pmc.md")
 # variant -> (precision, level-0 kernel of the inner solve); kernel ids as
 # gputo_params.l0_kernel (0 = FP64 node-gather for the plain path)
 VARIANTS = {
     "f64": (0, 0),
     "mixed_gather": (1, 1),  # FP32 inner solve, FP32 node-gather level-0 operator
-    "mixed_f32": (1, 2),     # FP32 inner solve, FP32 MFMA level-0 operator
-    "mixed_bf16x2": (2, 3),  # split-BF16 MFMA, hi+lo
-    "mixed_bf16": (2, 4),    # split-BF16 MFMA, hi+mid+lo
+    "mixed_mfma9": (2, 2),   # MFMA level-0 operator, three-way BF16 split, all nine products
+    "mixed_mfma": (2, 4),    # MFMA level-0 operator, three-way BF16 split, six products
+    "mixed_mfma3": (2, 3),   # MFMA level-0 operator, two-way BF16 split, three products
+    "mixed_mfma1": (2, 5),   # MFMA level-0 operator, one BF16 product
 }
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of level-0 kernel `kernel` at 192x96x96 from the
+    committed counter summary: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
+    factors that make the same counters read 2 x 256 MiB for two 256 MiB
+    device copies."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic_bytes" and w[1] == "kernel%d" % kernel:
+                    return int(w[2])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -114,6 +133,35 @@ def cpu_baseline(args, Ke, F, fixed, x0, nit_avg):
     }
 
 
+def probe_operator(args):
+    """Short program for rocprofv3 --pmc passes: the level-0 operator at the
+    bench size, then a plain device-to-device copy of known size (HIP's copy
+    kernel) that calibrates FETCH_SIZE / WRITE_SIZE."""
+    import gputo
+    Ke, F, fixed = gputo.cantilever(args.nelx, args.nely, args.nelz)
+    x0 = gputo.problems.perturbed_density(args.nelx, args.nely, args.nelz, 0.12, seed=args.seed)
+    CX = 1e-9 + x0 ** 3 * (1 - 1e-9)
+    s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=1)
+    s.setup(Ke, CX, fixed)
+    avg = ctypes.c_double()
+    for k in [int(v) for v in args.probe_op.split(",")]:
+        rc = s.lib.gputo_bench_operator(s.h, k, 3, ctypes.byref(avg))
+        log("level-0 kernel %d: %.1f us (rc %d)" % (k, avg.value * 1e3, rc))
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(2):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    log("copy calibration: 2 copies of %d bytes" % n)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,10 +179,16 @@ def main():
     ap.add_argument("--learn-lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-op", default="",
+                    help="counter probe: a few launches of these level-0 kernels (ids as gputo_bench_operator) "
+                         "and two 256 MiB device copies, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.probe_op:
+        probe_operator(args)
+        return
 
     import gputo
     lib = gputo.load()
@@ -151,17 +205,26 @@ def main():
     r = res[head]
     base = res.get("f64")
 
-    # roofline of the dominant kernel (level-0 operator)
-    s_bytes = 8 if head == "f64" else 4
+    # roofline of the dominant kernel (the level-0 operator of the head
+    # variant).  The node-gather kernels (0: FP64, 1: FP32) run their 1152
+    # flop per element on the vector ALUs, an arithmetic intensity above the
+    # ridge of both precisions: priced against the compute peak of their
+    # type.  The MFMA kernels (2-5) have the element products on the matrix
+    # cores at a small fraction of their peak: priced against HBM.
+    kernel = VARIANTS[head.replace("+learned", "")][1]
+    s_bytes = 8 if kernel == 0 else 4
     op_bytes = (2 * ndof + nel) * s_bytes
     op_flops = 1152.0 * nel
     op_s = r["op_ms"] * 1e-3 if r["op_ms"] else None
-    if head == "f64":
-        roof = {"bound": "mfma", "achieved": op_flops / op_s / 1e12 if op_s else None, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "traffic": None}
+    default_size = (args.nelx, args.nely, args.nelz, args.nl) == (192, 96, 96, 5)
+    traffic = pmc_traffic(kernel) if default_size else None
+    if kernel <= 1:
+        roof = {"bound": "mfma", "achieved": op_flops / op_s / 1e12 if op_s else None,
+                "peak": FP64_PEAK_TFLOPS if kernel == 0 else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "traffic": traffic}
     else:
         roof = {"bound": "hbm", "achieved": op_bytes / op_s / 1e9 if op_s else None, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "traffic": None}
+                "unit": "GB/s", "traffic": traffic}
     roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
 
     # the CPU baseline runs the FP64 reference algorithm: price it at the FP64 CG count

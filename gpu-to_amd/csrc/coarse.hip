@@ -439,12 +439,14 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
 }
 
 // One full wave factors the 32x32 diagonal tile at T (row stride ld, lower
-// triangle): lane r < NB holds row r in registers; column jj's pivot and
-// multipliers are read across lanes into scalar registers (v_readlane), so
-// the 32 elimination steps need neither LDS nor a barrier.  Returns true
-// on a non-positive pivot.
+// triangle): lane r < NB holds row r in registers.  At step jj every lane
+// publishes its multiplier L[r][jj] in LDS and reads the column back with
+// broadcast loads (LDS operations of one wave complete in order, so no
+// barrier is involved); the pivot comes from lane jj by v_readlane.
+// Returns true on a non-positive pivot.
 __device__ __forceinline__ bool potrf_tile(double *T, long ld)
 {
+    __shared__ double col[64];
     const int lane = threadIdx.x & 63;
     const int r = lane < NB ? lane : NB - 1;  // lanes >= NB shadow row NB-1, results dropped
     double L[NB];
@@ -461,10 +463,19 @@ __device__ __forceinline__ bool potrf_tile(double *T, long ld)
         d = sqrt(d);
         const double lj = r == jj ? d : (r > jj ? L[jj] / d : 0.0);
         L[jj] = lj;
+        if (jj + 1 < NB) {
+            col[lane] = lj;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int c = jj + 1; c < NB; ++c) {
-            const double lc = readlane_f64(lj, c);  // L[c][jj]
-            if (c <= r) L[c] -= lj * lc;
+            for (int c = jj + 1; c < NB; ++c) {
+                const double lc = col[c];  // L[c][jj], broadcast
+                if (c <= r) L[c] -= lj * lc;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
     if (lane < NB) {
@@ -656,11 +667,82 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_band(double *A, int N, in
 
 // Column block jb of L^-1 in one workgroup, marching down the block rows:
 // X[ib][jb] = -inv(L[ib][ib]) sum_{kb=jb}^{ib-1} L[ib][kb] X[kb][jb] with
-// the sum over the factor's band and X[jb][jb] = inv(L[jb][jb]) written by
-// k_trtri_diag.  The column blocks are independent: one launch of nb
-// workgroups replaces nb - 1 launches of the row-block kernel, with the same
-// operations in the same order.  256 threads, 2x2 outputs each.
+// the sum over the factor's band (kb >= ib - nbt) and X[jb][jb] =
+// inv(L[jb][jb]) written by k_trtri_diag.  The column blocks are
+// independent: one launch of nb - 1 workgroups.  The last kLinvWin tiles of
+// the column stay in LDS (a circular window), the factor tile of the next
+// product is prefetched into registers while the current one is
+// multiplied, and each thread owns 2x2 entries of the 32x32 result.
+constexpr int kLinvWin = 8;
+constexpr int kXs = NB + 2;  // even row stride: 16-byte aligned pairs
+
 __global__ __launch_bounds__(256) void k_linv_cols(const double *A, double *Linv, int N, int nbt)
+{
+    __shared__ __align__(16) double X[kLinvWin][NB][kXs];
+    __shared__ double La[NB][NB + 1];
+    const int jb = blockIdx.x, nb = N / NB, tid = threadIdx.x;
+    const int r0 = (tid >> 4) * 2, c0 = (tid & 15) * 2;
+    // 4 tile entries per thread for the copies: rows tid/32 + 8u, column tid%32
+    const int cr = tid >> 5, cc = tid & 31;
+    for (int u = 0; u < 4; ++u) X[jb % kLinvWin][cr + 8 * u][cc] = Linv[(long)(jb * NB + cr + 8 * u) * N + jb * NB + cc];
+    for (int ib = jb + 1; ib < nb; ++ib) {
+        const int kb0 = max(jb, ib - nbt);
+        double pf[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pf[u] = A[(long)(ib * NB + cr + 8 * u) * N + kb0 * NB + cc];
+        double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+        for (int kb = kb0; kb <= ib; ++kb) {
+            __syncthreads();  // previous readers of La are done
+#pragma unroll
+            for (int u = 0; u < 4; ++u) La[cr + 8 * u][cc] = pf[u];
+            if (kb < ib) {
+                // next factor tile, or inv(L[ib][ib]) for the final product
+                const double *src = kb + 1 < ib ? A + (long)(ib * NB) * N + (kb + 1) * NB
+                                                : Linv + (long)(ib * NB) * N + ib * NB;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pf[u] = src[(long)(cr + 8 * u) * N + cc];
+            } else {
+                // final product: B = the accumulated sum
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) X[ib % kLinvWin][r0 + i][c0 + j] = acc[i][j];
+            }
+            __syncthreads();
+            const double(*B)[kXs] = X[(kb < ib ? kb : ib) % kLinvWin];
+            double sm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll 8
+            for (int m = 0; m < NB; ++m) {
+                const double a0 = La[r0][m], a1 = La[r0 + 1][m];
+                const double2 bv = *reinterpret_cast<const double2 *>(&B[m][c0]);
+                sm[0][0] += a0 * bv.x;
+                sm[0][1] += a0 * bv.y;
+                sm[1][0] += a1 * bv.x;
+                sm[1][1] += a1 * bv.y;
+            }
+            if (kb < ib) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] += sm[i][j];
+            } else {
+                __syncthreads();  // every thread has read the accumulated sum
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        X[ib % kLinvWin][r0 + i][c0 + j] = -sm[i][j];
+                        Linv[(long)(ib * NB + r0 + i) * N + jb * NB + c0 + j] = -sm[i][j];
+                    }
+            }
+        }
+    }
+}
+
+// The same product for bands wider than the window: the previous tiles of
+// the column are read back from Linv (written by this workgroup, visible
+// after the barrier).
+__global__ __launch_bounds__(256) void k_linv_cols_wide(const double *A, double *Linv, int N, int nbt)
 {
     __shared__ double Ta[NB][NB + 1], Tb[NB][NB + 1];
     const int jb = blockIdx.x, nb = N / NB, tid = threadIdx.x;
@@ -715,15 +797,23 @@ __global__ __launch_bounds__(256) void k_linv_cols(const double *A, double *Linv
     }
 }
 
-int coarse_factor(hipStream_t s, Coarse &C)
+void coarse_factor(hipStream_t s0, Coarse &C, bool lag)
 {
+    C.target = (lag && C.have) ? 1 - C.cur : C.cur;
+    C.solve_waits = C.target == C.cur;
+    C.Linv = C.Lbuf[C.cur];
+    C.LinvT = C.LTbuf[C.cur];
+    double *Li = C.Lbuf[C.target], *LiT = C.LTbuf[C.target];
     const int N = (C.n + NB - 1) / NB * NB;
     const int nb = N / NB;
     // the factor of a banded matrix keeps its band: panel rows and trailing
     // tiles beyond it are exact zeros and are skipped
     const int bw = std::max(1, std::min(C.bw, N));
     const int band_rows = (bw + NB - 1) / NB * NB;
-    hipMemsetAsync(C.flag, 0, sizeof(int), s);
+    hipStream_t s = C.side;
+    HIPCHK(hipEventRecord(C.ev_in, s0));
+    HIPCHK(hipStreamWaitEvent(s, C.ev_in, 0));
+    HIPCHK(hipMemsetAsync(C.flag, 0, sizeof(int), s));
     if (band_rows <= kBandRowsMax) {
         k_chol_band<<<1, kCholThreads, 0, s>>>(C.AQ, N, band_rows, C.flag);
     } else {
@@ -738,15 +828,45 @@ int coarse_factor(hipStream_t s, Coarse &C)
             }
         }
     }
-    launch_fill(s, C.Linv, (long)N * N, 0.0);
-    k_trtri_diag<<<nb, 64, 0, s>>>(C.AQ, C.Linv, N);
-    const int nbt = (bw + 2 * NB - 2) / NB;
-    if (nb > 1) k_linv_cols<<<nb - 1, 256, 0, s>>>(C.AQ, C.Linv, N, nbt);
-    k_transpose<<<dim3(nb, nb), 256, 0, s>>>(C.Linv, C.LinvT, N);
+    launch_fill(s, Li, (long)N * N, 0.0);
+    k_trtri_diag<<<nb, 64, 0, s>>>(C.AQ, Li, N);
+    const int nbt = std::min((bw + 2 * NB - 2) / NB, nb);
+    if (nb > 1) {
+        if (nbt < kLinvWin)
+            k_linv_cols<<<nb - 1, 256, 0, s>>>(C.AQ, Li, N, nbt);
+        else
+            k_linv_cols_wide<<<nb - 1, 256, 0, s>>>(C.AQ, Li, N, nbt);
+    }
+    k_transpose<<<dim3(nb, nb), 256, 0, s>>>(Li, LiT, N);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(C.ev_done, s));
+    C.pending = true;
+}
+
+int coarse_check(Coarse &C)
+{
+    if (!C.pending || !C.solve_waits) return 0;  // a lagged factor is checked when adopted
     int flag = 0;
-    hipMemcpyAsync(&flag, C.flag, sizeof(int), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
+    HIPCHK(hipMemcpyAsync(&flag, C.flag, sizeof(int), hipMemcpyDeviceToHost, C.side));
+    HIPCHK(hipStreamSynchronize(C.side));
+    C.pending = false;
+    C.have = flag == 0;
     return flag ? GPUTO_ERR_NOT_SPD : 0;
+}
+
+void coarse_adopt(Coarse &C)
+{
+    if (!C.pending) return;
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, C.flag, sizeof(int), hipMemcpyDeviceToHost, C.side));
+    HIPCHK(hipStreamSynchronize(C.side));  // launched a solve ago: normally finished
+    C.pending = false;
+    if (flag) {
+        C.have = false;  // factor again, exactly, at this setup
+        return;
+    }
+    C.cur = C.target;
+    C.have = true;
 }
 
 // ---------------------------------------------------------------------------
@@ -762,8 +882,15 @@ __global__ __launch_bounds__(64) void k_trmv(const double *__restrict__ M, const
     if (ctl && ctl[0]) return;
     const int row = blockIdx.x;
     const int lo = UPPER ? row : 0, hi = UPPER ? n - 1 : (row < n ? row : n - 1);
-    double s = 0.0;
-    for (int c = lo + threadIdx.x; c <= hi; c += 64) s += M[(long)row * N + c] * (double)b[c];
+    // four independent partial sums keep four row loads in flight per lane
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    int c = lo + threadIdx.x;
+    for (; c + 192 <= hi; c += 256) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s4[u] += M[(long)row * N + c + 64 * u] * (double)b[c + 64 * u];
+    }
+    for (int u = 0; c <= hi; c += 64, ++u) s4[u & 3] += M[(long)row * N + c] * (double)b[c];
+    double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (threadIdx.x == 0 && row < n) {
@@ -778,6 +905,7 @@ template <class T>
 void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl)
 {
     const int N = (C.n + NB - 1) / NB * NB;
+    if (C.solve_waits) HIPCHK(hipStreamWaitEvent(s, C.ev_done, 0));  // the factorisation of this hierarchy
     k_trmv<false, T, double><<<C.n, 64, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);
     k_trmv<true, double, T><<<C.n, 64, 0, s>>>(C.LinvT, C.y, x, N, C.n, mask, ctl);
 }

@@ -114,11 +114,22 @@ struct Coarse {
     int n = 0;
     bool direct = true;      // factored (else smoothed: mgcg9 above 20000 DOFs)
     int bw = 0;              // lower bandwidth of the assembled matrix (node-major DOFs)
-    double *AQ = nullptr;    // dense matrix, then its Cholesky factor (lower)
-    double *Linv = nullptr;  // inverse of the factor (lower, row-major)
-    double *LinvT = nullptr; // its transpose (row-major)
+    double *AQ = nullptr;       // dense matrix, then its Cholesky factor (lower)
+    double *Linv = nullptr;     // inverse of the factor used by the solves (lower, row-major)
+    double *LinvT = nullptr;    // its transpose (row-major)
+    double *Lbuf[2] = {}, *LTbuf[2] = {};  // two inverse buffers (lagged factorisation)
+    int cur = 0;                // buffer of Linv/LinvT
+    int target = 0;             // buffer the factorisation in flight writes
+    bool have = false;          // buffers[cur] hold a factor of an earlier setup
+    bool solve_waits = true;    // solves wait for the factorisation in flight
     double *y = nullptr;     // work vector
     int *flag = nullptr;     // SPD failure flag
+    // the factorisation runs on a side stream, overlapped with the start of
+    // the solve: ev_in orders it after the assembly, ev_done orders every
+    // coarsest solve and the next assembly after it
+    hipStream_t side = nullptr;
+    hipEvent_t ev_in = nullptr, ev_done = nullptr;
+    bool pending = false;  // factorisation launched, SPD flag not read yet
 };
 
 struct MixedState;
@@ -192,7 +203,12 @@ void launch_coarse_cx(hipStream_t s, const Grid &gc, const double *CXf, double *
 void launch_ke_elements(hipStream_t s, const Grid &g, const double *CX, const double *Ke, double *A);
 void launch_zero_rowcol(hipStream_t s, int N, const int *fixed_d, int nfd, double *AQ);
 void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ);
-int coarse_factor(hipStream_t s, Coarse &C);
+// Factor C.AQ on C.side after the work on s.  lag: when an earlier factor
+// exists, write the new one into the other buffer and let the solves keep
+// the earlier one (coarse_adopt switches at the next setup).
+void coarse_factor(hipStream_t s, Coarse &C, bool lag);
+int coarse_check(Coarse &C);  // waits for an exact factorisation; GPUTO_ERR_NOT_SPD on a bad pivot
+void coarse_adopt(Coarse &C);  // before a setup: take a finished lagged factor
 template <class T>
 void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl);
 
@@ -209,7 +225,8 @@ template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long
 void launch_cg_beta_flex(hipStream_t s, const double *r, const double *z, const double *q, long n, CgScalars *sc,
                          double *partials, int maxit);
 
-void check_hip(hipError_t e, const char *what);
+void check_hip(hipError_t e, const char *what);  // throws on an error
+#define HIPCHK(x) ::gputo::check_hip((x), #x)
 void set_ke(gputo_handle *h, const double *Ke0);  // upload Ke0 and detect its class structure
 
 // Deterministic sum over a kBlock-thread block: fixed butterfly per wave,

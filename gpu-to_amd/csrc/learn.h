@@ -48,6 +48,7 @@ struct LearnState {
     double *sc = nullptr;        // device scalars: sigma, gamma, loss, dots
     int *count = nullptr;        // device counters: accepted, rejected
     double *partials = nullptr;  // reduction partials
+    float *wpart = nullptr;      // weight-gradient partials per position chunk
     // logged level-2 defects (training samples) of the current solve
     int nsamp_max = 0, nsamp = 0, log_every = 1, vc_count = 0;
     float *samples = nullptr;

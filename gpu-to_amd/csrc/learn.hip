@@ -36,94 +36,153 @@ __device__ __forceinline__ void pos3(long p, const Dims3 &d, int &i, int &k, int
     i = (int)(r / d.n1);
 }
 
-// out[co][p] = act(b[co] + sum_ci sum_t w[co][ci][t] in[ci][p*s + t - 1]);
-// CT output channels per thread; weights are block-uniform (scalar loads).
-template <int CT, bool RELU>
+// out[co][p] = act(b[co] + sum_ci sum_t w[co][ci][t] in[ci][p*s + t - 1]).
+// A block owns 256/KS output positions and CT output channels; its
+// CT x cin x 27 weights are staged in LDS as [ci][t][c] (one vector read
+// per tap feeds CT FMAs) and the 27 tap offsets of a position are computed
+// once.  KS > 1 splits the input channels over KS threads per position
+// (small grids have too few positions to fill the device); the KS partial
+// sums are added in a fixed order.
+template <int CT, int KS, bool RELU>
 __global__ __launch_bounds__(256) void k_conv_fwd(const float *__restrict__ in, int cin, Dims3 din,
                                                  const float *__restrict__ w, const float *__restrict__ b,
                                                  float *__restrict__ out, Dims3 dout, int stride, const int *ctl)
 {
     if (ctl && ctl[0]) return;
-    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    extern __shared__ float wl[];  // [cin*27][CT], then KS partials [KS][256/KS][CT]
+    constexpr int NP = 256 / KS;
     const int co0 = blockIdx.y * CT;
+    for (int q = threadIdx.x; q < cin * 27 * CT; q += 256) {
+        const int c = q % CT, k = q / CT;  // k = ci*27 + t
+        wl[q] = w[(long)(co0 + c) * cin * 27 + k];
+    }
+    __syncthreads();
+    const int pl = threadIdx.x % NP, ks = threadIdx.x / NP;
+    const long p = (long)blockIdx.x * NP + pl;
     const long nout = dout.size(), nin = din.size();
-    if (p >= nout) return;
-    int oi, ok, oj;
-    pos3(p, dout, oi, ok, oj);
     float acc[CT];
 #pragma unroll
-    for (int c = 0; c < CT; ++c) acc[c] = b[co0 + c];
-    for (int ci = 0; ci < cin; ++ci) {
-        const float *src = in + (long)ci * nin;
-        const float *wc = w + ((long)co0 * cin + ci) * 27;
+    for (int c = 0; c < CT; ++c) acc[c] = 0.0f;
+    if (p < nout) {
+        int oi, ok, oj;
+        pos3(p, dout, oi, ok, oj);
+        int off[27];
+        unsigned valid = 0;
 #pragma unroll
         for (int t = 0; t < 27; ++t) {
             const int ii = oi * stride + t / 9 - 1, ik = ok * stride + (t / 3) % 3 - 1, ij = oj * stride + t % 3 - 1;
-            float v = 0.0f;
-            if (ii >= 0 && ii < din.n0 && ik >= 0 && ik < din.n1 && ij >= 0 && ij < din.n2)
-                v = src[((long)ii * din.n1 + ik) * din.n2 + ij];
-#pragma unroll
-            for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[(long)c * cin * 27 + t], v, acc[c]);
+            const bool v = ii >= 0 && ii < din.n0 && ik >= 0 && ik < din.n1 && ij >= 0 && ij < din.n2;
+            valid |= (unsigned)v << t;
+            off[t] = v ? (ii * din.n1 + ik) * din.n2 + ij : 0;
         }
+        if (KS == 1) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[c] = b[co0 + c];
+        }
+        for (int ci = ks; ci < cin; ci += KS) {
+            const float *src = in + (long)ci * nin;
+            const float *wc = wl + ci * 27 * CT;
+#pragma unroll
+            for (int t = 0; t < 27; ++t) {
+                const float v = ((valid >> t) & 1u) ? src[off[t]] : 0.0f;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], v, acc[c]);
+            }
+        }
+    }
+    if (KS > 1) {
+        float *part = wl + cin * 27 * CT;
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < CT; ++c) part[(ks * NP + pl) * CT + c] = acc[c];
+        __syncthreads();
+        if (ks != 0 || p >= nout) return;
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            float sm = b[co0 + c];
+            for (int k = 0; k < KS; ++k) sm += part[(k * NP + pl) * CT + c];
+            acc[c] = sm;
+        }
+    } else if (p >= nout) {
+        return;
     }
 #pragma unroll
     for (int c = 0; c < CT; ++c) out[(long)(co0 + c) * nout + p] = RELU ? fmaxf(acc[c], 0.0f) : acc[c];
 }
 
 // gin[ci][q] = sum_co sum_t w[co][ci][t] gz[co][p] over the outputs p that
-// read q through tap t (q = p*s + t - 1), gz = gout * relu'(act).
+// read q through tap t (q = p*s + t - 1), gz = gout * relu'(act).  The
+// block's CT input channels of every output channel's weights are staged
+// in LDS as [co][t][c].
 template <int CT>
 __global__ __launch_bounds__(256) void k_conv_bwd_data(const float *__restrict__ gout, const float *__restrict__ act,
                                                       int cout, Dims3 dout, const float *__restrict__ w, int cin,
                                                       Dims3 din, int stride, float *__restrict__ gin)
 {
-    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    extern __shared__ float wl[];  // [cout*27][CT]
     const int ci0 = blockIdx.y * CT;
+    for (int q = threadIdx.x; q < cout * 27 * CT; q += 256) {
+        const int c = q % CT, k = q / CT;  // k = co*27 + t
+        const int co = k / 27, t = k - 27 * co;
+        wl[q] = w[((long)co * cin + ci0 + c) * 27 + t];
+    }
+    __syncthreads();
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
     const long nin = din.size(), nout = dout.size();
     if (q >= nin) return;
     int qi, qk, qj;
     pos3(q, din, qi, qk, qj);
+    int off[27];
+    unsigned valid = 0;
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        const int xi = qi + 1 - t / 9, xk = qk + 1 - (t / 3) % 3, xj = qj + 1 - t % 3;
+        int pi = xi, pk = xk, pj = xj;
+        bool ok = xi >= 0 && xk >= 0 && xj >= 0;
+        if (stride == 2) {
+            ok = ok && !(xi & 1) && !(xk & 1) && !(xj & 1);
+            pi = xi >> 1;
+            pk = xk >> 1;
+            pj = xj >> 1;
+        }
+        ok = ok && pi < dout.n0 && pk < dout.n1 && pj < dout.n2;
+        valid |= (unsigned)ok << t;
+        off[t] = ok ? (pi * dout.n1 + pk) * dout.n2 + pj : 0;
+    }
     float acc[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[c] = 0.0f;
     for (int co = 0; co < cout; ++co) {
         const float *g = gout + (long)co * nout;
         const float *a = act ? act + (long)co * nout : nullptr;
-        const float *wc = w + ((long)co * cin + ci0) * 27;
+        const float *wc = wl + co * 27 * CT;
 #pragma unroll
         for (int t = 0; t < 27; ++t) {
-            const int xi = qi + 1 - t / 9, xk = qk + 1 - (t / 3) % 3, xj = qj + 1 - t % 3;
-            int pi = xi, pk = xk, pj = xj;
-            bool ok = xi >= 0 && xk >= 0 && xj >= 0;
-            if (stride == 2) {
-                ok = ok && !(xi & 1) && !(xk & 1) && !(xj & 1);
-                pi = xi >> 1;
-                pk = xk >> 1;
-                pj = xj >> 1;
-            }
-            ok = ok && pi < dout.n0 && pk < dout.n1 && pj < dout.n2;
             float gz = 0.0f;
-            if (ok) {
-                const long p = ((long)pi * dout.n1 + pk) * dout.n2 + pj;
-                gz = g[p];
-                if (a && !(a[p] > 0.0f)) gz = 0.0f;
+            if ((valid >> t) & 1u) {
+                gz = g[off[t]];
+                if (a && !(a[off[t]] > 0.0f)) gz = 0.0f;
             }
 #pragma unroll
-            for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[(long)c * 27 + t], gz, acc[c]);
+            for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], gz, acc[c]);
         }
     }
 #pragma unroll
     for (int c = 0; c < CT; ++c) gin[(long)(ci0 + c) * nin + q] = acc[c];
 }
 
-// gw[co][ci][t] += sum_p gz[co][p] in[ci][p*s + t - 1];  gb[co] += sum_p gz
-// (ci == 0 block).  One block per (co, ci), fixed-order block reduction.
+// Weight gradient, split over chunks of kWChunk output positions:
+// part[ch][co*cin + ci][t] = sum_{p in chunk ch} gz[co][p] in[ci][p*s + t - 1]
+// and, at t = 27, sum_p gz[co][p].  One block per (co, ci, chunk), fixed-
+// order block reduction; k_conv_bwd_w_sum adds the chunks in order.
+constexpr int kWChunk = 2048;
+
 __global__ __launch_bounds__(256) void k_conv_bwd_w(const float *__restrict__ gout, const float *__restrict__ act,
                                                    Dims3 dout, const float *__restrict__ in, int cin, Dims3 din,
-                                                   int stride, float *__restrict__ gw, float *__restrict__ gb)
+                                                   int stride, float *__restrict__ part)
 {
     __shared__ float red[28][257];
-    const int co = blockIdx.x / cin, ci = blockIdx.x % cin;
+    const int pair = blockIdx.x, co = pair / cin, ci = pair % cin;
     const long nout = dout.size(), nin = din.size();
     const float *g = gout + (long)co * nout;
     const float *a = act ? act + (long)co * nout : nullptr;
@@ -131,7 +190,8 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(const float *__restrict__ go
     float acc[28];
 #pragma unroll
     for (int t = 0; t < 28; ++t) acc[t] = 0.0f;
-    for (long p = threadIdx.x; p < nout; p += 256) {
+    const long p1 = min(nout, (long)(blockIdx.y + 1) * kWChunk);
+    for (long p = (long)blockIdx.y * kWChunk + threadIdx.x; p < p1; p += 256) {
         float gz = g[p];
         if (a && !(a[p] > 0.0f)) gz = 0.0f;
         if (gz == 0.0f) continue;
@@ -153,8 +213,21 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(const float *__restrict__ go
             for (int t = 0; t < 28; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 27) gw[((long)co * cin + ci) * 27 + threadIdx.x] += red[threadIdx.x][0];
-    if (threadIdx.x == 27 && ci == 0) gb[co] += red[27][0];
+    if (threadIdx.x < 28) part[((long)blockIdx.y * gridDim.x + pair) * 28 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_conv_bwd_w_sum(const float *__restrict__ part, int nch, int npair, int cin, float *__restrict__ gw,
+                                 float *__restrict__ gb)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= npair * 28) return;
+    const int pair = q / 28, t = q - 28 * pair;
+    float sm = 0.0f;
+    for (int ch = 0; ch < nch; ++ch) sm += part[((long)ch * npair + pair) * 28 + t];
+    if (t < 27)
+        gw[(long)pair * 27 + t] += sm;
+    else if (pair % cin == 0)
+        gb[pair / cin] += sm;
 }
 
 // out[c][p] = d[c][parent(p)] + e[c][p]  (nearest upsampling by 2 plus skip)
@@ -361,42 +434,69 @@ T *ldalloc(LearnState *L, size_t n)
 
 Dims3 half(const Dims3 &d) { return {(d.n0 - 1) / 2 + 1, (d.n1 - 1) / 2 + 1, (d.n2 - 1) / 2 + 1}; }
 
+template <int CT, int KS>
+void conv_fwd_t(hipStream_t s, const NetLayer &ly, const float *w, const float *b, const float *in, Dims3 din,
+                float *out, Dims3 dout, const int *ctl)
+{
+    const dim3 grid((unsigned)((dout.size() + 256 / KS - 1) / (256 / KS)), ly.cout / CT);
+    const size_t lds = ((size_t)ly.cin * 27 * CT + (KS > 1 ? 256 * CT : 0)) * sizeof(float);
+    if (ly.relu)
+        k_conv_fwd<CT, KS, true><<<grid, 256, lds, s>>>(in, ly.cin, din, w, b, out, dout, ly.stride, ctl);
+    else
+        k_conv_fwd<CT, KS, false><<<grid, 256, lds, s>>>(in, ly.cin, din, w, b, out, dout, ly.stride, ctl);
+}
+
+// Input-channel split: the smallest KS (a power of two, at most cin) that
+// gives at least kConvBlocks blocks, so the small grids fill the device.
+constexpr long kConvBlocks = 1024;
+
+template <int CT>
+void conv_fwd_ct(hipStream_t s, const NetLayer &ly, const float *w, const float *b, const float *in, Dims3 din,
+                 float *out, Dims3 dout, const int *ctl)
+{
+    int ks = 1;
+    while (ks < 32 && 2 * ks <= ly.cin &&
+           ((dout.size() + 256 / ks - 1) / (256 / ks)) * (ly.cout / CT) < kConvBlocks)
+        ks *= 2;
+    switch (ks) {
+    case 1: conv_fwd_t<CT, 1>(s, ly, w, b, in, din, out, dout, ctl); break;
+    case 2: conv_fwd_t<CT, 2>(s, ly, w, b, in, din, out, dout, ctl); break;
+    case 4: conv_fwd_t<CT, 4>(s, ly, w, b, in, din, out, dout, ctl); break;
+    case 8: conv_fwd_t<CT, 8>(s, ly, w, b, in, din, out, dout, ctl); break;
+    case 16: conv_fwd_t<CT, 16>(s, ly, w, b, in, din, out, dout, ctl); break;
+    default: conv_fwd_t<CT, 32>(s, ly, w, b, in, din, out, dout, ctl); break;
+    }
+}
+
 void conv_fwd(hipStream_t s, const NetLayer &ly, const float *P, const float *in, Dims3 din, float *out, Dims3 dout,
               const int *ctl)
 {
     const float *w = P + ly.w_off, *b = P + ly.b_off;
-    const unsigned gx = (unsigned)((dout.size() + 255) / 256);
-    if (ly.cout % 8 == 0) {
-        const dim3 grid(gx, ly.cout / 8);
-        if (ly.relu)
-            k_conv_fwd<8, true><<<grid, 256, 0, s>>>(in, ly.cin, din, w, b, out, dout, ly.stride, ctl);
-        else
-            k_conv_fwd<8, false><<<grid, 256, 0, s>>>(in, ly.cin, din, w, b, out, dout, ly.stride, ctl);
-    } else {
-        const dim3 grid(gx, ly.cout);
-        if (ly.relu)
-            k_conv_fwd<1, true><<<grid, 256, 0, s>>>(in, ly.cin, din, w, b, out, dout, ly.stride, ctl);
-        else
-            k_conv_fwd<1, false><<<grid, 256, 0, s>>>(in, ly.cin, din, w, b, out, dout, ly.stride, ctl);
-    }
+    if (ly.cout % 4 == 0)
+        conv_fwd_ct<4>(s, ly, w, b, in, din, out, dout, ctl);
+    else if (ly.cout == 3)
+        conv_fwd_ct<3>(s, ly, w, b, in, din, out, dout, ctl);
+    else
+        conv_fwd_ct<1>(s, ly, w, b, in, din, out, dout, ctl);
 }
 
 // gradients of one layer: weights/bias accumulate into G; gin (if non-null)
 // is overwritten with the input gradient
-void conv_bwd(hipStream_t s, const NetLayer &ly, const float *P, float *G, const float *in, Dims3 din,
+void conv_bwd(hipStream_t s, const NetLayer &ly, const float *P, float *G, float *wpart, const float *in, Dims3 din,
               const float *out, Dims3 dout, const float *gout, float *gin)
 {
     const float *act = ly.relu ? out : nullptr;
-    k_conv_bwd_w<<<ly.cout * ly.cin, 256, 0, s>>>(gout, act, dout, in, ly.cin, din, ly.stride, G + ly.w_off,
-                                                   G + ly.b_off);
+    const int npair = ly.cout * ly.cin, nch = (int)((dout.size() + kWChunk - 1) / kWChunk);
+    k_conv_bwd_w<<<dim3(npair, nch), 256, 0, s>>>(gout, act, dout, in, ly.cin, din, ly.stride, wpart);
+    k_conv_bwd_w_sum<<<(npair * 28 + 255) / 256, 256, 0, s>>>(wpart, nch, npair, ly.cin, G + ly.w_off, G + ly.b_off);
     if (gin) {
         const unsigned gx = (unsigned)((din.size() + 255) / 256);
         if (ly.cin % 4 == 0)
-            k_conv_bwd_data<4><<<dim3(gx, ly.cin / 4), 256, 0, s>>>(gout, act, ly.cout, dout, P + ly.w_off, ly.cin,
-                                                                    din, ly.stride, gin);
+            k_conv_bwd_data<4><<<dim3(gx, ly.cin / 4), 256, (size_t)ly.cout * 27 * 4 * sizeof(float), s>>>(
+                gout, act, ly.cout, dout, P + ly.w_off, ly.cin, din, ly.stride, gin);
         else
-            k_conv_bwd_data<1><<<dim3(gx, ly.cin), 256, 0, s>>>(gout, act, ly.cout, dout, P + ly.w_off, ly.cin,
-                                                                din, ly.stride, gin);
+            k_conv_bwd_data<1><<<dim3(gx, ly.cin), 256, (size_t)ly.cout * 27 * sizeof(float), s>>>(
+                gout, act, ly.cout, dout, P + ly.w_off, ly.cin, din, ly.stride, gin);
     }
 }
 
@@ -457,6 +557,14 @@ void learn_create(gputo_handle *h)
     L->sc = ldalloc<double>(L, 8);
     L->count = ldalloc<int>(L, 4);
     L->partials = ldalloc<double>(L, 2 * kRedBlocks);
+    {
+        // weight-gradient partials: the largest (cout*cin) x chunks x 28 over the layers
+        const Dims3 dims[kNetLayers] = {L->d0, L->d1, L->d2, L->d2, L->d1, L->d0};
+        long mx = 1;
+        for (int i = 0; i < kNetLayers; ++i)
+            mx = std::max(mx, (long)L->L[i].cout * L->L[i].cin * 28 * ((dims[i].size() + kWChunk - 1) / kWChunk));
+        L->wpart = ldalloc<float>(L, mx);
+    }
     L->nsamp_max = 64;
     L->samples = ldalloc<float>(L, (size_t)L->nsamp_max * nd);
     check_hip(hipMemset(L->count, 0, 4 * sizeof(int)), "memset");
@@ -518,18 +626,18 @@ void learn_backward(gputo_handle *h)
     const Dims3 d0 = L->d0, d1 = L->d1, d2 = L->d2;
     const float *P = L->P;
     float *G = L->G;
-    conv_bwd(s, L->L[5], P, G, L->u1, d0, L->y, d0, L->gy, L->gu1);
+    conv_bwd(s, L->L[5], P, G, L->wpart, L->u1, d0, L->y, d0, L->gy, L->gu1);
     // u1 = up(a1) + e1
     k_up_bwd<<<grid256(16 * d1.size()), 256, 0, s>>>(L->gu1, d0, 16, d1, L->ga1);
-    conv_bwd(s, L->L[4], P, G, L->u2, d1, L->a1, d1, L->ga1, L->gu2);
+    conv_bwd(s, L->L[4], P, G, L->wpart, L->u2, d1, L->a1, d1, L->ga1, L->gu2);
     // u2 = up(a2) + e2
     k_up_bwd<<<grid256(32 * d2.size()), 256, 0, s>>>(L->gu2, d1, 32, d2, L->ga2);
-    conv_bwd(s, L->L[3], P, G, L->e3, d2, L->a2, d2, L->ga2, L->ge3);
-    conv_bwd(s, L->L[2], P, G, L->e2, d1, L->e3, d2, L->ge3, L->ge2);
+    conv_bwd(s, L->L[3], P, G, L->wpart, L->e3, d2, L->a2, d2, L->ga2, L->ge3);
+    conv_bwd(s, L->L[2], P, G, L->wpart, L->e2, d1, L->e3, d2, L->ge3, L->ge2);
     k_add_inplace<<<grid256(32 * d1.size()), 256, 0, s>>>(L->ge2, L->gu2, 32 * d1.size());
-    conv_bwd(s, L->L[1], P, G, L->e1, d0, L->e2, d1, L->ge2, L->ge1);
+    conv_bwd(s, L->L[1], P, G, L->wpart, L->e1, d0, L->e2, d1, L->ge2, L->ge1);
     k_add_inplace<<<grid256(16 * d0.size()), 256, 0, s>>>(L->ge1, L->gu1, 16 * d0.size());
-    conv_bwd(s, L->L[0], P, G, L->x0, d0, L->e1, d0, L->ge1, nullptr);
+    conv_bwd(s, L->L[0], P, G, L->wpart, L->x0, d0, L->e1, d0, L->ge1, nullptr);
 }
 
 // e = correction for the defect in L->d (features, forward, output scaling)

@@ -23,7 +23,6 @@ void check_hip(hipError_t e, const char *what)
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIPCHK(x) check_hip((x), #x)
 
 static_assert(kKeClasses <= kKeMaxClasses, "Ke class table larger than the kernel argument array");
 
@@ -295,12 +294,18 @@ static void ensure_tables(gputo_handle *h)
 
 // Build the hierarchy for the current Ke, CX (device) and fixed list
 // (initVcycle, mgcg1.cu:326-466).
+static bool coarse_lagged(const gputo_params &p) { return p.coarse_lag == 1; }
+
 int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
 {
     const Preset p = preset_of(h->prm);
     hipStream_t s = h->stream;
     Level &L0 = h->L[0];
     if (int rc = validate_fixed(fixed0, L0.g.ndof)) return rc;
+    // a factorisation still running on the side stream reads the coarsest
+    // matrix this setup overwrites; a finished lagged one becomes current
+    if (h->C.ev_done) HIPCHK(hipStreamWaitEvent(s, h->C.ev_done, 0));
+    coarse_adopt(h->C);
     ensure_tables(h);
     L0.fixed = fixed0;
     for (int l = 1; l < h->nl; ++l) h->L[l].fixed = restrict_fixed(h->L[l - 1].fixed, h->L[l].g);
@@ -327,17 +332,16 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
         // a smoothed coarsest level starts each solve from zero and then
         // keeps its iterate from one V-cycle to the next (mgcg9.cu:553-563)
         if (!h->C.direct) launch_fill<double>(s, vc.U, vc.g.ndof, 0.0);
-        int rc = 0;
         if (h->C.direct) {
             launch_ke_elements(s, vc.g, vc.CXl, h->Ke, vc.A);
             launch_assemble(s, vc.g, vc.A, h->C.AQ);
             launch_zero_rowcol(s, (h->C.n + 31) / 32 * 32, vc.fixed_d, (int)vc.fixed.size(), h->C.AQ);
             HIPCHK(hipGetLastError());
-            rc = coarse_factor(s, h->C);
+            coarse_factor(s, h->C, coarse_lagged(h->prm));
         }
         HIPCHK(hipGetLastError());
-        h->hier_ready = (rc == 0);
-        return rc;
+        h->hier_ready = true;
+        return 0;
     }
     h->C.direct = true;
     for (int l = 1; l < h->nl; ++l) {
@@ -355,10 +359,12 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
         launch_diagA(s, v.g, v.A, v.AD);
     }
     HIPCHK(hipGetLastError());
-    int rc = coarse_factor(s, h->C);
+    // overlaps the solve (lagged) or the first V-cycle's descent (exact);
+    // errors surface in coarse_check / coarse_adopt
+    coarse_factor(s, h->C, coarse_lagged(h->prm));
     HIPCHK(hipGetLastError());
-    h->hier_ready = (rc == 0);
-    return rc;
+    h->hier_ready = true;
+    return 0;
 }
 
 // Device-resident PCG (mgcg1.cu:235-280); U, F on the device.
@@ -421,6 +427,17 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     *relres = r.relres;
     *nit = r.done ? r.nit : maxit;
     if (maxit <= 0) *nit = 0;
+    if (int frc = coarse_check(h->C)) {  // the solve used a failed factorisation
+        h->hier_ready = false;
+        h->C.have = false;
+        *nit = frc;
+        if (h->prm.precision != GPUTO_PREC_F64) mixed_end(h, 0, 0);
+        if (h->ln) {
+            int acc = 0, rej = 0;
+            learn_end_solve(h, &acc, &rej);
+        }
+        return frc;
+    }
     if (h->prm.precision != GPUTO_PREC_F64) mixed_end(h, *nit, r.done);
     if (h->ln) {
         int acc = 0, rej = 0;
@@ -501,6 +518,9 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
     try {
         HIPCHK(hipGetDevice(&h->device));
         HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&h->C.side, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&h->C.ev_in, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&h->C.ev_done, hipEventDisableTiming));
         int maxblocks = 2 * kRedBlocks;  // two reductions share the partials (flexible beta)
         for (int l = 0; l < nl; ++l) {
             Level &v = h->L[l];
@@ -551,8 +571,12 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
         const long N = (C.n + 31) / 32 * 32;
         if (dense_ok) {
             C.AQ = dalloc<double>(h, N * N);
-            C.Linv = dalloc<double>(h, N * N);
-            C.LinvT = dalloc<double>(h, N * N);
+            for (int b = 0; b < 2; ++b) {
+                C.Lbuf[b] = dalloc<double>(h, N * N);
+                C.LTbuf[b] = dalloc<double>(h, N * N);
+            }
+            C.Linv = C.Lbuf[0];
+            C.LinvT = C.LTbuf[0];
             C.y = dalloc<double>(h, N);
         }
         C.direct = dense_ok;
@@ -579,9 +603,13 @@ int gputo_destroy(gputo_handle *h)
     learn_destroy(h);
     mixed_destroy(h);
     if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->C.side) hipStreamSynchronize(h->C.side);
     for (void *p : h->allocs) hipFree(p);
     if (h->sc_host) hipHostFree(h->sc_host);
     if (h->stream) hipStreamDestroy(h->stream);
+    if (h->C.side) hipStreamDestroy(h->C.side);
+    if (h->C.ev_in) hipEventDestroy(h->C.ev_in);
+    if (h->C.ev_done) hipEventDestroy(h->C.ev_done);
     delete h;
     return 0;
 }
@@ -594,6 +622,7 @@ int gputo_set_params(gputo_handle *h, const gputo_params *params)
         return GPUTO_ERR_ARGUMENT;  // the learned stage runs in the FP32 V-cycle
     if (!h->C.AQ && params->preset != GPUTO_PRESET_MGCG9) return GPUTO_ERR_WORKSPACE;  // no dense coarsest
     if (params->preset != h->prm.preset) h->hier_ready = false;
+    h->C.have = false;  // the next setup factors exactly
     h->prm = *params;
     try {
         if (h->prm.precision != GPUTO_PREC_F64) mixed_create(h);
@@ -622,7 +651,13 @@ int gputo_setup(gputo_handle *h, const double *Ke0, const double *CX, const doub
     try {
         std::vector<int> fx;
         if (int rc = upload_problem(h, Ke0, CX, fixed, nfd, fx)) return rc;
-        return setup_hierarchy(h, fx);
+        if (int rc = setup_hierarchy(h, fx)) return rc;
+        if (int rc = coarse_check(h->C)) {
+            h->hier_ready = false;
+            h->C.have = false;
+            return rc;
+        }
+        return 0;
     } catch (...) {
         return map_exception();
     }
@@ -718,6 +753,7 @@ static int host_op(gputo_handle *h, int mode, const double *Ke0, const double *C
         HIPCHK(hipMemcpyAsync(y, h->Q, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         h->hier_ready = false;  // the level-0 mask was replaced
+        h->C.have = false;
         return 0;
     } catch (...) {
         return map_exception();
@@ -747,6 +783,7 @@ int gputo_matvec_mixed(gputo_handle *h, int kernel, const double *Ke0, const dou
         HIPCHK(hipMemcpyAsync(y, h->Q, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         h->hier_ready = false;
+        h->C.have = false;
         return 0;
     } catch (...) {
         return map_exception();

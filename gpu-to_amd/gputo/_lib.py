@@ -49,6 +49,7 @@ class GputoParams(ctypes.Structure):
         ("l0_kernel", ctypes.c_int),
         ("nswp", ctypes.c_int),
         ("l1_kernel", ctypes.c_int),
+        ("coarse_lag", ctypes.c_int),
     ]
 
 

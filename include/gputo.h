@@ -80,6 +80,13 @@ typedef struct gputo_params {
                           where level 1 is rediscretised from the child densities,
                           stored element matrices otherwise), 1 stored element
                           matrices, 2 matrix-free (rediscretised level 1 only) */
+    int coarse_lag;    /* coarsest factorisation: 0 exact (the reference algorithm),
+                          1 lagged: each setup factors its coarsest matrix on a
+                          side stream while the solve uses the factor of the
+                          previous setup (the first setup, and any setup after a
+                          parameter change, waits for its own factor).  Lagging
+                          hides the factorisation but costs iterations when the
+                          design changes fast (early design iterations) */
 } gputo_params;
 
 /* Fill *p with the defaults (REF_C, F64, no learned stage, operator

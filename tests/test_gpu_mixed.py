@@ -71,3 +71,32 @@ def test_mixed_solve_matches_f64(gputo, oracle, prec, kernel, mesh):
     assert abs(Cm - C64) <= 1e-9 * abs(C64)
     s64.close()
     sm.close()
+
+
+def test_lagged_coarse_factor(gputo, oracle):
+    """Mixed solves of a sequence of slowly changing designs with the
+    coarsest factorisation lagged by one setup reach the FP64 solution of each
+    design to the CG tolerance, in about the iteration count of the exact
+    factorisation."""
+    nelx, nely, nelz, nl = 48, 24, 24, 4
+    Ke, F, fixed = gputo.cantilever(nelx, nely, nelz)
+    lag = gputo.Solver(nelx, nely, nelz, nl, precision=1, coarse_lag=1)
+    exact = gputo.Solver(nelx, nely, nelz, nl, precision=1)
+    s64 = gputo.Solver(nelx, nely, nelz, nl)
+    for k in range(4):
+        print("design", k, flush=True)
+        x = gputo.problems.perturbed_density(nelx, nely, nelz, 0.12 + 0.01 * k, seed=40 + k)
+        CX = 1e-9 + x ** 3 * (1 - 1e-9)
+        U64, _, n64, rc = s64.solve(Ke, F, CX, fixed, 1e-10, 300)
+        assert rc == 0
+        Ue, re, ne, rc = exact.solve(Ke, F, CX, fixed, 1e-10, 300)
+        assert rc == 0 and re < 1e-10
+        Ul, rl, nlag, rc = lag.solve(Ke, F, CX, fixed, 1e-10, 300)
+        assert rc == 0 and rl < 1e-10
+        print("iterations", n64, ne, nlag, flush=True)
+        assert rel(Ul, U64) < 1e-7 and rel(Ue, U64) < 1e-7
+        if k == 0:
+            assert np.array_equal(Ul, Ue)  # first setup: no earlier factor, exact
+        assert nlag <= ne + 4
+    for s in (lag, exact, s64):
+        s.close()
