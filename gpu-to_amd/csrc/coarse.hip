@@ -603,8 +603,31 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_band(double *A, int N, in
     for (int kb = 0; kb < nb; ++kb) {
         const int k0 = kb * NB;
         const int rows = min(N - k0 - NB, band_rows);  // multiple of NB
-        for (int q = tid; q < NB * NB; q += kCholThreads) T[q / NB][q % NB] = A[(long)(k0 + q / NB) * N + k0 + q % NB];
-        for (int q = tid; q < rows * NB; q += kCholThreads) P[q / NB][q % NB] = A[(long)(k0 + NB + q / NB) * N + k0 + q % NB];
+        {
+            // all loads of the step's tile and panel in flight before any store
+            constexpr int kT = NB * NB / kCholThreads, kP = kBandRowsMax * NB / kCholThreads;
+            double tv[kT], pv[kP];
+#pragma unroll
+            for (int u = 0; u < kT; ++u) {
+                const int q = tid + u * kCholThreads;
+                tv[u] = A[(long)(k0 + q / NB) * N + k0 + q % NB];
+            }
+#pragma unroll
+            for (int u = 0; u < kP; ++u) {
+                const int q = tid + u * kCholThreads;
+                pv[u] = A[q < rows * NB ? (long)(k0 + NB + q / NB) * N + k0 + q % NB : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < kT; ++u) {
+                const int q = tid + u * kCholThreads;
+                T[q / NB][q % NB] = tv[u];
+            }
+#pragma unroll
+            for (int u = 0; u < kP; ++u) {
+                const int q = tid + u * kCholThreads;
+                if (q < rows * NB) P[q / NB][q % NB] = pv[u];
+            }
+        }
         __syncthreads();
         if (tid < 64) {
             bad |= potrf_tile(&T[0][0], NB + 1);
@@ -637,6 +660,11 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_band(double *A, int N, in
             int ti, tj;
             tri_index(w >> 6, ti, tj);
             const int r0 = ti * NB + ((w >> 3) & 7) * 4, c0 = tj * NB + (w & 7) * 4;
+            double av[4][4];  // the trailing entries, loaded while the product runs
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) av[i][j] = A[(long)(k0 + NB + r0 + i) * N + k0 + NB + c0 + j];
             double acc[4][4];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -658,7 +686,7 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_band(double *A, int N, in
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) A[(long)(k0 + NB + r0 + i) * N + k0 + NB + c0 + j] -= acc[i][j];
+                for (int j = 0; j < 4; ++j) A[(long)(k0 + NB + r0 + i) * N + k0 + NB + c0 + j] = av[i][j] - acc[i][j];
         }
         __syncthreads();
     }

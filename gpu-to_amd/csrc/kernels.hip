@@ -64,41 +64,73 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
     // Stage the halo brick one j-row per wave (rows are contiguous in the DOF
     // vector, so the loads coalesce) and the element box the same way; the
     // row split is wave-uniform, so no per-lane integer division is needed.
+    // kStageRows rows per wave are loaded into registers before any is
+    // stored, so their loads are in flight together (addresses outside the
+    // mesh are clamped to 0 and the values masked).
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int rowlen = 3 * HJ;
-    for (int row = wave; row < HK * HI; row += kBlock / 64) {
-        const int hi = row / HK, hk = row - hi * HK;
-        const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
-        const bool rowok = gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
-        const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
-        for (int col = lane; col < rowlen; col += 64) {
-            const int hj = col / 3, c = col - 3 * hj;
-            const int gj = j0 - 1 + hj;
-            T v = T(0);
-            if (rowok && gj >= 0 && gj < g.nny) {
-                const long dof = base + col;
+    const int rowlen = 3 * HJ;  // <= 128: two values per lane
+    const int nrows = HK * HI;
+    for (int r0 = wave; r0 < nrows; r0 += (kBlock / 64) * kStageRows) {
+        T v[kStageRows][2];
+#pragma unroll
+        for (int u = 0; u < kStageRows; ++u) {
+            const int row = r0 + (kBlock / 64) * u;
+            const int hi = row / HK, hk = row - hi * HK;
+            const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
+            const bool rowok = row < nrows && gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
+            const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const int col = lane + 64 * cc;
+                const int gj = j0 - 1 + col / 3;
+                const bool ok = rowok && col < rowlen && gj >= 0 && gj < g.nny;
+                const long dof = ok ? base + col : 0;
+                T x;
                 if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
-                    v = T(a.prerelax) * a.F[dof] / a.AD[dof];
+                    x = T(a.prerelax) * a.F[dof] / a.AD[dof];
                 else
-                    v = a.U[dof];
+                    x = a.U[dof];
+                v[u][cc] = ok ? x : T(0);
             }
-            su[c][row * HJ + hj] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < kStageRows; ++u) {
+            const int row = r0 + (kBlock / 64) * u;
+            if (row >= nrows) break;
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const int col = lane + 64 * cc;
+                if (col < rowlen) {
+                    const int hj = col / 3, c = col - 3 * hj;
+                    su[c][row * HJ + hj] = v[u][cc];
+                }
+            }
         }
     }
     const int EJ = t.TJ + 1, EK = t.TK + 1;
     {
         const int EI = t.TI + 1;
-        for (int row = wave; row < EK * EI; row += kBlock / 64) {
-            const int ei = row / EK, ek = row - ei * EK;
-            const int gi = i0 - 1 + ei, gk = k0 - 1 + ek;
-            const bool rowok = gi >= 0 && gi < g.nx && gk >= 0 && gk < g.nz;
-            for (int ej = lane; ej < EJ; ej += 64) {
-                const int gj = j0 - 1 + ej;
-                T v = T(0);
-                if (rowok && gj >= 0 && gj < g.ny) v = a.CX[(long)g.ny * g.nz * gi + (long)g.ny * gk + gj];
-                if (a.cx_floor > 0.0 && v < T(a.cx_floor)) v = T(0);  // mgcg9 defect0c skip
-                sE[row * EJ + ej] = v;
+        const int erows = EK * EI;
+        for (int r0 = wave; r0 < erows; r0 += (kBlock / 64) * kStageRows) {
+            T v[kStageRows];
+#pragma unroll
+            for (int u = 0; u < kStageRows; ++u) {
+                const int row = r0 + (kBlock / 64) * u;
+                const int ei = row / EK, ek = row - ei * EK;
+                const int gi = i0 - 1 + ei, gk = k0 - 1 + ek, gj = j0 - 1 + lane;
+                const bool ok = row < erows && lane < EJ && gi >= 0 && gi < g.nx && gk >= 0 && gk < g.nz && gj >= 0 &&
+                                gj < g.ny;
+                const T x = a.CX[ok ? (long)g.ny * g.nz * gi + (long)g.ny * gk + gj : 0];
+                T e = ok ? x : T(0);
+                if (a.cx_floor > 0.0 && e < T(a.cx_floor)) e = T(0);  // mgcg9 defect0c skip
+                v[u] = e;
+            }
+#pragma unroll
+            for (int u = 0; u < kStageRows; ++u) {
+                const int row = r0 + (kBlock / 64) * u;
+                if (row >= erows) break;
+                if (lane < EJ) sE[row * EJ + lane] = v[u];
             }
         }
     }
@@ -210,6 +242,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
             const T y[3] = {y0, y1, y2};
             const int hc = hb + (HK + 1) * HJ + 1;  // the node itself
             const T om = T(a.omega), one = T(1);
+            // all epilogue operands loaded before use (one round trip)
+            T fv[3] = {T(0), T(0), T(0)}, adv[3] = {T(1), T(1), T(1)};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (MODE != OP_MATVEC) fv[c] = a.F[dof + c];
+                if (MODE == OP_JACOBI || MODE == OP_PRE_JACOBI) adv[c] = a.AD[dof + c];
+            }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const bool fixed = (m >> c) & 1u;
@@ -218,14 +257,14 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
                     a.out[dof + c] = v;
                     part += (double)su[c][hc] * (double)v;
                 } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
-                    const T rv = fixed ? T(0) : a.F[dof + c] - y[c];
+                    const T rv = fixed ? T(0) : fv[c] - y[c];
                     a.out[dof + c] = rv;
                     if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = su[c][hc];
                 } else {
-                    const T f = a.F[dof + c];
+                    const T f = fv[c];
                     const T rv = fixed ? T(0) : f - y[c];
                     const T u = su[c][hc];
-                    const T ad = a.AD[dof + c];
+                    const T ad = adv[c];
                     // U = (1-w) U + w (R + AD U)/AD  (mgcg1.cu:1199-1206)
                     const T un = (one - om) * u + om * (rv + ad * u) / ad;
                     a.out[dof + c] = un;

@@ -57,26 +57,44 @@ __global__ __launch_bounds__(256) void k_l1_mf(OpArgsT<T> a, Tile t, const T *__
     const int EJ = t.TJ + 1, EK = t.TK + 1, EI = t.TI + 1;
     const int nown = t.TJ * t.TK * t.TI;
 
-    // halo brick, one j-row per wave (coalesced rows, wave-uniform row split)
+    // halo brick, one j-row per wave (coalesced rows, wave-uniform row split);
+    // kStageRows rows per wave are loaded before any is stored, so their loads
+    // are in flight together (addresses outside the mesh clamped and masked)
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
-    const int rowlen = 3 * HJ;
-    for (int row = wv; row < HK * HI; row += 4) {
-        const int hi = row / HK, hk = row - hi * HK;
-        const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
-        const bool rowok = gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
-        const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
-        for (int col = ln; col < rowlen; col += 64) {
-            const int hj = col / 3;
-            const int gj = j0 - 1 + hj;
-            T v = T(0);
-            if (rowok && gj >= 0 && gj < g.nny) {
-                const long dof = base + col;
+    const int rowlen = 3 * HJ;  // <= 128
+    const int nrows = HK * HI;
+    for (int r0 = wv; r0 < nrows; r0 += 4 * kStageRows) {
+        T v[kStageRows][2];
+#pragma unroll
+        for (int u = 0; u < kStageRows; ++u) {
+            const int row = r0 + 4 * u;
+            const int hi = row / HK, hk = row - hi * HK;
+            const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
+            const bool rowok = row < nrows && gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
+            const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const int col = ln + 64 * cc;
+                const int gj = j0 - 1 + col / 3;
+                const bool ok = rowok && col < rowlen && gj >= 0 && gj < g.nny;
+                const long dof = ok ? base + col : 0;
+                T x;
                 if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
-                    v = T(a.prerelax) * a.F[dof] / a.AD[dof];
+                    x = T(a.prerelax) * a.F[dof] / a.AD[dof];
                 else
-                    v = a.U[dof];
+                    x = a.U[dof];
+                v[u][cc] = ok ? x : T(0);
             }
-            su[3 * row * HJ + col] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < kStageRows; ++u) {
+            const int row = r0 + 4 * u;
+            if (row >= nrows) break;
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const int col = ln + 64 * cc;
+                if (col < rowlen) su[3 * row * HJ + col] = v[u][cc];
+            }
         }
     }
     for (int q = threadIdx.x; q < 3 * nown; q += 256) acc[q] = T(0);
@@ -98,23 +116,34 @@ __global__ __launch_bounds__(256) void k_l1_mf(OpArgsT<T> a, Tile t, const T *__
         const int nj = (EJ - pj + 1) / 2, nk = (EK - pk + 1) / 2, ni = (EI - pi + 1) / 2;
         const int cnt = nj * nk * ni;
         const float inv_nj = 1.0f / (float)nj, inv_nk = 1.0f / (float)nk;
+        // element of batch q0 for this lane group, and its Gauss point's
+        // fine-element scale (loaded one batch ahead)
+        auto elem = [&](int q, int &ej, int &ek, int &ei, long &cxi) {
+            ej = ek = ei = 0;
+            cxi = 0;
+            if (q >= cnt) return false;
+            const int r = udiv_small(q, nj, inv_nj);
+            ej = pj + 2 * (q - r * nj);
+            const int r2 = udiv_small(r, nk, inv_nk);
+            ek = pk + 2 * (r - r2 * nk);
+            ei = pi + 2 * r2;
+            const int ex = i0 - 1 + ei, ey = j0 - 1 + ej, ez = k0 - 1 + ek;
+            if (!(ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz)) return false;
+            cxi = (long)fny * fnz * (2 * ex + is) + (long)fny * (2 * ez + it) + 2 * ey + ie;
+            return true;
+        };
+        int nej, nek, nei;
+        long ncxi;
+        bool nvalid = elem(grp, nej, nek, nei, ncxi);
+        T ncx = CXf[ncxi];
         for (int q0 = 0; q0 < cnt; q0 += 32) {
-            const int q = q0 + grp;
-            int ej = 0, ek = 0, ei = 0;
-            bool valid = q < cnt;
-            if (valid) {
-                const int r = udiv_small(q, nj, inv_nj);
-                ej = pj + 2 * (q - r * nj);
-                const int r2 = udiv_small(r, nk, inv_nk);
-                ek = pk + 2 * (r - r2 * nk);
-                ei = pi + 2 * r2;
-                const int ex = i0 - 1 + ei, ey = j0 - 1 + ej, ez = k0 - 1 + ek;
-                valid = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz;
-            }
+            const int ej = nej, ek = nek, ei = nei;
+            const bool valid = nvalid;
+            const T cx = ncx;
+            nvalid = elem(q0 + 32 + grp, nej, nek, nei, ncxi);
+            ncx = CXf[ncxi];
             T y[24];
             if (valid) {
-                const int ex = i0 - 1 + ei, ey = j0 - 1 + ej, ez = k0 - 1 + ek;
-                const T cx = CXf[(long)fny * fnz * (2 * ex + is) + (long)fny * (2 * ez + it) + 2 * ey + ie];
                 // nodal values u[x][y][z][c] of the element
                 T u[2][2][2][3];
 #pragma unroll

@@ -28,12 +28,34 @@ namespace gputo {
 
 namespace {
 
-__device__ __forceinline__ void pos3(long p, const Dims3 &d, int &i, int &k, int &j)
+// position p = (i*n1 + k)*n2 + j of a grid (all sizes here are < 2^31:
+// 32-bit divisions)
+__device__ __forceinline__ void pos3(int p, const Dims3 &d, int &i, int &k, int &j)
 {
-    j = (int)(p % d.n2);
-    const long r = p / d.n2;
-    k = (int)(r % d.n1);
-    i = (int)(r / d.n1);
+    const int r = p / d.n2;
+    j = p - r * d.n2;
+    i = r / d.n1;
+    k = r - i * d.n1;
+}
+
+// Copy n floats, element q from src[idx(q)], into LDS with 8 loads in flight
+// per thread (indices clamped to 0 past the end, those values dropped).
+template <class Idx>
+__device__ __forceinline__ void stage_lds(float *__restrict__ dst, int n, const float *__restrict__ src, Idx idx)
+{
+    for (int q0 = 0; q0 < n; q0 += 8 * 256) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u * 256 + (int)threadIdx.x;
+            v[u] = src[q < n ? idx(q) : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u * 256 + (int)threadIdx.x;
+            if (q < n) dst[q] = v[u];
+        }
+    }
 }
 
 // out[co][p] = act(b[co] + sum_ci sum_t w[co][ci][t] in[ci][p*s + t - 1]).
@@ -52,14 +74,14 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float *__restrict__ in, 
     extern __shared__ float wl[];  // [cin*27][CT], then KS partials [KS][256/KS][CT]
     constexpr int NP = 256 / KS;
     const int co0 = blockIdx.y * CT;
-    for (int q = threadIdx.x; q < cin * 27 * CT; q += 256) {
+    stage_lds(wl, cin * 27 * CT, w, [=](int q) {
         const int c = q % CT, k = q / CT;  // k = ci*27 + t
-        wl[q] = w[(long)(co0 + c) * cin * 27 + k];
-    }
+        return (co0 + c) * cin * 27 + k;
+    });
     __syncthreads();
     const int pl = threadIdx.x % NP, ks = threadIdx.x / NP;
-    const long p = (long)blockIdx.x * NP + pl;
-    const long nout = dout.size(), nin = din.size();
+    const int p = blockIdx.x * NP + pl;
+    const int nout = (int)dout.size(), nin = (int)din.size();
     float acc[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[c] = 0.0f;
@@ -80,13 +102,18 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float *__restrict__ in, 
             for (int c = 0; c < CT; ++c) acc[c] = b[co0 + c];
         }
         for (int ci = ks; ci < cin; ci += KS) {
-            const float *src = in + (long)ci * nin;
+            const float *src = in + ci * nin;
             const float *wc = wl + ci * 27 * CT;
+            // unconditional loads (off = 0 outside the grid), masked after:
+            // all 27 loads of a channel are in flight together
+            float v[27];
+#pragma unroll
+            for (int t = 0; t < 27; ++t) v[t] = src[off[t]];
 #pragma unroll
             for (int t = 0; t < 27; ++t) {
-                const float v = ((valid >> t) & 1u) ? src[off[t]] : 0.0f;
+                const float vt = ((valid >> t) & 1u) ? v[t] : 0.0f;
 #pragma unroll
-                for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], v, acc[c]);
+                for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], vt, acc[c]);
             }
         }
     }
@@ -121,14 +148,14 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const float *__restrict__
 {
     extern __shared__ float wl[];  // [cout*27][CT]
     const int ci0 = blockIdx.y * CT;
-    for (int q = threadIdx.x; q < cout * 27 * CT; q += 256) {
+    stage_lds(wl, cout * 27 * CT, w, [=](int q) {
         const int c = q % CT, k = q / CT;  // k = co*27 + t
         const int co = k / 27, t = k - 27 * co;
-        wl[q] = w[((long)co * cin + ci0 + c) * 27 + t];
-    }
+        return (co * cin + ci0 + c) * 27 + t;
+    });
     __syncthreads();
-    const long q = (long)blockIdx.x * 256 + threadIdx.x;
-    const long nin = din.size(), nout = dout.size();
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    const int nin = (int)din.size(), nout = (int)dout.size();
     if (q >= nin) return;
     int qi, qk, qj;
     pos3(q, din, qi, qk, qj);
@@ -153,22 +180,24 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(const float *__restrict__
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[c] = 0.0f;
     for (int co = 0; co < cout; ++co) {
-        const float *g = gout + (long)co * nout;
-        const float *a = act ? act + (long)co * nout : nullptr;
+        const float *g = gout + co * nout;
+        const float *a = act ? act + co * nout : nullptr;
         const float *wc = wl + co * 27 * CT;
+        float gv[27], av[27];
 #pragma unroll
         for (int t = 0; t < 27; ++t) {
-            float gz = 0.0f;
-            if ((valid >> t) & 1u) {
-                gz = g[off[t]];
-                if (a && !(a[off[t]] > 0.0f)) gz = 0.0f;
-            }
+            gv[t] = g[off[t]];  // off = 0 outside the grid: masked below
+            av[t] = a ? a[off[t]] : 1.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const float gz = (((valid >> t) & 1u) && av[t] > 0.0f) ? gv[t] : 0.0f;
 #pragma unroll
             for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], gz, acc[c]);
         }
     }
 #pragma unroll
-    for (int c = 0; c < CT; ++c) gin[(long)(ci0 + c) * nin + q] = acc[c];
+    for (int c = 0; c < CT; ++c) gin[(ci0 + c) * nin + q] = acc[c];
 }
 
 // Weight gradient, split over chunks of kWChunk output positions:
@@ -183,27 +212,31 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(const float *__restrict__ go
 {
     __shared__ float red[28][257];
     const int pair = blockIdx.x, co = pair / cin, ci = pair % cin;
-    const long nout = dout.size(), nin = din.size();
-    const float *g = gout + (long)co * nout;
-    const float *a = act ? act + (long)co * nout : nullptr;
-    const float *src = in + (long)ci * nin;
+    const int nout = (int)dout.size(), nin = (int)din.size();
+    const float *g = gout + co * nout;
+    const float *a = act ? act + co * nout : nullptr;
+    const float *src = in + ci * nin;
     float acc[28];
 #pragma unroll
     for (int t = 0; t < 28; ++t) acc[t] = 0.0f;
-    const long p1 = min(nout, (long)(blockIdx.y + 1) * kWChunk);
-    for (long p = (long)blockIdx.y * kWChunk + threadIdx.x; p < p1; p += 256) {
+    const int p1 = min(nout, (int)(blockIdx.y + 1) * kWChunk);
+    for (int p = blockIdx.y * kWChunk + threadIdx.x; p < p1; p += 256) {
         float gz = g[p];
         if (a && !(a[p] > 0.0f)) gz = 0.0f;
         if (gz == 0.0f) continue;
         int oi, ok, oj;
         pos3(p, dout, oi, ok, oj);
         acc[27] += gz;
+        float v[27];
 #pragma unroll
         for (int t = 0; t < 27; ++t) {
             const int ii = oi * stride + t / 9 - 1, ik = ok * stride + (t / 3) % 3 - 1, ij = oj * stride + t % 3 - 1;
-            if (ii >= 0 && ii < din.n0 && ik >= 0 && ik < din.n1 && ij >= 0 && ij < din.n2)
-                acc[t] = fmaf(gz, src[((long)ii * din.n1 + ik) * din.n2 + ij], acc[t]);
+            const bool in_grid = ii >= 0 && ii < din.n0 && ik >= 0 && ik < din.n1 && ij >= 0 && ij < din.n2;
+            v[t] = src[in_grid ? (ii * din.n1 + ik) * din.n2 + ij : 0];
+            v[t] = in_grid ? v[t] : 0.0f;
         }
+#pragma unroll
+        for (int t = 0; t < 27; ++t) acc[t] = fmaf(gz, v[t], acc[t]);
     }
 #pragma unroll
     for (int t = 0; t < 28; ++t) red[t][threadIdx.x] = acc[t];
@@ -235,23 +268,23 @@ __global__ void k_up_add(const float *__restrict__ dcoarse, Dims3 dc, const floa
                          int C, float *__restrict__ out, const int *ctl)
 {
     if (ctl && ctl[0]) return;
-    const long n = df.size(), total = n * C;
-    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < total; q += (long)gridDim.x * 256) {
-        const int c = (int)(q / n);
-        const long p = q % n;
+    const int n = (int)df.size(), total = n * C, ncs = (int)dc.size();
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < total; q += gridDim.x * 256) {
+        const int c = q / n;
+        const int p = q - c * n;
         int i, k, j;
         pos3(p, df, i, k, j);
-        out[q] = dcoarse[(long)c * dc.size() + ((long)(i >> 1) * dc.n1 + (k >> 1)) * dc.n2 + (j >> 1)] + efine[q];
+        out[q] = dcoarse[c * ncs + ((i >> 1) * dc.n1 + (k >> 1)) * dc.n2 + (j >> 1)] + efine[q];
     }
 }
 
 // gd[c][pc] = sum over the (up to 8) fine children of pc of gu[c][p]
 __global__ void k_up_bwd(const float *__restrict__ gu, Dims3 df, int C, Dims3 dc, float *__restrict__ gd)
 {
-    const long nc = dc.size(), total = nc * C;
-    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < total; q += (long)gridDim.x * 256) {
-        const int c = (int)(q / nc);
-        const long pc = q % nc;
+    const int nc = (int)dc.size(), total = nc * C, nf = (int)df.size();
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < total; q += gridDim.x * 256) {
+        const int c = q / nc;
+        const int pc = q - c * nc;
         int i, k, j;
         pos3(pc, dc, i, k, j);
         float s = 0.0f;
@@ -259,8 +292,7 @@ __global__ void k_up_bwd(const float *__restrict__ gu, Dims3 df, int C, Dims3 dc
             for (int b = 0; b < 2; ++b)
                 for (int e = 0; e < 2; ++e) {
                     const int fi = 2 * i + a, fk = 2 * k + b, fj = 2 * j + e;
-                    if (fi < df.n0 && fk < df.n1 && fj < df.n2)
-                        s += gu[(long)c * df.size() + ((long)fi * df.n1 + fk) * df.n2 + fj];
+                    if (fi < df.n0 && fk < df.n1 && fj < df.n2) s += gu[c * nf + (fi * df.n1 + fk) * df.n2 + fj];
                 }
         gd[q] = s;
     }

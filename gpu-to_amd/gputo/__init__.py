@@ -12,9 +12,10 @@ from ._lib import (GputoError, PREC_F64, PREC_MIXED_BF16, PREC_MIXED_F32, PRESET
 from .mgcg import Solver, mgcg1, mgcg9, release_handles
 from .problems import cantilever, fixed_x0_face, ke0_h8, uniform_edge_load, matlab_edge_load
 from .topopt import TopOpt
+from . import log
 
 __all__ = [
     "GputoError", "PREC_F64", "PREC_MIXED_F32", "PREC_MIXED_BF16", "PRESET_REF_C", "PRESET_MATLAB_CFG0", "PRESET_MGCG9", "load",
     "Solver", "mgcg1", "mgcg9", "release_handles", "cantilever", "fixed_x0_face", "ke0_h8", "uniform_edge_load",
-    "matlab_edge_load", "TopOpt",
+    "matlab_edge_load", "TopOpt", "log",
 ]

@@ -37,3 +37,24 @@ def test_design_loop_matches_oracle(gputo, oracle, ft):
     assert np.abs(x - st.x).max() <= 1e-3
     run.close()
     s.close()
+
+
+def test_design_loop_log(gputo, tmp_path):
+    """The loop writes the reference driver's log: header, then one line per
+    design iteration carrying that iteration's compliance and CG figures."""
+    nelx, nely, nelz, nl = 16, 8, 8, 3
+    Ke, F, fixed = gputo.cantilever(nelx, nely, nelz)
+    x0 = np.full(nelx * nely * nelz, 0.12)
+    s = gputo.Solver(nelx, nely, nelz, nl)
+    path = str(tmp_path / gputo.log.filename(nl, nelx, nely, nelz))
+    run = gputo.TopOpt(s, Ke, x0, F, fixed, log=path, maxit=3, cg_tol=1e-10, cg_maxit=300)
+    hist = [run.iterate() for _ in range(3)]
+    run.close()
+    s.close()
+    text = open(path).read()
+    assert text.startswith("Top3D125mod \n\nnl: 3, nelx:  16, nely:   8, nelz:   8, volfrac:0.12, penal: 3,")
+    lines = [l for l in text.splitlines() if l.startswith("It.:")]
+    assert len(lines) == 3
+    for k, (l, h) in enumerate(zip(lines, hist)):
+        assert l.startswith("It.:%5i C:%6.5e Time:" % (k + 1, h["compliance"]))
+        assert "cgiters: %3d MemUsed:" % h["cg_iters"] in l
