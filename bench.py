@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--nl", type=int, default=5)
     ap.add_argument("--cg-tol", type=float, default=1e-8)
     ap.add_argument("--cg-maxit", type=int, default=300)
-    ap.add_argument("--variants", default="f64,mixed_gather,mixed_gather+learned")
+    ap.add_argument("--variants", default="f64,mixed_mfma3,mixed_mfma3+learned")
     ap.add_argument("--op-reps", type=int, default=50)
     ap.add_argument("--learn-steps", type=int, default=300)
     ap.add_argument("--learn-lr", type=float, default=1e-3)

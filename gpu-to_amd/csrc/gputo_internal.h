@@ -18,7 +18,7 @@ namespace gputo {
 constexpr int kMaxHalo = 864;
 constexpr int kMaxElemBox = 640;
 constexpr int kBlock = 256;          // threads per block of the gather kernels
-constexpr int kStageRows = 8;        // halo rows per wave loaded before any is stored
+constexpr int kStageRows = 4;        // halo rows per wave loaded before any is stored
 constexpr int kStNodes = kBlock / 8; // nodes per block of the stored-matrix operator
 constexpr int kRedBlocks = 1024;     // blocks of the two-stage reductions
 constexpr int kMaxLevels = 12;

@@ -40,7 +40,7 @@ __device__ __forceinline__ T lane_w(int x, T xi)
 }
 
 template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_l1_mf(OpArgsT<T> a, Tile t, const T *__restrict__ CXf, T lam2mu, T lam,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_l1_mf(OpArgsT<T> a, Tile t, const T *__restrict__ CXf, T lam2mu, T lam,
                                               T mu)
 {
     if (a.ctl && a.ctl[0]) return;
@@ -132,16 +132,19 @@ __global__ __launch_bounds__(256) void k_l1_mf(OpArgsT<T> a, Tile t, const T *__
             cxi = (long)fny * fnz * (2 * ex + is) + (long)fny * (2 * ez + it) + 2 * ey + ie;
             return true;
         };
+        // FP32: the scale is loaded one batch ahead; FP64 keeps its registers
+        // for the element arithmetic (two waves per SIMD otherwise)
+        constexpr bool kAhead = sizeof(T) == 4;
         int nej, nek, nei;
         long ncxi;
         bool nvalid = elem(grp, nej, nek, nei, ncxi);
-        T ncx = CXf[ncxi];
+        T ncx = kAhead ? CXf[ncxi] : T(0);
         for (int q0 = 0; q0 < cnt; q0 += 32) {
             const int ej = nej, ek = nek, ei = nei;
             const bool valid = nvalid;
-            const T cx = ncx;
+            const T cx = kAhead ? ncx : CXf[ncxi];
             nvalid = elem(q0 + 32 + grp, nej, nek, nei, ncxi);
-            ncx = CXf[ncxi];
+            if (kAhead) ncx = CXf[ncxi];
             T y[24];
             if (valid) {
                 // nodal values u[x][y][z][c] of the element

@@ -336,8 +336,14 @@ __global__ __launch_bounds__(256, 3) void k_mfma_op(OpArgsT<float> a, Tile t)
     MfBatch B;
     mf_batches(g, m, j0, k0, B);
 
-    // prologue: planes i0-1, i0, i0+1 into slots 0, 1, 2; element layer i0-1
+    // prologue: planes i0-1, i0, i0+1 into slots 0, 1, 2; element layer i0-1.
+    // Software pipeline of the layer loop: plane q = tt + 3 is loaded two
+    // layers ahead (at layer tt - 1) and stored at the end of layer tt; the
+    // epilogue operands of node layer tt and the element scales of element
+    // layer tt are loaded one layer ahead.
     float En[kMfBatchesPerWave];
+    float pv[2][3];  // plane loaded one layer earlier, stored at the end of this layer
+    MfNodeIn in;
     {
         float E0[kMfBatchesPerWave];
         mf_load_E(a, i0 - 1, B, E0);
@@ -348,6 +354,8 @@ __global__ __launch_bounds__(256, 3) void k_mfma_op(OpArgsT<float> a, Tile t)
             mf_plane_store<NP>(su + q * NP * m.NPL, m, vals);
         }
         mf_load_E(a, i0, B, En);
+        mf_plane_load<MODE>(a, m, i0 + 2, j0, k0, pv);
+        mf_load_node<MODE>(a, t, i0, j0, k0, in);
         __syncthreads();
         mf_layer<FL>(m, su, su + NP * m.NPL, T, Ka, B, E0);
     }
@@ -356,12 +364,11 @@ __global__ __launch_bounds__(256, 3) void k_mfma_op(OpArgsT<float> a, Tile t)
     const float om = (float)a.omega;
     for (int tt = 0; tt < nlay; ++tt) {
         const int gi = i0 + tt;
-        // loads for later phases first: the next plane, this node layer's
-        // epilogue operands, the next element layer's scales
-        float vals[2][3];
-        mf_plane_load<MODE>(a, m, gi + 2, j0, k0, vals);  // plane slot (tt+3)%3, written below
-        MfNodeIn in;
-        mf_load_node<MODE>(a, t, gi, j0, k0, in);
+        // loads for the next layer first
+        float pn[2][3];
+        mf_plane_load<MODE>(a, m, gi + 3, j0, k0, pn);
+        MfNodeIn inn;
+        mf_load_node<MODE>(a, t, gi + 1, j0, k0, inn);
         float Enn[kMfBatchesPerWave];
         mf_load_E(a, gi + 1, B, Enn);
         mf_layer<FL>(m, su + ((tt + 1) % 3) * NP * m.NPL, su + ((tt + 2) % 3) * NP * m.NPL,
@@ -410,9 +417,14 @@ __global__ __launch_bounds__(256, 3) void k_mfma_op(OpArgsT<float> a, Tile t)
                 }
             }
         }
-        mf_plane_store<NP>(su + (tt % 3) * NP * m.NPL, m, vals);
+        mf_plane_store<NP>(su + (tt % 3) * NP * m.NPL, m, pv);  // plane gi + 2
 #pragma unroll
         for (int kb = 0; kb < kMfBatchesPerWave; ++kb) En[kb] = Enn[kb];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pv[r][c] = pn[r][c];
+        in = inn;
         __syncthreads();
     }
     if (a.partials) {
