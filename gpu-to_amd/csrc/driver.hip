@@ -12,13 +12,6 @@
 #include "gputo_internal.h"
 #include "mixed.h"
 
-namespace gputo {
-
-int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0);
-int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit);
-
-}  // namespace gputo
-
 using namespace gputo;
 
 struct ToState {

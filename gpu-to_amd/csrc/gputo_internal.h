@@ -167,6 +167,7 @@ struct gputo_handle {
     int gt_max = 0;
     int gt_preset = -1;          // preset the tables were built for (-1: none yet)
     std::vector<double> gt_ke;  // Ke they were built from (Galerkin level 1)
+    double *hom = nullptr;      // periodic homogenisation: 6 load vectors and 6 fluctuations
 };
 
 namespace gputo {
@@ -249,5 +250,37 @@ __device__ __forceinline__ double block_sum(double v)
 
 
 void to_release(gputo_handle *h);  // frees the design-loop state (driver.hip)
+
+// ---- periodic unit cell (periodic.hip) --------------------------------
+// A periodic vector lives on the full node grid; the nodes with i = nx,
+// j = ny or k = nz are images of the master node (i mod nx, j mod ny,
+// k mod nz).  Primal vectors (iterates) hold equal values on a master and
+// its images; dual vectors (right-hand sides, residuals) are kept with the
+// whole value on the master and zeros on the images, so that dot products
+// over the full grid count every periodic DOF once.  The non-periodic
+// operators applied to a primal vector give each master part of its row
+// and the rest on its images: summing the images into the master (fold)
+// completes the periodic operator.
+// fold: master += images; images = bcast ? master : 0
+template <class T> void launch_per_fold(hipStream_t s, const Grid &g, T *v, int bcast, const int *ctl);
+// images = master
+template <class T> void launch_per_bcast(hipStream_t s, const Grid &g, T *v, const int *ctl);
+// pre: U = prerelax F/AD (0 on fixed DOFs); else U += omega fold(R)/AD;
+// the new master value is written to the images too
+template <class T>
+void launch_per_jacobi(hipStream_t s, const Grid &g, T *U, const T *R, const T *F, const T *AD, const uint8_t *mask,
+                       double omega, double prerelax, int pre, const int *ctl);
+// dense coarsest matrix of the periodic operator in the full numbering:
+// master rows and columns hold the wrapped assembly, image rows are unit
+void launch_assemble_per(hipStream_t s, const Grid &g, const double *A, double *AQ);
+
+// unit-strain loads: fe[q] = Ke0 u0_q on the unit element, Q0[q][p] = u0_q.Ke0 u0_p
+void homog_loads(const double *Ke0, double fe[6][24], double Q0[6][6]);
+// the six load vectors (6 x ndof, master form) of the element scales CX
+void launch_homog_rhs(hipStream_t s, const Grid &g, const double *CX, const double fe[6][24], const uint8_t *mask,
+                      double *F);
+
+int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0);
+int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit);
 
 }  // namespace gputo

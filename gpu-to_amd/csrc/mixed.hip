@@ -17,8 +17,6 @@
 
 namespace gputo {
 
-int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0);
-
 template <class T>
 static T *malloc_dev(MixedState *m, size_t n)
 {
@@ -107,6 +105,7 @@ static void mixed_prepare(gputo_handle *h)
     launch_copy<double, float>(s, h->CX, m->CX32, g0.nel, nullptr);
     m->H.l1_mf = use_l1_mf(h->prm, h->nl);
     m->H.mg9 = preset_of(h->prm).mg9;
+    m->H.per = h->prm.periodic != 0;
     if (m->H.mg9 && !h->C.direct) launch_fill<float>(s, m->H.L[h->nl - 1].U, h->L[h->nl - 1].g.ndof, 0.0);
     for (int l = 0; l < h->nl; ++l) {
         const Level &v = h->L[l];

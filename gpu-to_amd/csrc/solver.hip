@@ -229,6 +229,7 @@ HierView<double> f64_view(const gputo_handle *h)
     H.CX = h->CX;
     H.l1_mf = use_l1_mf(h->prm, h->nl);
     H.mg9 = preset_of(h->prm).mg9;
+    H.per = h->prm.periodic != 0;
     return H;
 }
 
@@ -317,7 +318,9 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
             HIPCHK(hipMemcpyAsync(v.fixed_d, v.fixed.data(), v.fixed.size() * sizeof(int), hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));  // host vectors above are temporaries
     }
+    const bool per = h->prm.periodic != 0;
     launch_diag0(s, L0.g, h->Ke, h->CX, L0.AD);
+    if (per) launch_per_fold<double>(s, L0.g, L0.AD, 1, nullptr);
     if (p.mg9) {
         // mgcg9 (initVcycle, mgcg9.cu:342-414): coarse element scales and
         // diagonals from the unit Ke; the coarsest matrix, when factored,
@@ -326,6 +329,7 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
             Level &v = h->L[l];
             launch_coarse_cx(s, v.g, l == 1 ? h->CX : h->L[l - 1].CXl, v.CXl);
             launch_diag0(s, v.g, h->Ke, v.CXl, v.AD);
+            if (per) launch_per_fold<double>(s, v.g, v.AD, 1, nullptr);
         }
         Level &vc = h->L[h->nl - 1];
         h->C.direct = vc.g.ndof < kMaxCoarseDofs;
@@ -334,7 +338,10 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
         if (!h->C.direct) launch_fill<double>(s, vc.U, vc.g.ndof, 0.0);
         if (h->C.direct) {
             launch_ke_elements(s, vc.g, vc.CXl, h->Ke, vc.A);
-            launch_assemble(s, vc.g, vc.A, h->C.AQ);
+            if (per)
+                launch_assemble_per(s, vc.g, vc.A, h->C.AQ);
+            else
+                launch_assemble(s, vc.g, vc.A, h->C.AQ);
             launch_zero_rowcol(s, (h->C.n + 31) / 32 * 32, vc.fixed_d, (int)vc.fixed.size(), h->C.AQ);
             HIPCHK(hipGetLastError());
             coarse_factor(s, h->C, coarse_lagged(h->prm));
@@ -353,10 +360,17 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
         else
             launch_coarse_galerkin(s, v.g, l == 1 ? nullptr : h->L[l - 1].A, h->Ke, h->CX, v.A);
         if (l == h->nl - 1) {
-            launch_corrcsa(s, v.g, v.fixed_d, (int)v.fixed.size(), v.A, p.corr_mirrored);
-            launch_assemble(s, v.g, v.A, h->C.AQ);
+            if (per) {
+                // the wrapped matrix with the fixed rows and columns decoupled
+                launch_assemble_per(s, v.g, v.A, h->C.AQ);
+                launch_zero_rowcol(s, (h->C.n + 31) / 32 * 32, v.fixed_d, (int)v.fixed.size(), h->C.AQ);
+            } else {
+                launch_corrcsa(s, v.g, v.fixed_d, (int)v.fixed.size(), v.A, p.corr_mirrored);
+                launch_assemble(s, v.g, v.A, h->C.AQ);
+            }
         }
         launch_diagA(s, v.g, v.A, v.AD);
+        if (per) launch_per_fold<double>(s, v.g, v.AD, 1, nullptr);
     }
     HIPCHK(hipGetLastError());
     // overlaps the solve (lagged) or the first V-cycle's descent (exact);
@@ -388,9 +402,11 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
     const int nmv = (int)xcd_launch(L0.t.blocks());  // partials of the level-0 matvec
+    const bool per = h->prm.periodic != 0;
     launch_fill(s, U, n, 0.0);
     launch_copy(s, F, h->R, n, nullptr);
-    launch_dot_partial(s, F, F, n, h->partials, nullptr);
+    if (per) launch_per_fold<double>(s, L0.g, h->R, 0, nullptr);  // images' share to the masters
+    launch_dot_partial(s, h->R, h->R, n, h->partials, nullptr);
     launch_cg_init(s, h->sc, h->partials, kRedBlocks);
     precond(h, h->R, h->P);
     launch_copy(s, h->P, h->Z, n, ctl);
@@ -406,6 +422,9 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
             a.out = h->Q;
             a.partials = h->partials;
             launch_op0(s, OP_MATVEC, a);
+            // p.q over the full grid is the periodic p.Kp before the fold
+            // (p is equal on images); the fold completes q for the update
+            if (per) launch_per_fold<double>(s, L0.g, h->Q, 0, ctl);
             launch_cg_alpha(s, h->sc, h->partials, nmv);
             launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials);
             launch_cg_check(s, h->sc, h->partials, kRedBlocks, tol, maxit);
@@ -509,6 +528,10 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
     if (!dense_ok && !(params && params->preset == GPUTO_PRESET_MGCG9)) return GPUTO_ERR_WORKSPACE;
     // the learned stage works on level 2 of the FP32 V-cycle
     if (params && params->learned && (params->precision == GPUTO_PREC_F64 || nl < 4)) return GPUTO_ERR_ARGUMENT;
+    // a periodic cell keeps a master distinct from its images on every
+    // level; the learned stage is trained on non-periodic defects only
+    if (params && params->periodic && (gc.nx < 2 || gc.ny < 2 || gc.nz < 2 || params->learned))
+        return GPUTO_ERR_ARGUMENT;
     gputo_handle *h = new gputo_handle();
     if (params)
         h->prm = *params;
@@ -567,6 +590,7 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
             // DOFs of one element are at most 3(nny*nnz + nny + 1) + 2 apart
             const Grid &gc = h->L[nl - 1].g;
             C.bw = 3 * (gc.nny * gc.nnz + gc.nny + 1) + 2;
+            if (h->prm.periodic) C.bw = C.n;  // wrapped couplings: no band
         }
         const long N = (C.n + 31) / 32 * 32;
         if (dense_ok) {
@@ -618,6 +642,7 @@ int gputo_set_params(gputo_handle *h, const gputo_params *params)
 {
     if (!h || !params) return GPUTO_ERR_ARGUMENT;
     if (!!params->learned != !!h->prm.learned) return GPUTO_ERR_ARGUMENT;  // fixed at creation
+    if (!!params->periodic != !!h->prm.periodic) return GPUTO_ERR_ARGUMENT;
     if (params->precision != h->prm.precision && params->precision == GPUTO_PREC_F64 && h->ln)
         return GPUTO_ERR_ARGUMENT;  // the learned stage runs in the FP32 V-cycle
     if (!h->C.AQ && params->preset != GPUTO_PRESET_MGCG9) return GPUTO_ERR_WORKSPACE;  // no dense coarsest
@@ -1059,6 +1084,92 @@ int gputo_coarse_solve_f64(gputo_handle *h, const double *b, double *x)
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(x, v.T, v.g.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    } catch (...) {
+        return map_exception();
+    }
+}
+
+}  // extern "C"
+
+namespace {
+
+// a.b over the full grid, partials summed on the host in block order
+double dot_host(gputo_handle *h, const double *a, const double *b, long n)
+{
+    launch_dot_partial(h->stream, a, b, n, h->partials, nullptr);
+    std::vector<double> p(kRedBlocks);
+    HIPCHK(hipMemcpyAsync(p.data(), h->partials, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    double acc = 0.0;
+    for (double v : p) acc += v;
+    return acc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gputo_homogenize(gputo_handle *h, const double *Ke0, const double *CX, double tol, int maxit, double *CH,
+                     double *relres, int *nit, double *chi)
+{
+    if (!h || !Ke0 || !CX || !CH || !h->prm.periodic) return GPUTO_ERR_ARGUMENT;
+    try {
+        const Grid &g0 = h->L[0].g;
+        const long n = g0.ndof;
+        if (!all_finite(Ke0, 576) || !all_finite(CX, g0.nel)) return GPUTO_ERR_NONFINITE;
+        hipStream_t s = h->stream;
+        set_ke(h, Ke0);
+        HIPCHK(hipMemcpyAsync(h->CX, CX, g0.nel * sizeof(double), hipMemcpyHostToDevice, s));
+        // node 0 and its 7 images are pinned: the periodic operator's null
+        // space is the three rigid translations
+        std::vector<int> fixed;
+        for (int a = 0; a < 2; ++a)
+            for (int c = 0; c < 2; ++c)
+                for (int b = 0; b < 2; ++b) {
+                    const long nod = (long)g0.nny * g0.nnz * (a * g0.nx) + (long)g0.nny * (c * g0.nz) + b * g0.ny;
+                    for (int d = 0; d < 3; ++d) fixed.push_back((int)(3 * nod + d));
+                }
+        if (int rc = setup_hierarchy(h, fixed)) return rc;
+        if (!h->hom) h->hom = dalloc<double>(h, 12 * (size_t)n);
+        double *Fq = h->hom, *Xq = h->hom + 6 * n;
+        double fe[6][24], Q0[6][6];
+        homog_loads(Ke0, fe, Q0);  // Ke0 read row-major, as the operators read it
+        launch_homog_rhs(s, g0, h->CX, fe, h->L[0].mask, Fq);
+        HIPCHK(hipGetLastError());
+        for (int q = 0; q < 6; ++q) {
+            double rr = 0.0;
+            int ni = 0;
+            int rc = run_pcg(h, Fq + q * n, Xq + q * n, tol, maxit, &rr, &ni);
+            if (relres) relres[q] = rr;
+            if (nit) nit[q] = rc ? rc : ni;
+            if (rc) return rc;
+        }
+        // CH_ab |Y| = sum_e CX_e (u0_a - chi_a)^T Ke0 (u0_b - chi_b)
+        //           = Q0_ab sum CX - chi_a.f_b - chi_b.f_a + chi_a.K chi_b
+        double sumcx = 0.0;
+        for (long e = 0; e < g0.nel; ++e) sumcx += CX[e];
+        double XF[6][6], XKX[6][6];
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b < 6; ++b) XF[a][b] = dot_host(h, Xq + a * n, Fq + b * n, n);
+        for (int b = 0; b < 6; ++b) {
+            OpArgs o = op_args(h, 0);
+            o.ctl = nullptr;
+            o.cx_floor = 0.0;
+            o.U = Xq + b * n;
+            o.out = h->Q;
+            o.partials = nullptr;
+            launch_op0(s, OP_MATVEC, o);  // unfolded: chi_a is equal on images
+            for (int a = 0; a <= b; ++a) XKX[a][b] = XKX[b][a] = dot_host(h, Xq + a * n, h->Q, n);
+        }
+        const double vol = (double)g0.nel;
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b < 6; ++b)
+                CH[6 * a + b] = (Q0[a][b] * sumcx - XF[a][b] - XF[b][a] + XKX[a][b]) / vol;
+        if (chi) {
+            HIPCHK(hipMemcpyAsync(chi, Xq, 6 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
         return 0;
     } catch (...) {
         return map_exception();

@@ -51,6 +51,7 @@ struct HierView {
     bool l1_mf = false;            // level 1 applied matrix-free from CX
     bool learn = false;            // learned level-2 stage (FP32 views only)
     bool mg9 = false;              // every level matrix-free (mgcg9 engine)
+    bool per = false;              // periodic cell: folded defects, images kept equal
 };
 
 constexpr int kLearnLevel = 2;
@@ -120,11 +121,32 @@ void level_op(const gputo_handle *h, const HierView<T> &H, int l, int mode, cons
     }
 }
 
+// Periodic sweeps: each is the non-periodic defect into R, then the
+// Jacobi update of every master from its folded defect, copied to its
+// images; `pre` first sets U = prerelax*F/AD.
+template <class T>
+void sweeps_per(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T *U, int n, bool pre, const int *ctl)
+{
+    const LevelView<T> &v = H.L[l];
+    const Level &lv = h->L[l];
+    const Preset p = preset_of(h->prm);
+    if (pre) launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 1, ctl);
+    for (int s = 0; s < n; ++s) {
+        OpArgsT<T> a = view_args(h, H, l, ctl);
+        a.U = U;
+        a.F = F;
+        a.out = v.R;
+        level_op(h, H, l, OP_DEFECT, a);
+        launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 0, ctl);
+    }
+}
+
 // n damped-Jacobi sweeps on level l ending in U; `pre` forms the first
 // sweep's input as prerelax*F/AD (preRelc, mgcg1.cu:1190-1197).
 template <class T>
 void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T *U, int n, bool pre, const int *ctl)
 {
+    if (H.per) return sweeps_per(h, H, l, F, U, n, pre, ctl);
     const LevelView<T> &v = H.L[l];
     const T *src = U;
     for (int s = 0; s < n; ++s) {
@@ -153,21 +175,28 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
         const Level &lv = h->L[l];
         const T *F = l == 0 ? F0 : v.F;
         T *U = l == 0 ? U0 : v.U;
-        if (l > 0) launch_restrict<T>(h->stream, lv.g, H.L[l - 1].R, v.F, lv.mask, ctl);
+        if (l > 0) {
+            launch_restrict<T>(h->stream, lv.g, H.L[l - 1].R, v.F, lv.mask, ctl);
+            if (H.per) launch_per_fold<T>(h->stream, lv.g, v.F, 0, ctl);
+        }
         if (l < nl - 1) {
-            if (p.npre == 0) {
+            if (p.npre == 0 && !H.per) {
                 OpArgsT<T> a = view_args(h, H, l, ctl);
                 a.F = F;
                 a.out = v.R;
                 a.out2 = U;
                 level_op(h, H, l, OP_PRE_DEFECT, a);
             } else {
-                sweeps_t(h, H, l, F, U, p.npre, true, ctl);
+                if (p.npre == 0)
+                    launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 1, ctl);
+                else
+                    sweeps_t(h, H, l, F, U, p.npre, true, ctl);
                 OpArgsT<T> a = view_args(h, H, l, ctl);
                 a.U = U;
                 a.F = F;
                 a.out = v.R;
                 level_op(h, H, l, OP_DEFECT, a);
+                if (H.per) launch_per_fold<T>(h->stream, lv.g, v.R, 0, ctl);
             }
         } else if (!h->C.direct) {
             // coarsest level too large to factor: 10*npre sweeps from the
@@ -175,6 +204,7 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
             sweeps_t(h, H, l, F, U, 10 * p.npre, false, ctl);
         } else {
             launch_coarse_solve<T>(h->stream, h->C, F, U, lv.mask, ctl);
+            if (H.per) launch_per_bcast<T>(h->stream, lv.g, U, ctl);
         }
     }
     for (int l = nl - 2; l >= 0; --l) {

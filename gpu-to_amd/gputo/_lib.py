@@ -50,6 +50,7 @@ class GputoParams(ctypes.Structure):
         ("nswp", ctypes.c_int),
         ("l1_kernel", ctypes.c_int),
         ("coarse_lag", ctypes.c_int),
+        ("periodic", ctypes.c_int),
     ]
 
 
@@ -137,6 +138,7 @@ SIGNATURES = {
     "gputo_to_end": (_I, [_P]),
     "gputo_filter_f64": (_I, [_P, _P, _D, _P]),
     "gputo_sens_f64": (_I, [_P, _P, _P, _P]),
+    "gputo_homogenize": (_I, [_P, _P, _P, _D, _I, _P, _P, _P, _P]),
     "gputo_version": (ctypes.c_char_p, []),
     "gputo_device_synchronize": (_I, []),
 }

@@ -27,7 +27,7 @@ class Solver:
     """One libgputo handle for an nelx x nely x nelz mesh with nl levels."""
 
     def __init__(self, nelx, nely, nelz, nl, preset=_lib.PRESET_REF_C, precision=_lib.PREC_F64, learned=False,
-                 print_level=0, l0_kernel=0, l1_kernel=0, nswp=0, coarse_lag=0):
+                 print_level=0, l0_kernel=0, l1_kernel=0, nswp=0, coarse_lag=0, periodic=False):
         self.lib = _lib.load()
         self.nelx, self.nely, self.nelz, self.nl = int(nelx), int(nely), int(nelz), int(nl)
         self.nel = self.nelx * self.nely * self.nelz
@@ -42,6 +42,7 @@ class Solver:
         self.params.l1_kernel = l1_kernel
         self.params.nswp = nswp
         self.params.coarse_lag = coarse_lag
+        self.params.periodic = int(bool(periodic))
         h = ctypes.c_void_p()
         check(self.lib.gputo_create(self.nelx, self.nely, self.nelz, self.nl, ctypes.byref(self.params),
                                     ctypes.byref(h)), "gputo_create")
@@ -75,6 +76,23 @@ class Solver:
         rc = self.lib.gputo_mgcg1(self.h, ptr(Ke0), ptr(F), ptr(U), ptr(CX), ptr(fixed), len(fixed), float(tol),
                                   int(maxit), ctypes.byref(relres), ctypes.byref(nit))
         return U, relres.value, nit.value, rc
+
+    # ---- periodic unit cell ----------------------------------------------
+    def homogenize(self, Ke0, CX, tol, maxit, fields=False):
+        """Homogenised 6x6 elasticity tensor of a periodic cell (handle created
+        with periodic=True).  Voigt order xx, yy, zz, xy, yz, zx with
+        engineering shears, x along i, y along j, z along k.  Returns
+        (CH, relres[6], nit[6], chi) where chi (6 x ndof, full node layout,
+        equal on image nodes) is None unless fields=True."""
+        Ke0, CX = f64(Ke0), f64(CX)
+        CH = np.zeros(36)
+        relres = np.zeros(6)
+        nit = np.zeros(6, dtype=np.int32)
+        chi = np.zeros((6, self.ndof)) if fields else None
+        rc = self.lib.gputo_homogenize(self.h, ptr(Ke0), ptr(CX), float(tol), int(maxit), ptr(CH), ptr(relres),
+                                       ptr(nit), ptr(chi))
+        check(rc, "gputo_homogenize")
+        return CH.reshape(6, 6), relres, nit, chi
 
     # ---- kernel-level entry points --------------------------------------
     def matvec(self, Ke0, CX, u, fixed):

@@ -98,3 +98,34 @@ def perturbed_density(nelx, nely, nelz, volfrac, seed=None, amplitude=0.01):
     if seed is not None:
         x += np.random.default_rng(seed).uniform(-amplitude, amplitude, nel)
     return x
+
+
+def inclusion_cell(n, seed=0, volfrac=0.3, radius=3.0, inclusion=0.25):
+    """Initial density of the periodic unit cell (n x n x n elements, element
+    e = n*n*i + n*k + j): a seeded U(0, 1) field smoothed with the cone
+    filter h = max(0, radius - dist) under periodic wrap, shifted to mean
+    volfrac and clipped to [0, 1], with the elements within inclusion*n of
+    the cell centre set to 0 (a spherical void inclusion)."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, 1.0, (n, n, n))  # axes (i, k, j)
+    r = int(np.ceil(radius)) - 1
+    acc = np.zeros_like(x)
+    wsum = 0.0
+    for di in range(-r, r + 1):
+        for dk in range(-r, r + 1):
+            for dj in range(-r, r + 1):
+                w = max(0.0, radius - np.sqrt(di * di + dj * dj + dk * dk))
+                if w > 0.0:
+                    acc += w * np.roll(x, (di, dk, dj), axis=(0, 1, 2))
+                    wsum += w
+    x = acc / wsum
+    x = np.clip(x - x.mean() + volfrac, 0.0, 1.0)
+    c = np.arange(n) + 0.5 - n / 2.0
+    ci, ck, cj = np.meshgrid(c, c, c, indexing="ij")
+    x[ci ** 2 + cj ** 2 + ck ** 2 <= (inclusion * n) ** 2] = 0.0
+    return x.ravel()
+
+
+def simp_cx(x, penal=3.0, Emin=1e-9, E0=1.0):
+    """E(x) = Emin + x^penal (E0 - Emin)."""
+    return Emin + np.asarray(x, dtype=np.float64) ** penal * (E0 - Emin)

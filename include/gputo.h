@@ -87,6 +87,14 @@ typedef struct gputo_params {
                           parameter change, waits for its own factor).  Lagging
                           hides the factorisation but costs iterations when the
                           design changes fast (early design iterations) */
+    int periodic;      /* 1: periodic unit cell.  The mesh wraps in x, y and z: the
+                          nodes of the faces i = nelx, j = nely, k = nelz are the
+                          images of those at 0.  Vectors keep the
+                          (nelx+1)(nely+1)(nelz+1)-node layout; a right-hand side
+                          may be split between a node and its images (they are
+                          summed), a solution holds equal values on them.  Every
+                          level needs at least 2 elements per direction; the
+                          learned stage is not available */
 } gputo_params;
 
 /* Fill *p with the defaults (REF_C, F64, no learned stage, operator
@@ -248,6 +256,23 @@ int gputo_to_end(gputo_handle *h);
 /* Device-side driver kernels on host arrays (parity tests). */
 int gputo_filter_f64(gputo_handle *h, const double *in, double rmin, double *out);
 int gputo_sens_f64(gputo_handle *h, const double *Ke0, const double *U_cframe, double *ce);
+
+/* Periodic homogenisation of a unit cell (no counterpart in the reference,
+ * which has no periodic solve; QNC_V1_mod1m.m is a supported compliance
+ * problem).  The handle must have params.periodic = 1.  The six unit
+ * macroscopic strains (Voigt order xx, yy, zz, xy, yz, zx, engineering
+ * shears; x along i, y along j, z along k, the frame of Ke0's local node
+ * order) give six load cases f_q = sum_e CX_e Ke0 u0_q, u0_q = eps_q . x;
+ * each is solved with the handle's MGCG (one hierarchy, set up once, node 0
+ * and its images pinned) and
+ *   CH[6a+b] = 1/nel * sum_e CX_e (u0_a - chi_a)_e^T Ke0 (u0_b - chi_b)_e,
+ * evaluated as (sum CX) Ke0-energy of the unit strains minus the cross
+ * terms chi_a.f_b + chi_b.f_a plus chi_a.K chi_b.  CH is 36 doubles
+ * (row-major, symmetric).  relres and nit get 6 entries each (may be NULL);
+ * chi (6 x ndof doubles, may be NULL) gets the periodic fluctuation fields
+ * on the full node layout, equal on image nodes. */
+int gputo_homogenize(gputo_handle *h, const double *Ke0, const double *CX, double tol, int maxit, double *CH,
+                     double *relres, int *nit, double *chi);
 
 /* Library version string. */
 const char *gputo_version(void);

@@ -40,3 +40,21 @@ def test_workspace_estimate_formula(gputo):
         if lev == nl - 1:
             lw += 3 * nd + nd * nd
     assert lib.gputo_est_workspace(nfd, nl, nelx, nely, nelz) == lw
+
+
+def test_periodic_argument_checks(gputo):
+    """A periodic handle needs 2 elements per direction on its coarsest level
+    and no learned stage; both are rejected before any device call."""
+    import ctypes
+    from gputo import _lib
+    lib = gputo.load()
+    p = _lib.GputoParams()
+    lib.gputo_default_params(ctypes.byref(p))
+    p.periodic = 1
+    h = ctypes.c_void_p()
+    # 8 elements, 4 levels: the coarsest level has 1 element per direction
+    assert lib.gputo_create(8, 8, 8, 4, ctypes.byref(p), ctypes.byref(h)) == _lib.GPUTO_ERR_ARGUMENT
+    p.precision = _lib.PREC_MIXED_F32
+    p.learned = 1
+    assert lib.gputo_create(32, 32, 32, 4, ctypes.byref(p), ctypes.byref(h)) == _lib.GPUTO_ERR_ARGUMENT
+    assert not h.value
