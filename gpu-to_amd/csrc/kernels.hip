@@ -347,20 +347,34 @@ __global__ __launch_bounds__(kBlock) void k_stored(OpArgsT<T> a)
             const long nel = g.nel;
             const T *__restrict__ Ae = a.A + ((long)g.ny * g.nz * ex + (long)g.ny * ez + ey);
             const int la = lidx(1 - ei, 1 - ej, 1 - ek);
+            // all loads of a pass (the nodal values and the 3 x 3 blocks of
+            // NB nodes) are issued before the products: one memory latency
+            // per pass instead of one per node; FP64 takes two passes of 4
+            // nodes to bound its registers
+            constexpr int NB = sizeof(T) == 8 ? 4 : 8;
 #pragma unroll
-            for (int lb = 0; lb < 8; ++lb) {
-                const int bx = (lb == 1 || lb == 2 || lb == 5 || lb == 6) ? 1 : 0;
-                const int by = (lb == 2 || lb == 3 || lb == 6 || lb == 7) ? 1 : 0;
-                const int bz = lb >= 4 ? 1 : 0;
-                const long dof = 3 * ((long)g.nny * g.nnz * (ex + bx) + (long)g.nny * (ez + bz) + (ey + by));
-                const T u0 = stored_u<T, MODE>(a, dof), u1 = stored_u<T, MODE>(a, dof + 1),
-                        u2 = stored_u<T, MODE>(a, dof + 2);
-                y0 += Ae[pidx(3 * la + 0, 3 * lb + 0) * nel] * u0 + Ae[pidx(3 * la + 0, 3 * lb + 1) * nel] * u1 +
-                      Ae[pidx(3 * la + 0, 3 * lb + 2) * nel] * u2;
-                y1 += Ae[pidx(3 * la + 1, 3 * lb + 0) * nel] * u0 + Ae[pidx(3 * la + 1, 3 * lb + 1) * nel] * u1 +
-                      Ae[pidx(3 * la + 1, 3 * lb + 2) * nel] * u2;
-                y2 += Ae[pidx(3 * la + 2, 3 * lb + 0) * nel] * u0 + Ae[pidx(3 * la + 2, 3 * lb + 1) * nel] * u1 +
-                      Ae[pidx(3 * la + 2, 3 * lb + 2) * nel] * u2;
+            for (int l0 = 0; l0 < 8; l0 += NB) {
+                T uu[NB][3], av[NB][3][3];
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+                    const int lb = l0 + q;
+                    const int bx = (lb == 1 || lb == 2 || lb == 5 || lb == 6) ? 1 : 0;
+                    const int by = (lb == 2 || lb == 3 || lb == 6 || lb == 7) ? 1 : 0;
+                    const int bz = lb >= 4 ? 1 : 0;
+                    const long dof = 3 * ((long)g.nny * g.nnz * (ex + bx) + (long)g.nny * (ez + bz) + (ey + by));
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) uu[q][c] = stored_u<T, MODE>(a, dof + c);
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) av[q][r][c] = Ae[pidx(3 * la + r, 3 * lb + c) * nel];
+                }
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+                    y0 += av[q][0][0] * uu[q][0] + av[q][0][1] * uu[q][1] + av[q][0][2] * uu[q][2];
+                    y1 += av[q][1][0] * uu[q][0] + av[q][1][1] * uu[q][1] + av[q][1][2] * uu[q][2];
+                    y2 += av[q][2][0] * uu[q][0] + av[q][2][1] * uu[q][1] + av[q][2][2] * uu[q][2];
+                }
             }
         }
     }
