@@ -182,6 +182,9 @@ void launch_restrict(hipStream_t s, const Grid &gc, const T *Rf, T *Fc, const ui
 template <class T>
 void launch_interp(hipStream_t s, const Grid &gf, const T *Uc, T *Uf, const uint8_t *maskf, int zero_fixed,
                    const int *ctl);
+template <class T>  // Uout = Uin + P Uc (Uout may equal Uin)
+void launch_interp_to(hipStream_t s, const Grid &gf, const T *Uc, const T *Uin, T *Uout, const uint8_t *maskf,
+                      int zero_fixed, const int *ctl);
 template <class T> void launch_fill(hipStream_t s, T *v, long n, double val);
 template <class Ta, class Tb>
 void launch_dot_partial(hipStream_t s, const Ta *a, const Tb *b, long n, double *partials, const int *ctl);

@@ -533,9 +533,11 @@ template void launch_restrict<float>(hipStream_t, const Grid &, const float *, f
 // Prolongation U_f += P U_c (interpc, mgcg1.cu:1587-1641) followed by
 // U_f[fixed] = 0 (mgcg1.cu:631): one thread per fine node averages the 1,
 // 2, 4 or 8 coarse nodes around it according to the parity of (i,j,k).
+// The sum may go to another vector (Uout = Uin + P U_c; Uout == Uin is the
+// in-place form).
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict__ Uc, T *__restrict__ Uf,
+__global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict__ Uc, const T *Uin, T *Uout,
                                                   const uint8_t *__restrict__ maskf, int zero_fixed, const int *ctl)
 {
     if (ctl && ctl[0]) return;
@@ -568,16 +570,28 @@ __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict_
     }
     const unsigned m = zero_fixed ? maskf[nod] : 0u;
     const long d = 3 * nod;
-    Uf[d + 0] = (m & 1u) ? T(0) : Uf[d + 0] + w * s0;
-    Uf[d + 1] = (m & 2u) ? T(0) : Uf[d + 1] + w * s1;
-    Uf[d + 2] = (m & 4u) ? T(0) : Uf[d + 2] + w * s2;
+    Uout[d + 0] = (m & 1u) ? T(0) : Uin[d + 0] + w * s0;
+    Uout[d + 1] = (m & 2u) ? T(0) : Uin[d + 1] + w * s1;
+    Uout[d + 2] = (m & 4u) ? T(0) : Uin[d + 2] + w * s2;
 }
+
+template <class T>
+void launch_interp_to(hipStream_t s, const Grid &gf, const T *Uc, const T *Uin, T *Uout, const uint8_t *maskf,
+                      int zero_fixed, const int *ctl)
+{
+    k_interp<T><<<(unsigned)((gf.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gf, Uc, Uin, Uout, maskf, zero_fixed,
+                                                                               ctl);
+}
+template void launch_interp_to<double>(hipStream_t, const Grid &, const double *, const double *, double *,
+                                       const uint8_t *, int, const int *);
+template void launch_interp_to<float>(hipStream_t, const Grid &, const float *, const float *, float *,
+                                      const uint8_t *, int, const int *);
 
 template <class T>
 void launch_interp(hipStream_t s, const Grid &gf, const T *Uc, T *Uf, const uint8_t *maskf, int zero_fixed,
                    const int *ctl)
 {
-    k_interp<T><<<(unsigned)((gf.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gf, Uc, Uf, maskf, zero_fixed, ctl);
+    launch_interp_to<T>(s, gf, Uc, Uf, Uf, maskf, zero_fixed, ctl);
 }
 template void launch_interp<double>(hipStream_t, const Grid &, const double *, double *, const uint8_t *, int,
                                     const int *);

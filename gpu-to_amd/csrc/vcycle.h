@@ -142,13 +142,15 @@ void sweeps_per(const gputo_handle *h, const HierView<T> &H, int l, const T *F, 
 }
 
 // n damped-Jacobi sweeps on level l ending in U; `pre` forms the first
-// sweep's input as prerelax*F/AD (preRelc, mgcg1.cu:1190-1197).
+// sweep's input as prerelax*F/AD (preRelc, mgcg1.cu:1190-1197); `in_tmp`:
+// the input iterate is in the level's Tmp (an odd n then needs no copy).
 template <class T>
-void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T *U, int n, bool pre, const int *ctl)
+void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T *U, int n, bool pre, const int *ctl,
+              bool in_tmp = false)
 {
     if (H.per) return sweeps_per(h, H, l, F, U, n, pre, ctl);
     const LevelView<T> &v = H.L[l];
-    const T *src = U;
+    const T *src = in_tmp ? v.Tmp : U;
     for (int s = 0; s < n; ++s) {
         T *dst = ((n - 1 - s) % 2 == 0) ? U : v.Tmp;
         if (!(pre && s == 0) && dst == src) {
@@ -211,8 +213,12 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
         const LevelView<T> &v = H.L[l];
         const T *F = l == 0 ? F0 : v.F;
         T *U = l == 0 ? U0 : v.U;
-        launch_interp<T>(h->stream, h->L[l].g, H.L[l + 1].U, U, h->L[l].mask, p.zero_after_interp, ctl);
-        sweeps_t(h, H, l, F, U, p.npost, false, ctl);
+        // an odd number of post-sweeps starts from Tmp so the last one
+        // writes U without a copy
+        const bool to_tmp = !H.per && p.npost % 2 == 1;
+        launch_interp_to<T>(h->stream, h->L[l].g, H.L[l + 1].U, U, to_tmp ? H.L[l].Tmp : U, h->L[l].mask,
+                            p.zero_after_interp, ctl);
+        sweeps_t(h, H, l, F, U, p.npost, false, ctl, to_tmp);
         if constexpr (std::is_same<T, float>::value) {
             if (H.learn && l == kLearnLevel) learn_stage(const_cast<gputo_handle *>(h), H, l, F, U, ctl);
         }
