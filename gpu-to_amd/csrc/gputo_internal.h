@@ -198,9 +198,10 @@ constexpr int kGtMaxDepth = 4;  // deepest level built from tables (8^4 x 300 do
 // A table product costs 300 FMA per level-0 element at every level, spread
 // over nel_l/16 blocks that each stream 8^l table rows; below this many
 // elements the child-by-child Galerkin of the level above (about 1.7e5 FMA
-// per element, one block each) is faster (measured at 192x96x96: level 3
-// with 3456 elements took 263 us from its table).
-constexpr long kGtMinElems = 8192;
+// per element, one block each) is faster.  Measured at 192x96x96: level 3
+// (3456 elements) takes 260 us from its table and 452 us child by child;
+// level 4 (432 elements) 95 us child by child.
+constexpr long kGtMinElems = 2048;
 void launch_corrcsa(hipStream_t s, const Grid &g, const int *fixed_d, int nfd, double *A, int mirrored);
 // mgcg9 coarse levels: CX_c = 2 mean(children), element matrices CX Ke, and
 // the zeroing of fixed rows/columns of the dense coarsest matrix
