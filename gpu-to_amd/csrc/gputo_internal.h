@@ -157,6 +157,7 @@ struct gputo_handle {
     int *ctl_zero = nullptr;      // an always-zero control word
     long long bytes = 0;
     bool hier_ready = false;
+    bool masks_ready = false;     // level fixed lists and masks match L[0].fixed
     std::vector<void *> allocs;
     struct ToState *to = nullptr;
     gputo::MixedState *mx = nullptr;

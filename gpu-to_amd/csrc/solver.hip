@@ -308,15 +308,21 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
     if (h->C.ev_done) HIPCHK(hipStreamWaitEvent(s, h->C.ev_done, 0));
     coarse_adopt(h->C);
     ensure_tables(h);
-    L0.fixed = fixed0;
-    for (int l = 1; l < h->nl; ++l) h->L[l].fixed = restrict_fixed(h->L[l - 1].fixed, h->L[l].g);
-    for (int l = 0; l < h->nl; ++l) {
-        Level &v = h->L[l];
-        std::vector<uint8_t> m = node_mask(v.fixed, v.g);
-        HIPCHK(hipMemcpyAsync(v.mask, m.data(), m.size(), hipMemcpyHostToDevice, s));
-        if (!v.fixed.empty())
-            HIPCHK(hipMemcpyAsync(v.fixed_d, v.fixed.data(), v.fixed.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));  // host vectors above are temporaries
+    // the fixed lists and masks of every level are rebuilt only when the
+    // fixed DOFs change (a design loop keeps them)
+    if (!h->masks_ready || fixed0 != L0.fixed) {
+        L0.fixed = fixed0;
+        for (int l = 1; l < h->nl; ++l) h->L[l].fixed = restrict_fixed(h->L[l - 1].fixed, h->L[l].g);
+        for (int l = 0; l < h->nl; ++l) {
+            Level &v = h->L[l];
+            std::vector<uint8_t> m = node_mask(v.fixed, v.g);
+            HIPCHK(hipMemcpyAsync(v.mask, m.data(), m.size(), hipMemcpyHostToDevice, s));
+            if (!v.fixed.empty())
+                HIPCHK(hipMemcpyAsync(v.fixed_d, v.fixed.data(), v.fixed.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));  // host vectors above are temporaries
+        }
+        h->masks_ready = true;
     }
     const bool per = h->prm.periodic != 0;
     launch_diag0(s, L0.g, h->Ke, h->CX, L0.AD);
@@ -761,6 +767,7 @@ static int host_op(gputo_handle *h, int mode, const double *Ke0, const double *C
         std::vector<uint8_t> m = node_mask(fx, g0);
         hipStream_t s = h->stream;
         HIPCHK(hipMemcpyAsync(h->L[0].mask, m.data(), m.size(), hipMemcpyHostToDevice, s));
+        h->masks_ready = false;  // the level-0 mask no longer matches L[0].fixed
         if (mode == 2) {
             launch_diag0(s, g0, h->Ke, h->CX, h->Q);
         } else {
@@ -803,6 +810,7 @@ int gputo_matvec_mixed(gputo_handle *h, int kernel, const double *Ke0, const dou
         std::vector<uint8_t> m = node_mask(fx, g0);
         hipStream_t s = h->stream;
         HIPCHK(hipMemcpyAsync(h->L[0].mask, m.data(), m.size(), hipMemcpyHostToDevice, s));
+        h->masks_ready = false;  // the level-0 mask no longer matches L[0].fixed
         HIPCHK(hipMemcpyAsync(h->P, u, g0.ndof * sizeof(double), hipMemcpyHostToDevice, s));
         mixed_apply_l0(h, kernel, h->P, h->Q);
         HIPCHK(hipMemcpyAsync(y, h->Q, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
