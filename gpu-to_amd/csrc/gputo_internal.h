@@ -190,6 +190,8 @@ template <class T> void launch_fill(hipStream_t s, T *v, long n, double val);
 template <class Ta, class Tb>
 void launch_dot_partial(hipStream_t s, const Ta *a, const Tb *b, long n, double *partials, const int *ctl);
 template <class Ti, class To> void launch_copy(hipStream_t s, const Ti *src, To *dst, long n, const int *ctl);
+template <class T>  // U = prerelax F / AD
+void launch_prerelax(hipStream_t s, const T *F, const T *AD, T *U, long n, double prerelax, const int *ctl);
 void launch_coarse_l1(hipStream_t s, const Grid &gc, const double *Aelm, const double *CXf, double *A);
 void launch_coarse_galerkin(hipStream_t s, const Grid &gc, const double *Af, const double *Ke, const double *CXf,
                             double *Ac);

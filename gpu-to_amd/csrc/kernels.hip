@@ -639,6 +639,26 @@ void launch_copy(hipStream_t s, const Ti *src, To *dst, long n, const int *ctl)
 {
     k_copy<Ti, To><<<grid_for(n), kBlock, 0, s>>>(src, dst, n, ctl);
 }
+// U = prerelax F / AD (preRelc, mgcg1.cu:1190-1197), the same expression the
+// fused PRE modes form on the fly
+template <class T>
+__global__ void k_prerelax(const T *__restrict__ F, const T *__restrict__ AD, T *__restrict__ U, long n, T pr,
+                           const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        U[i] = pr * F[i] / AD[i];
+}
+
+template <class T>
+void launch_prerelax(hipStream_t s, const T *F, const T *AD, T *U, long n, double prerelax, const int *ctl)
+{
+    k_prerelax<T><<<grid_for(n), kBlock, 0, s>>>(F, AD, U, n, T(prerelax), ctl);
+}
+template void launch_prerelax<double>(hipStream_t, const double *, const double *, double *, long, double,
+                                      const int *);
+template void launch_prerelax<float>(hipStream_t, const float *, const float *, float *, long, double, const int *);
+
 template void launch_copy<double, double>(hipStream_t, const double *, double *, long, const int *);
 template void launch_copy<float, float>(hipStream_t, const float *, float *, long, const int *);
 template void launch_copy<double, float>(hipStream_t, const double *, float *, long, const int *);

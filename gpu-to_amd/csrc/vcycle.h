@@ -150,6 +150,16 @@ void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T 
 {
     if (H.per) return sweeps_per(h, H, l, F, U, n, pre, ctl);
     const LevelView<T> &v = H.L[l];
+    if (pre && l == 0 && n > 0 && !H.mg9 && H.l0 >= L0_MFMA_F32) {
+        // MFMA level 0: the prerelaxed iterate is written by an elementwise
+        // pass and the first sweep is a plain one (measured at 192x96x96:
+        // 11 + 64 us against 93 us for the fused form, which stages F and AD
+        // for every halo node; the FP64 node-gather keeps its fused form,
+        // 87 us against 22 + 70 us).  It goes where the sweeps end in U.
+        in_tmp = n % 2 == 1;
+        launch_prerelax<T>(h->stream, F, v.AD, in_tmp ? v.Tmp : U, h->L[l].g.ndof, preset_of(h->prm).prerelax, ctl);
+        pre = false;
+    }
     const T *src = in_tmp ? v.Tmp : U;
     for (int s = 0; s < n; ++s) {
         T *dst = ((n - 1 - s) % 2 == 0) ? U : v.Tmp;
