@@ -498,21 +498,30 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const T *__restric
     const int I = (int)(nod / ((long)gc.nny * gc.nnz));
     const int fny = 2 * gc.ny + 1, fnz = 2 * gc.nz + 1, fnx = 2 * gc.nx + 1;
     T s0 = T(0), s1 = T(0), s2 = T(0);
+    // one fine plane (9 nodes) per pass, its loads issued together; nodes
+    // outside the mesh read node 0 with weight 0 (no branches around loads)
+#pragma unroll
     for (int di = -1; di <= 1; ++di) {
         const int iP = 2 * I + di;
-        if (iP < 0 || iP >= fnx) continue;
-        for (int dk = -1; dk <= 1; ++dk) {
-            const int kP = 2 * K + dk;
-            if (kP < 0 || kP >= fnz) continue;
+        T v[9][3], w[9];
+#pragma unroll
+        for (int dk = -1; dk <= 1; ++dk)
+#pragma unroll
             for (int dj = -1; dj <= 1; ++dj) {
-                const int jP = 2 * J + dj;
-                if (jP < 0 || jP >= fny) continue;
-                const T w = T(1) / (T)(1 << (abs(di) + abs(dj) + abs(dk)));
-                const long f = 3 * ((long)fny * fnz * iP + (long)fny * kP + jP);
-                s0 += w * Rf[f];
-                s1 += w * Rf[f + 1];
-                s2 += w * Rf[f + 2];
+                const int q = 3 * (dk + 1) + (dj + 1);
+                const int kP = 2 * K + dk, jP = 2 * J + dj;
+                const bool ok = iP >= 0 && iP < fnx && kP >= 0 && kP < fnz && jP >= 0 && jP < fny;
+                const long f = ok ? 3 * ((long)fny * fnz * iP + (long)fny * kP + jP) : 0;
+                w[q] = ok ? T(1) / (T)(1 << (abs(di) + abs(dj) + abs(dk))) : T(0);
+                v[q][0] = Rf[f];
+                v[q][1] = Rf[f + 1];
+                v[q][2] = Rf[f + 2];
             }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            s0 += w[q] * v[q][0];
+            s1 += w[q] * v[q][1];
+            s2 += w[q] * v[q][2];
         }
     }
     const unsigned m = maskc[nod];
@@ -549,30 +558,38 @@ __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict_
     const int cny = gf.ny / 2 + 1, cnz = gf.nz / 2 + 1;
     const int ia = i >> 1, ib = (i + 1) >> 1, ja = j >> 1, jb = (j + 1) >> 1, ka = k >> 1, kb = (k + 1) >> 1;
     const T w = T(1) / (T)(1 << ((i & 1) + (j & 1) + (k & 1)));
-    T s0 = T(0), s1 = T(0), s2 = T(0);
-    // corners (ia|ib, ja|jb, ka|kb); for even indices a == b and the
-    // duplicate corners are skipped so each coarse node counts once
-    for (int ci = 0; ci < 2; ++ci) {
-        const int I = ci ? ib : ia;
-        if (ci && ib == ia) continue;
-        for (int ck = 0; ck < 2; ++ck) {
-            const int K = ck ? kb : ka;
-            if (ck && kb == ka) continue;
+    // corners (ia|ib, ja|jb, ka|kb), all 8 loaded together; for an even
+    // index a == b and the second corner counts 0, so each coarse node
+    // enters once
+    T v[8][3], on[8];
+#pragma unroll
+    for (int ci = 0; ci < 2; ++ci)
+#pragma unroll
+        for (int ck = 0; ck < 2; ++ck)
+#pragma unroll
             for (int cj = 0; cj < 2; ++cj) {
-                const int J = cj ? jb : ja;
-                if (cj && jb == ja) continue;
+                const int q = 4 * ci + 2 * ck + cj;
+                const int I = ci ? ib : ia, K = ck ? kb : ka, J = cj ? jb : ja;
+                const bool dup = (ci && ib == ia) || (ck && kb == ka) || (cj && jb == ja);
                 const long c = 3 * ((long)cny * cnz * I + (long)cny * K + J);
-                s0 += Uc[c];
-                s1 += Uc[c + 1];
-                s2 += Uc[c + 2];
+                on[q] = dup ? T(0) : T(1);
+                v[q][0] = Uc[c];
+                v[q][1] = Uc[c + 1];
+                v[q][2] = Uc[c + 2];
             }
-        }
+    const long d = 3 * nod;
+    const T u0 = Uin[d + 0], u1 = Uin[d + 1], u2 = Uin[d + 2];
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        s0 += on[q] * v[q][0];
+        s1 += on[q] * v[q][1];
+        s2 += on[q] * v[q][2];
     }
     const unsigned m = zero_fixed ? maskf[nod] : 0u;
-    const long d = 3 * nod;
-    Uout[d + 0] = (m & 1u) ? T(0) : Uin[d + 0] + w * s0;
-    Uout[d + 1] = (m & 2u) ? T(0) : Uin[d + 1] + w * s1;
-    Uout[d + 2] = (m & 4u) ? T(0) : Uin[d + 2] + w * s2;
+    Uout[d + 0] = (m & 1u) ? T(0) : u0 + w * s0;
+    Uout[d + 1] = (m & 2u) ? T(0) : u1 + w * s1;
+    Uout[d + 2] = (m & 4u) ? T(0) : u2 + w * s2;
 }
 
 template <class T>
