@@ -298,6 +298,207 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_l1_mf(OpArgsT<T
     }
 }
 
+// FP32 form: one thread per coarse element of the tile (and its one-element
+// halo) evaluates all 8 Gauss points in registers and writes its 24 nodal
+// forces to LDS; after one barrier every owned node sums the forces of its 8
+// elements in a fixed order.  No colouring, one barrier, deterministic.
+constexpr int kL1EwElems = 512;  // elements per block: (TJ+1)(TK+1)(TI+1) <= 512
+constexpr int kL1Ys = 25;        // force row stride (odd: conflict-free rows)
+
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<float> a, Tile t, const float *__restrict__ CXf,
+                                                 float lam2mu, float lam, float mu)
+{
+    if (a.ctl && a.ctl[0]) return;
+    __shared__ float su[3 * kL1Halo];
+    __shared__ float sy[kL1EwElems * kL1Ys];
+    const Grid g = a.g;  // the coarse (level-1) grid
+    int b = blockIdx.x;
+    const int bj = b % t.ntj;
+    b /= t.ntj;
+    const int bk = b % t.ntk;
+    const int bi = b / t.ntk;
+    const int j0 = bj * t.TJ, k0 = bk * t.TK, i0 = bi * t.TI;
+    const int HJ = t.TJ + 2, HK = t.TK + 2, HI = t.TI + 2;
+    const int EJ = t.TJ + 1, EK = t.TK + 1, EI = t.TI + 1;
+    const int nown = t.TJ * t.TK * t.TI;
+    const int nel = EJ * EK * EI;
+
+    // halo brick of u, as in k_l1_mf
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
+    const int rowlen = 3 * HJ;
+    const int nrows = HK * HI;
+    for (int r0 = wv; r0 < nrows; r0 += 4 * kStageRows) {
+        float v[kStageRows][2];
+#pragma unroll
+        for (int u = 0; u < kStageRows; ++u) {
+            const int row = r0 + 4 * u;
+            const int hi = row / HK, hk = row - hi * HK;
+            const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
+            const bool rowok = row < nrows && gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
+            const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const int col = ln + 64 * cc;
+                const int gj = j0 - 1 + col / 3;
+                const bool ok = rowok && col < rowlen && gj >= 0 && gj < g.nny;
+                const long dof = ok ? base + col : 0;
+                float x;
+                if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
+                    x = (float)a.prerelax * a.F[dof] / a.AD[dof];
+                else
+                    x = a.U[dof];
+                v[u][cc] = ok ? x : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStageRows; ++u) {
+            const int row = r0 + 4 * u;
+            if (row >= nrows) break;
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const int col = ln + 64 * cc;
+                if (col < rowlen) su[3 * row * HJ + col] = v[u][cc];
+            }
+        }
+    }
+    __syncthreads();
+
+    // element phase
+    const int fny = 2 * g.ny, fnz = 2 * g.nz;
+    const float g1 = 0.57735026918962576451f;  // 1/sqrt(3)
+    const float ih = 0.5f;                     // 1/h, h = 2; |J| w_g = h^3/8 = 1
+#pragma unroll 1
+    for (int q = threadIdx.x; q < nel; q += 256) {
+        const int ej = q % EJ, r = q / EJ;
+        const int ek = r % EK, ei = r / EK;
+        const int ex = i0 - 1 + ei, ey = j0 - 1 + ej, ez = k0 - 1 + ek;
+        float y[24];
+#pragma unroll
+        for (int m = 0; m < 24; ++m) y[m] = 0.0f;
+        if (ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz) {
+            float cx[8];
+#pragma unroll
+            for (int gp = 0; gp < 8; ++gp) {
+                const int is = gp >> 2, it = (gp >> 1) & 1, ie = gp & 1;
+                cx[gp] = CXf[(long)fny * fnz * (2 * ex + is) + (long)fny * (2 * ez + it) + 2 * ey + ie];
+            }
+            float u[2][2][2][3];  // [x][y][z][c]
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int yy = 0; yy < 2; ++yy)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) {
+                        const int h = 3 * (((ei + x) * HK + (ek + z)) * HJ + (ej + yy));
+                        u[x][yy][z][0] = su[h];
+                        u[x][yy][z][1] = su[h + 1];
+                        u[x][yy][z][2] = su[h + 2];
+                    }
+#pragma unroll
+            for (int gp = 0; gp < 8; ++gp) {
+                const int is = gp >> 2, it = (gp >> 1) & 1, ie = gp & 1;
+                const float sx = is ? g1 : -g1, sy = ie ? g1 : -g1, sz = it ? g1 : -g1;
+                const float wx[2] = {lane_w(0, sx), lane_w(1, sx)};
+                const float wy[2] = {lane_w(0, sy), lane_w(1, sy)};
+                const float wz[2] = {lane_w(0, sz), lane_w(1, sz)};
+                float G[3][3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+                    for (int p = 0; p < 2; ++p)
+#pragma unroll
+                        for (int q2 = 0; q2 < 2; ++q2) {
+                            gx = fmaf(wy[p] * wz[q2], u[1][p][q2][c] - u[0][p][q2][c], gx);
+                            gy = fmaf(wx[p] * wz[q2], u[p][1][q2][c] - u[p][0][q2][c], gy);
+                            gz = fmaf(wx[p] * wy[q2], u[p][q2][1][c] - u[p][q2][0][c], gz);
+                        }
+                    G[c][0] = ih * gx;
+                    G[c][1] = ih * gy;
+                    G[c][2] = ih * gz;
+                }
+                const float c0 = cx[gp];
+                const float tr = G[0][0] + G[1][1] + G[2][2];
+                float S[3][3];
+                S[0][0] = c0 * (lam * tr + (lam2mu - lam) * G[0][0]);
+                S[1][1] = c0 * (lam * tr + (lam2mu - lam) * G[1][1]);
+                S[2][2] = c0 * (lam * tr + (lam2mu - lam) * G[2][2]);
+                S[0][1] = S[1][0] = c0 * mu * (G[0][1] + G[1][0]);
+                S[1][2] = S[2][1] = c0 * mu * (G[1][2] + G[2][1]);
+                S[0][2] = S[2][0] = c0 * mu * (G[2][0] + G[0][2]);
+#pragma unroll
+                for (int n = 0; n < 8; ++n) {
+                    const int x = (n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0;
+                    const int yy = (n == 2 || n == 3 || n == 6 || n == 7) ? 1 : 0;
+                    const int z = n >= 4 ? 1 : 0;
+                    const float dx = (x ? ih : -ih) * wy[yy] * wz[z];
+                    const float dy = (yy ? ih : -ih) * wx[x] * wz[z];
+                    const float dz = (z ? ih : -ih) * wx[x] * wy[yy];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) y[3 * n + c] += dx * S[c][0] + dy * S[c][1] + dz * S[c][2];
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 24; ++m) sy[q * kL1Ys + m] = y[m];
+    }
+    __syncthreads();
+
+    // node phase: the forces of the 8 elements of each owned node
+    double part = 0.0;
+    const float om = (float)a.omega;
+    for (int own = threadIdx.x; own < nown; own += 256) {
+        const int tj = own % t.TJ;
+        const int r = own / t.TJ;
+        const int tk = r % t.TK, ti = r / t.TK;
+        const int gi = i0 + ti, gj = j0 + tj, gk = k0 + tk;
+        if (gi >= g.nnx || gj >= g.nny || gk >= g.nnz) continue;
+        float yv[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int x = 1; x >= 0; --x)
+#pragma unroll
+            for (int z = 1; z >= 0; --z)
+#pragma unroll
+                for (int yy = 1; yy >= 0; --yy) {
+                    // element with first node at halo (tj + 1 - yy, tk + 1 - z, ti + 1 - x)
+                    const int q = ((ti + 1 - x) * EK + (tk + 1 - z)) * EJ + (tj + 1 - yy);
+                    const int nloc = 4 * z + (yy ? (x ? 2 : 3) : (x ? 1 : 0));
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) yv[c] += sy[q * kL1Ys + 3 * nloc + c];
+                }
+        const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
+        const long dof = 3 * nod;
+        const unsigned msk = a.mask[nod];
+        const int hc = 3 * (((ti + 1) * HK + (tk + 1)) * HJ + (tj + 1));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const bool fixed = (msk >> c) & 1u;
+            const float u = su[hc + c];
+            if (MODE == OP_MATVEC) {
+                const float v = fixed ? 0.0f : yv[c];
+                a.out[dof + c] = v;
+                part += (double)u * (double)v;
+            } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
+                const float rv = fixed ? 0.0f : a.F[dof + c] - yv[c];
+                a.out[dof + c] = rv;
+                if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = u;
+            } else {
+                const float f = a.F[dof + c];
+                const float rv = fixed ? 0.0f : f - yv[c];
+                const float ad = a.AD[dof + c];
+                const float un = (1.0f - om) * u + om * (rv + ad * u) / ad;
+                a.out[dof + c] = un;
+                part += (double)f * (double)un;
+            }
+        }
+    }
+    if (a.partials) {
+        const double sm = block_sum(part);
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = sm;
+    }
+}
+
 static int l1_balance(int n, int tmax)
 {
     if (tmax >= n) return n;
@@ -311,7 +512,8 @@ Tile l1_tile(const Grid &g)
     t.TJ = l1_balance(g.nny, 7);
     t.TK = l1_balance(g.nnz, 7);
     t.TI = l1_balance(g.nnx, 7);
-    while ((t.TJ + 2) * (t.TK + 2) * (t.TI + 2) > kL1Halo || t.TJ * t.TK * t.TI > kL1Own) {
+    while ((t.TJ + 2) * (t.TK + 2) * (t.TI + 2) > kL1Halo || t.TJ * t.TK * t.TI > kL1Own ||
+           (t.TJ + 1) * (t.TK + 1) * (t.TI + 1) > kL1EwElems) {
         if (t.TI > 1)
             t.TI--;
         else if (t.TK > 1)
@@ -332,6 +534,17 @@ void launch_op1_mf(hipStream_t s, int mode, const OpArgsT<T> &a, const T *CXf, c
     const double c1 = 1.0 / ((1 + nu) * (1 - 2 * nu));
     const T l2m = T(c1 * (1 - nu)), lam = T(c1 * nu), mu = T(c1 * (1 - 2 * nu) / 2);
     const dim3 grid((unsigned)t.blocks()), block(256);
+    if constexpr (std::is_same<T, float>::value) {
+        switch (mode) {
+        case OP_MATVEC: k_l1_ew<OP_MATVEC><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        case OP_DEFECT: k_l1_ew<OP_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        case OP_JACOBI: k_l1_ew<OP_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        case OP_PRE_JACOBI: k_l1_ew<OP_PRE_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        case OP_PRE_DEFECT: k_l1_ew<OP_PRE_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        default: break;
+        }
+        return;
+    }
     switch (mode) {
     case OP_MATVEC: k_l1_mf<T, OP_MATVEC><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
     case OP_DEFECT: k_l1_mf<T, OP_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
