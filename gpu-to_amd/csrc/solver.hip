@@ -402,8 +402,11 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     hipStream_t s = h->stream;
     if (h->prm.precision != GPUTO_PREC_F64) mixed_begin(h);
     if (h->ln) learn_begin_solve(h);
-    // a trained learned stage makes the preconditioner vary: flexible beta
-    const bool flex = h->ln && h->ln->trained;
+    // a trained learned stage makes the preconditioner vary, and a level-0
+    // sweep schedule other than the preset's makes it less symmetric:
+    // flexible beta
+    const bool custom = h->prm.precision != GPUTO_PREC_F64 && (h->prm.mixed_npre0 > 0 || h->prm.mixed_npost0 > 0);
+    const bool flex = (h->ln && h->ln->trained) || custom;
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
@@ -520,6 +523,21 @@ long long gputo_est_workspace(int nfd, int nl, int nelx, int nely, int nelz)
 
 long long gputo_device_bytes(const gputo_handle *h) { return h ? h->bytes : 0; }
 
+// A level-0 schedule of the FP32 V-cycle must take an even number of
+// smoothing steps (the prerelaxation is the first pre-step): damped
+// Jacobi's error factor 1 - omega*lambda is negative on the highest modes,
+// so an odd count makes the preconditioner indefinite (measured: 1 + 2 and
+// 2 + 3 sweeps stall the CG at 48x24x24, 1 + 1, 2 + 2 and 1 + 3 do not).
+static bool schedule_ok(const gputo_params &p)
+{
+    if (p.mixed_npre0 < 0 || p.mixed_npost0 < 0) return false;
+    if (p.mixed_npre0 == 0 && p.mixed_npost0 == 0) return true;
+    const Preset ps = preset_of(p);
+    const int pre = p.mixed_npre0 > 0 ? p.mixed_npre0 : ps.npre;
+    const int post = p.mixed_npost0 > 0 ? p.mixed_npost0 : ps.npost;
+    return (std::max(pre, 1) + post) % 2 == 0;
+}
+
 int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *params, gputo_handle **out)
 {
     if (!out) return GPUTO_ERR_ARGUMENT;
@@ -534,6 +552,7 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
     if (!dense_ok && !(params && params->preset == GPUTO_PRESET_MGCG9)) return GPUTO_ERR_WORKSPACE;
     // the learned stage works on level 2 of the FP32 V-cycle
     if (params && params->learned && (params->precision == GPUTO_PREC_F64 || nl < 4)) return GPUTO_ERR_ARGUMENT;
+    if (params && !schedule_ok(*params)) return GPUTO_ERR_ARGUMENT;
     // a periodic cell keeps a master distinct from its images on every
     // level; the learned stage is trained on non-periodic defects only
     if (params && params->periodic && (gc.nx < 2 || gc.ny < 2 || gc.nz < 2 || params->learned))
@@ -649,6 +668,7 @@ int gputo_set_params(gputo_handle *h, const gputo_params *params)
     if (!h || !params) return GPUTO_ERR_ARGUMENT;
     if (!!params->learned != !!h->prm.learned) return GPUTO_ERR_ARGUMENT;  // fixed at creation
     if (!!params->periodic != !!h->prm.periodic) return GPUTO_ERR_ARGUMENT;
+    if (!schedule_ok(*params)) return GPUTO_ERR_ARGUMENT;
     if (params->precision != h->prm.precision && params->precision == GPUTO_PREC_F64 && h->ln)
         return GPUTO_ERR_ARGUMENT;  // the learned stage runs in the FP32 V-cycle
     if (!h->C.AQ && params->preset != GPUTO_PRESET_MGCG9) return GPUTO_ERR_WORKSPACE;  // no dense coarsest

@@ -176,6 +176,19 @@ void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T 
     }
 }
 
+// Sweeps before / after the coarse correction on level l: the preset's,
+// or on level 0 of the FP32 V-cycle the mixed_npre0 / mixed_npost0 params.
+template <class T>
+int sweeps_of(const gputo_handle *h, int l, bool post)
+{
+    const Preset p = preset_of(h->prm);
+    if (std::is_same<T, float>::value && l == 0) {
+        const int n = post ? h->prm.mixed_npost0 : h->prm.mixed_npre0;
+        if (n > 0) return n;
+    }
+    return post ? p.npost : p.npre;
+}
+
 // U0 = V(F0).
 template <class T>
 void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, const int *ctl)
@@ -192,17 +205,18 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
             if (H.per) launch_per_fold<T>(h->stream, lv.g, v.F, 0, ctl);
         }
         if (l < nl - 1) {
-            if (p.npre == 0 && !H.per) {
+            const int npre = sweeps_of<T>(h, l, false);
+            if (npre == 0 && !H.per) {
                 OpArgsT<T> a = view_args(h, H, l, ctl);
                 a.F = F;
                 a.out = v.R;
                 a.out2 = U;
                 level_op(h, H, l, OP_PRE_DEFECT, a);
             } else {
-                if (p.npre == 0)
+                if (npre == 0)
                     launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 1, ctl);
                 else
-                    sweeps_t(h, H, l, F, U, p.npre, true, ctl);
+                    sweeps_t(h, H, l, F, U, npre, true, ctl);
                 OpArgsT<T> a = view_args(h, H, l, ctl);
                 a.U = U;
                 a.F = F;
@@ -225,10 +239,11 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
         T *U = l == 0 ? U0 : v.U;
         // an odd number of post-sweeps starts from Tmp so the last one
         // writes U without a copy
-        const bool to_tmp = !H.per && p.npost % 2 == 1;
+        const int npost = sweeps_of<T>(h, l, true);
+        const bool to_tmp = !H.per && npost % 2 == 1;
         launch_interp_to<T>(h->stream, h->L[l].g, H.L[l + 1].U, U, to_tmp ? H.L[l].Tmp : U, h->L[l].mask,
                             p.zero_after_interp, ctl);
-        sweeps_t(h, H, l, F, U, p.npost, false, ctl, to_tmp);
+        sweeps_t(h, H, l, F, U, npost, false, ctl, to_tmp);
         if constexpr (std::is_same<T, float>::value) {
             if (H.learn && l == kLearnLevel) learn_stage(const_cast<gputo_handle *>(h), H, l, F, U, ctl);
         }

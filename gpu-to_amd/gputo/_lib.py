@@ -51,6 +51,8 @@ class GputoParams(ctypes.Structure):
         ("l1_kernel", ctypes.c_int),
         ("coarse_lag", ctypes.c_int),
         ("periodic", ctypes.c_int),
+        ("mixed_npre0", ctypes.c_int),
+        ("mixed_npost0", ctypes.c_int),
     ]
 
 

@@ -95,6 +95,11 @@ typedef struct gputo_params {
                           summed), a solution holds equal values on them.  Every
                           level needs at least 2 elements per direction; the
                           learned stage is not available */
+    int mixed_npre0;   /* FP32 V-cycle of the mixed precisions: Jacobi sweeps on level 0
+                          before / after the coarse correction (0: the preset's).  The
+                          preconditioner changes, not the solution: the FP64 CG still
+                          stops at the FP64 tolerance */
+    int mixed_npost0;
 } gputo_params;
 
 /* Fill *p with the defaults (REF_C, F64, no learned stage, operator

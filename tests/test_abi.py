@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from conftest import ROOT
 
 
@@ -58,3 +60,13 @@ def test_periodic_argument_checks(gputo):
     p.learned = 1
     assert lib.gputo_create(32, 32, 32, 4, ctypes.byref(p), ctypes.byref(h)) == _lib.GPUTO_ERR_ARGUMENT
     assert not h.value
+
+
+def test_odd_sweep_total_rejected(gputo):
+    """An odd number of level-0 smoothing steps (the prerelaxation counts as
+    the first pre-sweep) makes the V-cycle indefinite: Jacobi's error factor
+    is negative on the highest modes.  The handle refuses such schedules."""
+    with pytest.raises(gputo.GputoError):
+        gputo.Solver(16, 8, 8, 3, precision=2, l0_kernel=3, mixed_npre0=1, mixed_npost0=2)
+    with pytest.raises(gputo.GputoError):
+        gputo.Solver(16, 8, 8, 3, precision=2, l0_kernel=3, mixed_npre0=2, mixed_npost0=0)

@@ -73,6 +73,32 @@ def test_mixed_solve_matches_f64(gputo, oracle, prec, kernel, mesh):
     sm.close()
 
 
+@pytest.mark.parametrize("kernel", [3, 1])
+@pytest.mark.parametrize("sweeps", [(1, 3), (2, 2), (1, 1)])
+def test_level0_sweep_schedule(gputo, oracle, sweeps, kernel):
+    """Fewer level-0 sweeps in the FP32 V-cycle change the preconditioner,
+    not the solution: the FP64 true residual meets the tolerance and the
+    compliance matches the FP64 solve."""
+    nelx, nely, nelz, nl = 48, 24, 24, 4
+    print("case", sweeps, kernel, flush=True)
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=23, kind="perturbed")
+    s64 = gputo.Solver(nelx, nely, nelz, nl)
+    U64, r64, n64, rc = s64.solve(Ke, F, CX, fixed, 1e-10, 300)
+    assert rc == 0
+    sm = gputo.Solver(nelx, nely, nelz, nl, precision=2, l0_kernel=kernel, mixed_npre0=sweeps[0],
+                      mixed_npost0=sweeps[1])
+    Um, rm, nm, rc = sm.solve(Ke, F, CX, fixed, 1e-10, 300)
+    print("iterations", n64, nm, flush=True)
+    assert rc == 0 and rm < 1e-10 and nm <= n64 + 10
+    Uc = Um.copy()
+    Uc[1::3] *= -1
+    r = oracle.defect0(Ke, F, Uc, CX, fixed, nelx, nely, nelz)
+    assert np.linalg.norm(r) / np.linalg.norm(F) < 1e-10
+    assert abs(F @ Um - F @ U64) <= 1e-9 * abs(F @ U64)
+    s64.close()
+    sm.close()
+
+
 def test_lagged_coarse_factor(gputo, oracle):
     """Mixed solves of a sequence of slowly changing designs with the
     coarsest factorisation lagged by one setup reach the FP64 solution of each
