@@ -24,8 +24,8 @@ FP32_PEAK_TFLOPS = 157.3
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "level0_operator_cantilever_192x96x96_pmc.md")
 # variant -> (precision, level-0 kernel of the inner solve); kernel ids as
 # gputo_params.l0_kernel (0 = FP64 node-gather for the plain path)
-# name: (precision, level-0 kernel, level-0 sweeps before / after the coarse
-# correction in the FP32 V-cycle, 0 = the preset's 3 / 3)
+# name: (precision, level-0 kernel, sweeps before / after the coarse
+# correction on levels 0 and 1 of the FP32 V-cycle, 0 = the preset's 3 / 3)
 VARIANTS = {
     "f64": (0, 0, 0, 0),
     "mixed_gather": (1, 1, 0, 0),  # FP32 inner solve, FP32 node-gather level-0 operator
@@ -33,7 +33,7 @@ VARIANTS = {
     "mixed_mfma": (2, 4, 0, 0),    # MFMA level-0 operator, three-way BF16 split, six products
     "mixed_mfma3": (2, 3, 0, 0),   # MFMA level-0 operator, two-way BF16 split, three products
     "mixed_mfma1": (2, 5, 0, 0),   # MFMA level-0 operator, one BF16 product
-    "mixed_mfma3s": (2, 3, 1, 3),  # as mixed_mfma3 with 1 + 3 level-0 sweeps
+    "mixed_mfma3s": (2, 3, 1, 3),  # as mixed_mfma3 with 1 + 3 sweeps on levels 0 and 1
 }
 
 
@@ -54,8 +54,8 @@ def pmc_traffic(kernel):
 
 
 def level0_sweeps(name):
-    """'pre+post' Jacobi sweeps on level 0 of a mixed variant's FP32 V-cycle
-    (None for the FP64 variant, whose V-cycle is the preset's)."""
+    """'pre+post' Jacobi sweeps on levels 0 and 1 of a mixed variant's FP32
+    V-cycle (None for the FP64 variant, whose V-cycle is the preset's)."""
     prec, _, npre0, npost0 = VARIANTS[name.replace("+learned", "")]
     if prec == 0:
         return None
@@ -70,7 +70,8 @@ def run_variant(gputo, name, args, Ke, F, fixed, x0):
     prec, kernel, npre0, npost0 = VARIANTS[name.replace("+learned", "")]
     learned = name.endswith("+learned")
     s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=learned,
-                     l0_kernel=kernel, mixed_npre0=npre0, mixed_npost0=npost0)
+                     l0_kernel=kernel, mixed_npre0=npre0, mixed_npost0=npost0, mixed_npre1=npre0,
+                     mixed_npost1=npost0)
     run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
                        cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
     train_s = 0.0
@@ -259,7 +260,7 @@ def main():
                         "seed %d, density filter rmin=sqrt(3), OC move 0.2, preset ref_c"
                         % (args.nelx, args.nely, args.nelz, nel, ndof, args.nl, args.cg_tol, args.seed),
             "variant": head,
-            "fp32_level0_sweeps": level0_sweeps(head),
+            "fp32_level01_sweeps": level0_sweeps(head),
         },
         "roofline": roof,
         "matvec": {

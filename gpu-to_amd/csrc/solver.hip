@@ -387,6 +387,30 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
     return 0;
 }
 
+// A level-0 or level-1 schedule of the FP32 V-cycle must take an even number of
+// smoothing steps (the prerelaxation is the first pre-step): damped
+// Jacobi's error factor 1 - omega*lambda is negative on the highest modes,
+// so an odd count makes the preconditioner indefinite (measured: 1 + 2 and
+// 2 + 3 sweeps stall the CG at 48x24x24, 1 + 1, 2 + 2 and 1 + 3 do not).
+static bool schedule_ok(const gputo_params &p)
+{
+    const Preset ps = preset_of(p);
+    for (int l = 0; l < 2; ++l) {
+        const int a = mixed_sweeps(p, l, false), b = mixed_sweeps(p, l, true);
+        if (a < 0 || b < 0) return false;
+        if (a == 0 && b == 0) continue;
+        const int pre = a > 0 ? a : ps.npre, post = b > 0 ? b : ps.npost;
+        if ((std::max(pre, 1) + post) % 2) return false;
+    }
+    return true;
+}
+
+static bool custom_schedule(const gputo_params &p)
+{
+    return p.precision != GPUTO_PREC_F64 &&
+           (p.mixed_npre0 > 0 || p.mixed_npost0 > 0 || p.mixed_npre1 > 0 || p.mixed_npost1 > 0);
+}
+
 // Device-resident PCG (mgcg1.cu:235-280); U, F on the device.
 // z = M r: the FP64 V-cycle, or the FP32 V-cycle of the mixed precisions
 static void precond(gputo_handle *h, const double *r, double *z)
@@ -405,8 +429,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     // a trained learned stage makes the preconditioner vary, and a level-0
     // sweep schedule other than the preset's makes it less symmetric:
     // flexible beta
-    const bool custom = h->prm.precision != GPUTO_PREC_F64 && (h->prm.mixed_npre0 > 0 || h->prm.mixed_npost0 > 0);
-    const bool flex = (h->ln && h->ln->trained) || custom;
+    const bool flex = (h->ln && h->ln->trained) || custom_schedule(h->prm);
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
@@ -523,20 +546,6 @@ long long gputo_est_workspace(int nfd, int nl, int nelx, int nely, int nelz)
 
 long long gputo_device_bytes(const gputo_handle *h) { return h ? h->bytes : 0; }
 
-// A level-0 schedule of the FP32 V-cycle must take an even number of
-// smoothing steps (the prerelaxation is the first pre-step): damped
-// Jacobi's error factor 1 - omega*lambda is negative on the highest modes,
-// so an odd count makes the preconditioner indefinite (measured: 1 + 2 and
-// 2 + 3 sweeps stall the CG at 48x24x24, 1 + 1, 2 + 2 and 1 + 3 do not).
-static bool schedule_ok(const gputo_params &p)
-{
-    if (p.mixed_npre0 < 0 || p.mixed_npost0 < 0) return false;
-    if (p.mixed_npre0 == 0 && p.mixed_npost0 == 0) return true;
-    const Preset ps = preset_of(p);
-    const int pre = p.mixed_npre0 > 0 ? p.mixed_npre0 : ps.npre;
-    const int post = p.mixed_npost0 > 0 ? p.mixed_npost0 : ps.npost;
-    return (std::max(pre, 1) + post) % 2 == 0;
-}
 
 int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *params, gputo_handle **out)
 {

@@ -176,14 +176,22 @@ void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T 
     }
 }
 
+// Level-l sweeps of the FP32 V-cycle set by the params (0: the preset's).
+inline int mixed_sweeps(const gputo_params &prm, int l, bool post)
+{
+    if (l == 0) return post ? prm.mixed_npost0 : prm.mixed_npre0;
+    if (l == 1) return post ? prm.mixed_npost1 : prm.mixed_npre1;
+    return 0;
+}
+
 // Sweeps before / after the coarse correction on level l: the preset's,
-// or on level 0 of the FP32 V-cycle the mixed_npre0 / mixed_npost0 params.
+// or on levels 0 and 1 of the FP32 V-cycle the mixed_npre/npost params.
 template <class T>
 int sweeps_of(const gputo_handle *h, int l, bool post)
 {
     const Preset p = preset_of(h->prm);
-    if (std::is_same<T, float>::value && l == 0) {
-        const int n = post ? h->prm.mixed_npost0 : h->prm.mixed_npre0;
+    if (std::is_same<T, float>::value) {
+        const int n = mixed_sweeps(h->prm, l, post);
         if (n > 0) return n;
     }
     return post ? p.npost : p.npre;

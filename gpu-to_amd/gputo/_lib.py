@@ -53,6 +53,8 @@ class GputoParams(ctypes.Structure):
         ("periodic", ctypes.c_int),
         ("mixed_npre0", ctypes.c_int),
         ("mixed_npost0", ctypes.c_int),
+        ("mixed_npre1", ctypes.c_int),
+        ("mixed_npost1", ctypes.c_int),
     ]
 
 

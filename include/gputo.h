@@ -98,8 +98,11 @@ typedef struct gputo_params {
     int mixed_npre0;   /* FP32 V-cycle of the mixed precisions: Jacobi sweeps on level 0
                           before / after the coarse correction (0: the preset's).  The
                           preconditioner changes, not the solution: the FP64 CG still
-                          stops at the FP64 tolerance */
+                          stops at the FP64 tolerance.  With the prerelaxation counted as
+                          the first pre-sweep the total must be even (-5 otherwise) */
     int mixed_npost0;
+    int mixed_npre1;   /* the same for level 1 */
+    int mixed_npost1;
 } gputo_params;
 
 /* Fill *p with the defaults (REF_C, F64, no learned stage, operator

@@ -70,3 +70,5 @@ def test_odd_sweep_total_rejected(gputo):
         gputo.Solver(16, 8, 8, 3, precision=2, l0_kernel=3, mixed_npre0=1, mixed_npost0=2)
     with pytest.raises(gputo.GputoError):
         gputo.Solver(16, 8, 8, 3, precision=2, l0_kernel=3, mixed_npre0=2, mixed_npost0=0)
+    with pytest.raises(gputo.GputoError):
+        gputo.Solver(16, 8, 8, 3, precision=2, l0_kernel=3, mixed_npre1=1, mixed_npost1=2)
