@@ -25,15 +25,17 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "level0_operator_cantilever_192x96x
 # variant -> (precision, level-0 kernel of the inner solve); kernel ids as
 # gputo_params.l0_kernel (0 = FP64 node-gather for the plain path)
 # name: (precision, level-0 kernel, sweeps before / after the coarse
-# correction on levels 0 and 1 of the FP32 V-cycle, 0 = the preset's 3 / 3)
+# correction on levels 0 and 1 of the FP32 V-cycle (0 = the preset's 3 / 3),
+# coarse_lag: V-cycles that use the previous design's coarsest factor)
 VARIANTS = {
-    "f64": (0, 0, 0, 0),
-    "mixed_gather": (1, 1, 0, 0),  # FP32 inner solve, FP32 node-gather level-0 operator
-    "mixed_mfma9": (2, 2, 0, 0),   # MFMA level-0 operator, three-way BF16 split, all nine products
-    "mixed_mfma": (2, 4, 0, 0),    # MFMA level-0 operator, three-way BF16 split, six products
-    "mixed_mfma3": (2, 3, 0, 0),   # MFMA level-0 operator, two-way BF16 split, three products
-    "mixed_mfma1": (2, 5, 0, 0),   # MFMA level-0 operator, one BF16 product
-    "mixed_mfma3s": (2, 3, 1, 3),  # as mixed_mfma3 with 1 + 3 sweeps on levels 0 and 1
+    "f64": (0, 0, 0, 0, 0),
+    "mixed_gather": (1, 1, 0, 0, 0),  # FP32 inner solve, FP32 node-gather level-0 operator
+    "mixed_mfma9": (2, 2, 0, 0, 0),   # MFMA level-0 operator, three-way BF16 split, all nine products
+    "mixed_mfma": (2, 4, 0, 0, 0),    # MFMA level-0 operator, three-way BF16 split, six products
+    "mixed_mfma3": (2, 3, 0, 0, 0),   # MFMA level-0 operator, two-way BF16 split, three products
+    "mixed_mfma1": (2, 5, 0, 0, 0),   # MFMA level-0 operator, one BF16 product
+    "mixed_mfma3s": (2, 3, 1, 3, 0),  # as mixed_mfma3 with 1 + 3 sweeps on levels 0 and 1
+    "mixed_mfma3s3": (2, 3, 1, 3, 3),  # and the previous coarsest factor in the first 3 V-cycles
 }
 
 
@@ -56,7 +58,7 @@ def pmc_traffic(kernel):
 def level0_sweeps(name):
     """'pre+post' Jacobi sweeps on levels 0 and 1 of a mixed variant's FP32
     V-cycle (None for the FP64 variant, whose V-cycle is the preset's)."""
-    prec, _, npre0, npost0 = VARIANTS[name.replace("+learned", "")]
+    prec, _, npre0, npost0, _ = VARIANTS[name.replace("+learned", "")]
     if prec == 0:
         return None
     return "%d+%d" % (npre0 or 3, npost0 or 3)
@@ -67,11 +69,11 @@ def log(*a):
 
 
 def run_variant(gputo, name, args, Ke, F, fixed, x0):
-    prec, kernel, npre0, npost0 = VARIANTS[name.replace("+learned", "")]
+    prec, kernel, npre0, npost0, lag = VARIANTS[name.replace("+learned", "")]
     learned = name.endswith("+learned")
     s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=learned,
                      l0_kernel=kernel, mixed_npre0=npre0, mixed_npost0=npost0, mixed_npre1=npre0,
-                     mixed_npost1=npost0)
+                     mixed_npost1=npost0, coarse_lag=lag)
     run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
                        cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
     train_s = 0.0

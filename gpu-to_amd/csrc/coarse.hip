@@ -897,6 +897,16 @@ void coarse_adopt(Coarse &C)
     C.have = true;
 }
 
+void coarse_switch(Coarse &C, hipStream_t s)
+{
+    if (!C.pending || C.solve_waits) return;  // no lagged factorisation in flight
+    HIPCHK(hipStreamWaitEvent(s, C.ev_done, 0));
+    C.cur = C.target;
+    C.Linv = C.Lbuf[C.cur];
+    C.LinvT = C.LTbuf[C.cur];
+    C.solve_waits = true;  // coarse_check reads its SPD flag when the solve ends
+}
+
 // ---------------------------------------------------------------------------
 // Coarsest solve x = L^-T L^-1 b as two row-parallel triangular products
 // (one 64-lane wave per row, fixed reduction order), then x[fixed] = 0

@@ -218,6 +218,9 @@ void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ);
 void coarse_factor(hipStream_t s, Coarse &C, bool lag);
 int coarse_check(Coarse &C);  // waits for an exact factorisation; GPUTO_ERR_NOT_SPD on a bad pivot
 void coarse_adopt(Coarse &C);  // before a setup: take a finished lagged factor
+// mid-solve: make the solves on s wait for the factorisation in flight and
+// use it from now on (no-op without a lagged factorisation)
+void coarse_switch(Coarse &C, hipStream_t s);
 template <class T>
 void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl);
 

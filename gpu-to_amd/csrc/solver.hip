@@ -295,7 +295,7 @@ static void ensure_tables(gputo_handle *h)
 
 // Build the hierarchy for the current Ke, CX (device) and fixed list
 // (initVcycle, mgcg1.cu:326-466).
-static bool coarse_lagged(const gputo_params &p) { return p.coarse_lag == 1; }
+static bool coarse_lagged(const gputo_params &p) { return p.coarse_lag >= 1; }
 
 int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
 {
@@ -429,7 +429,16 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     // a trained learned stage makes the preconditioner vary, and a level-0
     // sweep schedule other than the preset's makes it less symmetric:
     // flexible beta
-    const bool flex = (h->ln && h->ln->trained) || custom_schedule(h->prm);
+    // a factor switched in mid-solve (coarse_lag >= 2) changes it too
+    const bool flex = (h->ln && h->ln->trained) || custom_schedule(h->prm) || h->prm.coarse_lag >= 2;
+    // coarse_lag = n >= 2: the first n V-cycles use the previous setup's
+    // coarsest factor, later ones the factor of this setup
+    int vcycles = 0;
+    auto precond_n = [&](const double *r, double *z) {
+        if (h->prm.coarse_lag >= 2 && vcycles == h->prm.coarse_lag) coarse_switch(h->C, s);
+        precond(h, r, z);
+        ++vcycles;
+    };
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
@@ -440,7 +449,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     if (per) launch_per_fold<double>(s, L0.g, h->R, 0, nullptr);  // images' share to the masters
     launch_dot_partial(s, h->R, h->R, n, h->partials, nullptr);
     launch_cg_init(s, h->sc, h->partials, kRedBlocks);
-    precond(h, h->R, h->P);
+    precond_n(h->R, h->P);
     launch_copy(s, h->P, h->Z, n, ctl);
     launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
     launch_cg_rho0(s, h->sc, h->partials, kRedBlocks);
@@ -460,7 +469,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
             launch_cg_alpha(s, h->sc, h->partials, nmv);
             launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials);
             launch_cg_check(s, h->sc, h->partials, kRedBlocks, tol, maxit);
-            precond(h, h->R, h->Z);
+            precond_n(h->R, h->Z);
             if (flex) {
                 launch_cg_beta_flex(s, h->R, h->Z, h->Q, n, h->sc, h->partials, maxit);
             } else {

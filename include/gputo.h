@@ -86,7 +86,11 @@ typedef struct gputo_params {
                           previous setup (the first setup, and any setup after a
                           parameter change, waits for its own factor).  Lagging
                           hides the factorisation but costs iterations when the
-                          design changes fast (early design iterations) */
+                          design changes fast (early design iterations).
+                          n >= 2: the first n V-cycles of a solve use the previous
+                          factor, the later ones wait for this setup's (flexible CG;
+                          the switch is at a fixed V-cycle, so results do not
+                          depend on timing) */
     int periodic;      /* 1: periodic unit cell.  The mesh wraps in x, y and z: the
                           nodes of the faces i = nelx, j = nely, k = nelz are the
                           images of those at 0.  Vectors keep the

@@ -99,14 +99,15 @@ def test_level0_sweep_schedule(gputo, oracle, sweeps, kernel):
     sm.close()
 
 
-def test_lagged_coarse_factor(gputo, oracle):
+@pytest.mark.parametrize("lag_mode", [1, 2])
+def test_lagged_coarse_factor(gputo, oracle, lag_mode):
     """Mixed solves of a sequence of slowly changing designs with the
-    coarsest factorisation lagged by one setup reach the FP64 solution of each
-    design to the CG tolerance, in about the iteration count of the exact
-    factorisation."""
+    coarsest factorisation lagged by one setup (for the whole solve, or for
+    its first 2 V-cycles) reach the FP64 solution of each design to the CG
+    tolerance, in about the iteration count of the exact factorisation."""
     nelx, nely, nelz, nl = 48, 24, 24, 4
     Ke, F, fixed = gputo.cantilever(nelx, nely, nelz)
-    lag = gputo.Solver(nelx, nely, nelz, nl, precision=1, coarse_lag=1)
+    lag = gputo.Solver(nelx, nely, nelz, nl, precision=1, coarse_lag=lag_mode)
     exact = gputo.Solver(nelx, nely, nelz, nl, precision=1)
     s64 = gputo.Solver(nelx, nely, nelz, nl)
     for k in range(4):
@@ -121,7 +122,7 @@ def test_lagged_coarse_factor(gputo, oracle):
         assert rc == 0 and rl < 1e-10
         print("iterations", n64, ne, nlag, flush=True)
         assert rel(Ul, U64) < 1e-7 and rel(Ue, U64) < 1e-7
-        if k == 0:
+        if k == 0 and lag_mode == 1:
             assert np.array_equal(Ul, Ue)  # first setup: no earlier factor, exact
         assert nlag <= ne + 4
     for s in (lag, exact, s64):
