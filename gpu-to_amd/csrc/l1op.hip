@@ -10,11 +10,12 @@
 // g_1 = +1/sqrt3 (the Gauss-point / child pairing of coarseAe2,
 // Cantilever/omp/mgcg1.c:1539-1584).
 //
-// Eight lanes work on one coarse element, one Gauss point each; their 24
-// nodal contributions are summed by a fixed xor butterfly that leaves each
-// lane the 3 values of one node, which it adds into LDS node accumulators.
-// Elements are visited in 8-colour order (no two elements of a colour share
-// a node), so the accumulation is conflict-free and deterministic.
+// FP64 form (k_l1_mf): eight lanes work on one coarse element, one Gauss
+// point each; their 24 nodal contributions are summed by a fixed xor
+// butterfly that leaves each lane the 3 values of one node, which it adds
+// into LDS node accumulators.  Elements are visited in 8-colour order (no
+// two elements of a colour share a node), so the accumulation is
+// conflict-free and deterministic.  The FP32 form is k_l1_ew below.
 #include "vcycle.h"
 
 namespace gputo {
@@ -543,15 +544,15 @@ void launch_op1_mf(hipStream_t s, int mode, const OpArgsT<T> &a, const T *CXf, c
         case OP_PRE_DEFECT: k_l1_ew<OP_PRE_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
         default: break;
         }
-        return;
-    }
-    switch (mode) {
-    case OP_MATVEC: k_l1_mf<T, OP_MATVEC><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-    case OP_DEFECT: k_l1_mf<T, OP_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-    case OP_JACOBI: k_l1_mf<T, OP_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-    case OP_PRE_JACOBI: k_l1_mf<T, OP_PRE_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-    case OP_PRE_DEFECT: k_l1_mf<T, OP_PRE_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-    default: break;
+    } else {
+        switch (mode) {
+        case OP_MATVEC: k_l1_mf<T, OP_MATVEC><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        case OP_DEFECT: k_l1_mf<T, OP_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        case OP_JACOBI: k_l1_mf<T, OP_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        case OP_PRE_JACOBI: k_l1_mf<T, OP_PRE_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        case OP_PRE_DEFECT: k_l1_mf<T, OP_PRE_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+        default: break;
+        }
     }
 }
 template void launch_op1_mf<double>(hipStream_t, int, const OpArgsT<double> &, const double *, const Tile &, double);
