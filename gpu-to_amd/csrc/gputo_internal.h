@@ -101,7 +101,8 @@ struct CgScalars {
 struct Level {
     Grid g;
     Tile t;
-    Tile tmf;                     // tiling of the matrix-free level-1 operator
+    Tile tmf;                     // tiling of the matrix-free level-1 operator (FP32)
+    Tile tmf64;                   // its FP64 tiling (half the elements per block)
     Tile tg;                      // node-gather tiling of this level (matrix-free levels)
     double *CXl = nullptr;        // element scales of a matrix-free coarse level (mgcg9)
     double *U = nullptr, *F = nullptr, *R = nullptr, *AD = nullptr, *T = nullptr;

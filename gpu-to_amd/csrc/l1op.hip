@@ -10,28 +10,13 @@
 // g_1 = +1/sqrt3 (the Gauss-point / child pairing of coarseAe2,
 // Cantilever/omp/mgcg1.c:1539-1584).
 //
-// FP64 form (k_l1_mf): eight lanes work on one coarse element, one Gauss
-// point each; their 24 nodal contributions are summed by a fixed xor
-// butterfly that leaves each lane the 3 values of one node, which it adds
-// into LDS node accumulators.  Elements are visited in 8-colour order (no
-// two elements of a colour share a node), so the accumulation is
-// conflict-free and deterministic.  The FP32 form is k_l1_ew below.
+// One thread per coarse element evaluates the 8 Gauss points (k_l1_ew
+// below); the nodal forces meet in LDS.
 #include "vcycle.h"
 
 namespace gputo {
 
-constexpr int kL1Halo = 1024;  // halo nodes per block
-constexpr int kL1Own = 512;    // owned nodes per block
-
-// q / d for 0 <= q < 2^22, d >= 1, with inv = 1/d: the float quotient is
-// within one of the true one and is corrected both ways
-__device__ __forceinline__ int udiv_small(int q, int d, float inv)
-{
-    int r = (int)((float)q * inv);
-    r -= (r * d > q) ? 1 : 0;
-    r += ((r + 1) * d <= q) ? 1 : 0;
-    return r;
-}
+constexpr int kL1Own = 512;  // owned nodes per block
 
 template <class T>
 __device__ __forceinline__ T lane_w(int x, T xi)
@@ -40,13 +25,24 @@ __device__ __forceinline__ T lane_w(int x, T xi)
     return x ? T(0.5) * (T(1) + xi) : T(0.5) * (T(1) - xi);
 }
 
+// Element-per-thread form: one thread per coarse element of the tile (and
+// its one-element halo) evaluates all 8 Gauss points in registers and writes
+// its 24 nodal forces to LDS; after one barrier every owned node sums the
+// forces of its 8 elements in a fixed order.  No colouring, one barrier,
+// deterministic.  HALO nodes and NE elements per block: 1024 and 512 in
+// FP32, 512 and 256 in FP64 (63.5 KB of LDS either way, two blocks per CU).
+constexpr int kL1Ys = 25;  // force row stride (odd: conflict-free rows)
+template <class T> struct L1Ew;
+template <> struct L1Ew<float> { static constexpr int HALO = 1024, NE = 512; };
+template <> struct L1Ew<double> { static constexpr int HALO = 512, NE = 256; };
+
 template <class T, int MODE>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_l1_mf(OpArgsT<T> a, Tile t, const T *__restrict__ CXf, T lam2mu, T lam,
-                                              T mu)
+__global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<T> a, Tile t, const T *__restrict__ CXf, T lam2mu,
+                                                 T lam, T mu)
 {
     if (a.ctl && a.ctl[0]) return;
-    __shared__ T su[3 * kL1Halo];
-    __shared__ T acc[3 * kL1Own];
+    __shared__ T su[3 * L1Ew<T>::HALO];
+    __shared__ T sy[L1Ew<T>::NE * kL1Ys];
     const Grid g = a.g;  // the coarse (level-1) grid
     int b = blockIdx.x;
     const int bj = b % t.ntj;
@@ -57,12 +53,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_l1_mf(OpArgsT<T
     const int HJ = t.TJ + 2, HK = t.TK + 2, HI = t.TI + 2;
     const int EJ = t.TJ + 1, EK = t.TK + 1, EI = t.TI + 1;
     const int nown = t.TJ * t.TK * t.TI;
+    const int nel = EJ * EK * EI;
 
-    // halo brick, one j-row per wave (coalesced rows, wave-uniform row split);
-    // kStageRows rows per wave are loaded before any is stored, so their loads
-    // are in flight together (addresses outside the mesh clamped and masked)
+    // halo brick of u, as in k_l1_mf
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
-    const int rowlen = 3 * HJ;  // <= 128
+    const int rowlen = 3 * HJ;
     const int nrows = HK * HI;
     for (int r0 = wv; r0 < nrows; r0 += 4 * kStageRows) {
         T v[kStageRows][2];
@@ -98,293 +93,28 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_l1_mf(OpArgsT<T
             }
         }
     }
-    for (int q = threadIdx.x; q < 3 * nown; q += 256) acc[q] = T(0);
-    __syncthreads();
-
-    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 3;  // 32 groups of 8 lanes
-    const int gp = lane & 7;                                       // Gauss point of this lane
-    const int is = gp >> 2, it = (gp >> 1) & 1, ie = gp & 1;
-    const T g1 = T(0.57735026918962576451);                        // 1/sqrt(3)
-    const T sx = is ? g1 : -g1, sy = ie ? g1 : -g1, sz = it ? g1 : -g1;
-    const T wx[2] = {lane_w(0, sx), lane_w(1, sx)};
-    const T wy[2] = {lane_w(0, sy), lane_w(1, sy)};
-    const T wz[2] = {lane_w(0, sz), lane_w(1, sz)};
-    const T ih = T(0.5);  // 1/h, h = 2; |J| w_g = h^3/8 = 1
-    const int fny = 2 * g.ny, fnz = 2 * g.nz;
-
-    for (int c = 0; c < 8; ++c) {
-        const int pj = c & 1, pk = (c >> 1) & 1, pi = (c >> 2) & 1;
-        const int nj = (EJ - pj + 1) / 2, nk = (EK - pk + 1) / 2, ni = (EI - pi + 1) / 2;
-        const int cnt = nj * nk * ni;
-        const float inv_nj = 1.0f / (float)nj, inv_nk = 1.0f / (float)nk;
-        // element of batch q0 for this lane group, and its Gauss point's
-        // fine-element scale (loaded one batch ahead)
-        auto elem = [&](int q, int &ej, int &ek, int &ei, long &cxi) {
-            ej = ek = ei = 0;
-            cxi = 0;
-            if (q >= cnt) return false;
-            const int r = udiv_small(q, nj, inv_nj);
-            ej = pj + 2 * (q - r * nj);
-            const int r2 = udiv_small(r, nk, inv_nk);
-            ek = pk + 2 * (r - r2 * nk);
-            ei = pi + 2 * r2;
-            const int ex = i0 - 1 + ei, ey = j0 - 1 + ej, ez = k0 - 1 + ek;
-            if (!(ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz)) return false;
-            cxi = (long)fny * fnz * (2 * ex + is) + (long)fny * (2 * ez + it) + 2 * ey + ie;
-            return true;
-        };
-        // FP32: the scale is loaded one batch ahead; FP64 keeps its registers
-        // for the element arithmetic (two waves per SIMD otherwise)
-        constexpr bool kAhead = sizeof(T) == 4;
-        int nej, nek, nei;
-        long ncxi;
-        bool nvalid = elem(grp, nej, nek, nei, ncxi);
-        T ncx = kAhead ? CXf[ncxi] : T(0);
-        for (int q0 = 0; q0 < cnt; q0 += 32) {
-            const int ej = nej, ek = nek, ei = nei;
-            const bool valid = nvalid;
-            const T cx = kAhead ? ncx : CXf[ncxi];
-            nvalid = elem(q0 + 32 + grp, nej, nek, nei, ncxi);
-            if (kAhead) ncx = CXf[ncxi];
-            T y[24];
-            if (valid) {
-                // nodal values u[x][y][z][c] of the element
-                T u[2][2][2][3];
-#pragma unroll
-                for (int x = 0; x < 2; ++x)
-#pragma unroll
-                    for (int yy = 0; yy < 2; ++yy)
-#pragma unroll
-                        for (int z = 0; z < 2; ++z) {
-                            const int h = 3 * (((ei + x) * HK + (ek + z)) * HJ + (ej + yy));
-                            u[x][yy][z][0] = su[h];
-                            u[x][yy][z][1] = su[h + 1];
-                            u[x][yy][z][2] = su[h + 2];
-                        }
-                // displacement gradient G[c][d] at this Gauss point
-                T G[3][3];
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) {
-                    T gx = T(0), gy = T(0), gz = T(0);
-#pragma unroll
-                    for (int p = 0; p < 2; ++p)
-#pragma unroll
-                        for (int q2 = 0; q2 < 2; ++q2) {
-                            gx = fma(wy[p] * wz[q2], u[1][p][q2][cc] - u[0][p][q2][cc], gx);
-                            gy = fma(wx[p] * wz[q2], u[p][1][q2][cc] - u[p][0][q2][cc], gy);
-                            gz = fma(wx[p] * wy[q2], u[p][q2][1][cc] - u[p][q2][0][cc], gz);
-                        }
-                    G[cc][0] = ih * gx;
-                    G[cc][1] = ih * gy;
-                    G[cc][2] = ih * gz;
-                }
-                // stress S = cx D eps (isotropic, engineering shear strains)
-                const T tr = G[0][0] + G[1][1] + G[2][2];
-                T S[3][3];
-                S[0][0] = cx * (lam * tr + (lam2mu - lam) * G[0][0]);
-                S[1][1] = cx * (lam * tr + (lam2mu - lam) * G[1][1]);
-                S[2][2] = cx * (lam * tr + (lam2mu - lam) * G[2][2]);
-                S[0][1] = S[1][0] = cx * mu * (G[0][1] + G[1][0]);
-                S[1][2] = S[2][1] = cx * mu * (G[1][2] + G[2][1]);
-                S[0][2] = S[2][0] = cx * mu * (G[2][0] + G[0][2]);
-                // nodal forces y_a^c = sum_d dN_a/dx_d S_cd, node order ie/je/ke
-#pragma unroll
-                for (int n = 0; n < 8; ++n) {
-                    const int x = (n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0;
-                    const int yy = (n == 2 || n == 3 || n == 6 || n == 7) ? 1 : 0;
-                    const int z = n >= 4 ? 1 : 0;
-                    const T dx = (x ? ih : -ih) * wy[yy] * wz[z];
-                    const T dy = (yy ? ih : -ih) * wx[x] * wz[z];
-                    const T dz = (z ? ih : -ih) * wx[x] * wy[yy];
-#pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) y[3 * n + cc] = dx * S[cc][0] + dy * S[cc][1] + dz * S[cc][2];
-                }
-            } else {
-#pragma unroll
-                for (int m = 0; m < 24; ++m) y[m] = T(0);
-            }
-            // sum over the 8 Gauss-point lanes, reduce-scatter by lane bits:
-            // bit 2 splits nodes {0..3 | 4..7}, bit 1 {0,1 | 2,3}, bit 0 {0 | 1}
-            T h12[12];
-            {
-                const bool hi = (gp >> 2) & 1;
-#pragma unroll
-                for (int m = 0; m < 12; ++m) {
-                    const T keep = hi ? y[12 + m] : y[m];
-                    const T send = hi ? y[m] : y[12 + m];
-                    h12[m] = keep + __shfl_xor(send, 4, 64);
-                }
-            }
-            T h6[6];
-            {
-                const bool hi = (gp >> 1) & 1;
-#pragma unroll
-                for (int m = 0; m < 6; ++m) {
-                    const T keep = hi ? h12[6 + m] : h12[m];
-                    const T send = hi ? h12[m] : h12[6 + m];
-                    h6[m] = keep + __shfl_xor(send, 2, 64);
-                }
-            }
-            T h3[3];
-            {
-                const bool hi = gp & 1;
-#pragma unroll
-                for (int m = 0; m < 3; ++m) {
-                    const T keep = hi ? h6[3 + m] : h6[m];
-                    const T send = hi ? h6[m] : h6[3 + m];
-                    h3[m] = keep + __shfl_xor(send, 1, 64);
-                }
-            }
-            // lane gp now holds node n = gp of the element (ie/je/ke order)
-            if (valid) {
-                const int n = gp;
-                const int x = (n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0;
-                const int yy = (n == 2 || n == 3 || n == 6 || n == 7) ? 1 : 0;
-                const int z = n >= 4 ? 1 : 0;
-                const int hj = ej + yy, hk = ek + z, hi = ei + x;  // halo coords
-                if (hj >= 1 && hj <= t.TJ && hk >= 1 && hk <= t.TK && hi >= 1 && hi <= t.TI) {
-                    const int own = 3 * (((hi - 1) * t.TK + (hk - 1)) * t.TJ + (hj - 1));
-                    acc[own] += h3[0];
-                    acc[own + 1] += h3[1];
-                    acc[own + 2] += h3[2];
-                }
-            }
-        }
-        __syncthreads();
-    }
-
-    double part = 0.0;
-    const T om = T(a.omega), one = T(1);
-    for (int own = threadIdx.x; own < nown; own += 256) {
-        const int tj = own % t.TJ;
-        const int r = own / t.TJ;
-        const int tk = r % t.TK, ti = r / t.TK;
-        const int gi = i0 + ti, gj = j0 + tj, gk = k0 + tk;
-        if (gi >= g.nnx || gj >= g.nny || gk >= g.nnz) continue;
-        const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
-        const long dof = 3 * nod;
-        const unsigned msk = a.mask[nod];
-        const int hc = 3 * (((ti + 1) * HK + (tk + 1)) * HJ + (tj + 1));
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const bool fixed = (msk >> c) & 1u;
-            const T yv = acc[3 * own + c];
-            const T u = su[hc + c];
-            if (MODE == OP_MATVEC) {
-                const T v = fixed ? T(0) : yv;
-                a.out[dof + c] = v;
-                part += (double)u * (double)v;
-            } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
-                const T rv = fixed ? T(0) : a.F[dof + c] - yv;
-                a.out[dof + c] = rv;
-                if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = u;
-            } else {
-                const T f = a.F[dof + c];
-                const T rv = fixed ? T(0) : f - yv;
-                const T ad = a.AD[dof + c];
-                const T un = (one - om) * u + om * (rv + ad * u) / ad;
-                a.out[dof + c] = un;
-                part += (double)f * (double)un;
-            }
-        }
-    }
-    if (a.partials) {
-        __shared__ double ws[4];
-        double v = part;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) a.partials[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
-    }
-}
-
-// FP32 form: one thread per coarse element of the tile (and its one-element
-// halo) evaluates all 8 Gauss points in registers and writes its 24 nodal
-// forces to LDS; after one barrier every owned node sums the forces of its 8
-// elements in a fixed order.  No colouring, one barrier, deterministic.
-constexpr int kL1EwElems = 512;  // elements per block: (TJ+1)(TK+1)(TI+1) <= 512
-constexpr int kL1Ys = 25;        // force row stride (odd: conflict-free rows)
-
-template <int MODE>
-__global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<float> a, Tile t, const float *__restrict__ CXf,
-                                                 float lam2mu, float lam, float mu)
-{
-    if (a.ctl && a.ctl[0]) return;
-    __shared__ float su[3 * kL1Halo];
-    __shared__ float sy[kL1EwElems * kL1Ys];
-    const Grid g = a.g;  // the coarse (level-1) grid
-    int b = blockIdx.x;
-    const int bj = b % t.ntj;
-    b /= t.ntj;
-    const int bk = b % t.ntk;
-    const int bi = b / t.ntk;
-    const int j0 = bj * t.TJ, k0 = bk * t.TK, i0 = bi * t.TI;
-    const int HJ = t.TJ + 2, HK = t.TK + 2, HI = t.TI + 2;
-    const int EJ = t.TJ + 1, EK = t.TK + 1, EI = t.TI + 1;
-    const int nown = t.TJ * t.TK * t.TI;
-    const int nel = EJ * EK * EI;
-
-    // halo brick of u, as in k_l1_mf
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
-    const int rowlen = 3 * HJ;
-    const int nrows = HK * HI;
-    for (int r0 = wv; r0 < nrows; r0 += 4 * kStageRows) {
-        float v[kStageRows][2];
-#pragma unroll
-        for (int u = 0; u < kStageRows; ++u) {
-            const int row = r0 + 4 * u;
-            const int hi = row / HK, hk = row - hi * HK;
-            const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
-            const bool rowok = row < nrows && gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
-            const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
-#pragma unroll
-            for (int cc = 0; cc < 2; ++cc) {
-                const int col = ln + 64 * cc;
-                const int gj = j0 - 1 + col / 3;
-                const bool ok = rowok && col < rowlen && gj >= 0 && gj < g.nny;
-                const long dof = ok ? base + col : 0;
-                float x;
-                if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
-                    x = (float)a.prerelax * a.F[dof] / a.AD[dof];
-                else
-                    x = a.U[dof];
-                v[u][cc] = ok ? x : 0.0f;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kStageRows; ++u) {
-            const int row = r0 + 4 * u;
-            if (row >= nrows) break;
-#pragma unroll
-            for (int cc = 0; cc < 2; ++cc) {
-                const int col = ln + 64 * cc;
-                if (col < rowlen) su[3 * row * HJ + col] = v[u][cc];
-            }
-        }
-    }
     __syncthreads();
 
     // element phase
     const int fny = 2 * g.ny, fnz = 2 * g.nz;
-    const float g1 = 0.57735026918962576451f;  // 1/sqrt(3)
-    const float ih = 0.5f;                     // 1/h, h = 2; |J| w_g = h^3/8 = 1
+    const T g1 = T(0.57735026918962576451);  // 1/sqrt(3)
+    const T ih = T(0.5);                     // 1/h, h = 2; |J| w_g = h^3/8 = 1
 #pragma unroll 1
     for (int q = threadIdx.x; q < nel; q += 256) {
         const int ej = q % EJ, r = q / EJ;
         const int ek = r % EK, ei = r / EK;
         const int ex = i0 - 1 + ei, ey = j0 - 1 + ej, ez = k0 - 1 + ek;
-        float y[24];
+        T y[24];
 #pragma unroll
-        for (int m = 0; m < 24; ++m) y[m] = 0.0f;
+        for (int m = 0; m < 24; ++m) y[m] = T(0);
         if (ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz) {
-            float cx[8];
+            T cx[8];
 #pragma unroll
             for (int gp = 0; gp < 8; ++gp) {
                 const int is = gp >> 2, it = (gp >> 1) & 1, ie = gp & 1;
                 cx[gp] = CXf[(long)fny * fnz * (2 * ex + is) + (long)fny * (2 * ez + it) + 2 * ey + ie];
             }
-            float u[2][2][2][3];  // [x][y][z][c]
+            T u[2][2][2][3];  // [x][y][z][c]
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -399,29 +129,29 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<float> a, Tile t, cons
 #pragma unroll
             for (int gp = 0; gp < 8; ++gp) {
                 const int is = gp >> 2, it = (gp >> 1) & 1, ie = gp & 1;
-                const float sx = is ? g1 : -g1, sy = ie ? g1 : -g1, sz = it ? g1 : -g1;
-                const float wx[2] = {lane_w(0, sx), lane_w(1, sx)};
-                const float wy[2] = {lane_w(0, sy), lane_w(1, sy)};
-                const float wz[2] = {lane_w(0, sz), lane_w(1, sz)};
-                float G[3][3];
+                const T sx = is ? g1 : -g1, sy = ie ? g1 : -g1, sz = it ? g1 : -g1;
+                const T wx[2] = {lane_w(0, sx), lane_w(1, sx)};
+                const T wy[2] = {lane_w(0, sy), lane_w(1, sy)};
+                const T wz[2] = {lane_w(0, sz), lane_w(1, sz)};
+                T G[3][3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                    T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
                     for (int p = 0; p < 2; ++p)
 #pragma unroll
                         for (int q2 = 0; q2 < 2; ++q2) {
-                            gx = fmaf(wy[p] * wz[q2], u[1][p][q2][c] - u[0][p][q2][c], gx);
-                            gy = fmaf(wx[p] * wz[q2], u[p][1][q2][c] - u[p][0][q2][c], gy);
-                            gz = fmaf(wx[p] * wy[q2], u[p][q2][1][c] - u[p][q2][0][c], gz);
+                            gx = fma(wy[p] * wz[q2], u[1][p][q2][c] - u[0][p][q2][c], gx);
+                            gy = fma(wx[p] * wz[q2], u[p][1][q2][c] - u[p][0][q2][c], gy);
+                            gz = fma(wx[p] * wy[q2], u[p][q2][1][c] - u[p][q2][0][c], gz);
                         }
                     G[c][0] = ih * gx;
                     G[c][1] = ih * gy;
                     G[c][2] = ih * gz;
                 }
-                const float c0 = cx[gp];
-                const float tr = G[0][0] + G[1][1] + G[2][2];
-                float S[3][3];
+                const T c0 = cx[gp];
+                const T tr = G[0][0] + G[1][1] + G[2][2];
+                T S[3][3];
                 S[0][0] = c0 * (lam * tr + (lam2mu - lam) * G[0][0]);
                 S[1][1] = c0 * (lam * tr + (lam2mu - lam) * G[1][1]);
                 S[2][2] = c0 * (lam * tr + (lam2mu - lam) * G[2][2]);
@@ -433,9 +163,9 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<float> a, Tile t, cons
                     const int x = (n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0;
                     const int yy = (n == 2 || n == 3 || n == 6 || n == 7) ? 1 : 0;
                     const int z = n >= 4 ? 1 : 0;
-                    const float dx = (x ? ih : -ih) * wy[yy] * wz[z];
-                    const float dy = (yy ? ih : -ih) * wx[x] * wz[z];
-                    const float dz = (z ? ih : -ih) * wx[x] * wy[yy];
+                    const T dx = (x ? ih : -ih) * wy[yy] * wz[z];
+                    const T dy = (yy ? ih : -ih) * wx[x] * wz[z];
+                    const T dz = (z ? ih : -ih) * wx[x] * wy[yy];
 #pragma unroll
                     for (int c = 0; c < 3; ++c) y[3 * n + c] += dx * S[c][0] + dy * S[c][1] + dz * S[c][2];
                 }
@@ -448,14 +178,14 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<float> a, Tile t, cons
 
     // node phase: the forces of the 8 elements of each owned node
     double part = 0.0;
-    const float om = (float)a.omega;
+    const T om = T(a.omega);
     for (int own = threadIdx.x; own < nown; own += 256) {
         const int tj = own % t.TJ;
         const int r = own / t.TJ;
         const int tk = r % t.TK, ti = r / t.TK;
         const int gi = i0 + ti, gj = j0 + tj, gk = k0 + tk;
         if (gi >= g.nnx || gj >= g.nny || gk >= g.nnz) continue;
-        float yv[3] = {0.0f, 0.0f, 0.0f};
+        T yv[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int x = 1; x >= 0; --x)
 #pragma unroll
@@ -475,20 +205,20 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<float> a, Tile t, cons
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const bool fixed = (msk >> c) & 1u;
-            const float u = su[hc + c];
+            const T u = su[hc + c];
             if (MODE == OP_MATVEC) {
-                const float v = fixed ? 0.0f : yv[c];
+                const T v = fixed ? T(0) : yv[c];
                 a.out[dof + c] = v;
                 part += (double)u * (double)v;
             } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
-                const float rv = fixed ? 0.0f : a.F[dof + c] - yv[c];
+                const T rv = fixed ? T(0) : a.F[dof + c] - yv[c];
                 a.out[dof + c] = rv;
                 if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = u;
             } else {
-                const float f = a.F[dof + c];
-                const float rv = fixed ? 0.0f : f - yv[c];
-                const float ad = a.AD[dof + c];
-                const float un = (1.0f - om) * u + om * (rv + ad * u) / ad;
+                const T f = a.F[dof + c];
+                const T rv = fixed ? T(0) : f - yv[c];
+                const T ad = a.AD[dof + c];
+                const T un = (T(1) - om) * u + om * (rv + ad * u) / ad;
                 a.out[dof + c] = un;
                 part += (double)f * (double)un;
             }
@@ -507,14 +237,14 @@ static int l1_balance(int n, int tmax)
     return (n + nt - 1) / nt;
 }
 
-Tile l1_tile(const Grid &g)
+static Tile l1_tile_limits(const Grid &g, int halo, int own, int elems)
 {
     Tile t;
     t.TJ = l1_balance(g.nny, 7);
     t.TK = l1_balance(g.nnz, 7);
     t.TI = l1_balance(g.nnx, 7);
-    while ((t.TJ + 2) * (t.TK + 2) * (t.TI + 2) > kL1Halo || t.TJ * t.TK * t.TI > kL1Own ||
-           (t.TJ + 1) * (t.TK + 1) * (t.TI + 1) > kL1EwElems) {
+    while ((t.TJ + 2) * (t.TK + 2) * (t.TI + 2) > halo || t.TJ * t.TK * t.TI > own ||
+           (t.TJ + 1) * (t.TK + 1) * (t.TI + 1) > elems) {
         if (t.TI > 1)
             t.TI--;
         else if (t.TK > 1)
@@ -528,6 +258,9 @@ Tile l1_tile(const Grid &g)
     return t;
 }
 
+Tile l1_tile(const Grid &g) { return l1_tile_limits(g, L1Ew<float>::HALO, kL1Own, L1Ew<float>::NE); }
+Tile l1_tile64(const Grid &g) { return l1_tile_limits(g, L1Ew<double>::HALO, kL1Own, L1Ew<double>::NE); }
+
 template <class T>
 void launch_op1_mf(hipStream_t s, int mode, const OpArgsT<T> &a, const T *CXf, const Tile &t, double nu)
 {
@@ -535,24 +268,13 @@ void launch_op1_mf(hipStream_t s, int mode, const OpArgsT<T> &a, const T *CXf, c
     const double c1 = 1.0 / ((1 + nu) * (1 - 2 * nu));
     const T l2m = T(c1 * (1 - nu)), lam = T(c1 * nu), mu = T(c1 * (1 - 2 * nu) / 2);
     const dim3 grid((unsigned)t.blocks()), block(256);
-    if constexpr (std::is_same<T, float>::value) {
-        switch (mode) {
-        case OP_MATVEC: k_l1_ew<OP_MATVEC><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        case OP_DEFECT: k_l1_ew<OP_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        case OP_JACOBI: k_l1_ew<OP_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        case OP_PRE_JACOBI: k_l1_ew<OP_PRE_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        case OP_PRE_DEFECT: k_l1_ew<OP_PRE_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        default: break;
-        }
-    } else {
-        switch (mode) {
-        case OP_MATVEC: k_l1_mf<T, OP_MATVEC><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        case OP_DEFECT: k_l1_mf<T, OP_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        case OP_JACOBI: k_l1_mf<T, OP_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        case OP_PRE_JACOBI: k_l1_mf<T, OP_PRE_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        case OP_PRE_DEFECT: k_l1_mf<T, OP_PRE_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
-        default: break;
-        }
+    switch (mode) {
+    case OP_MATVEC: k_l1_ew<T, OP_MATVEC><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    case OP_DEFECT: k_l1_ew<T, OP_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    case OP_JACOBI: k_l1_ew<T, OP_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    case OP_PRE_JACOBI: k_l1_ew<T, OP_PRE_JACOBI><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    case OP_PRE_DEFECT: k_l1_ew<T, OP_PRE_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
+    default: break;
     }
 }
 template void launch_op1_mf<double>(hipStream_t, int, const OpArgsT<double> &, const double *, const Tile &, double);

@@ -597,7 +597,8 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
             maxblocks = std::max<long>(maxblocks, xcd_launch(v.tg.blocks()));
             if (l == 1) {
                 v.tmf = l1_tile(v.g);
-                maxblocks = std::max<long>(maxblocks, v.tmf.blocks());
+                v.tmf64 = l1_tile64(v.g);
+                maxblocks = std::max<long>(maxblocks, std::max(v.tmf.blocks(), v.tmf64.blocks()));
             }
             v.R = dalloc<double>(h, v.g.ndof);
             v.AD = dalloc<double>(h, v.g.ndof);

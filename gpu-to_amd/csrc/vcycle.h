@@ -64,7 +64,8 @@ void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const voi
 // matrix-free rediscretised level-1 operator (l1op.hip); CXf = level-0 CX
 template <class T>
 void launch_op1_mf(hipStream_t s, int mode, const OpArgsT<T> &a, const T *CXf, const Tile &t, double nu);
-Tile l1_tile(const Grid &g);
+Tile l1_tile(const Grid &g);    // FP32 tiling
+Tile l1_tile64(const Grid &g);  // FP64 tiling
 constexpr double kL1Nu = 0.3;  // Poisson ratio of the level-1 rediscretisation (mgcg1.c:1539-1584)
 
 // Whether level 1 runs matrix-free under the handle's parameters.
@@ -108,7 +109,8 @@ void level_op(const gputo_handle *h, const HierView<T> &H, int l, int mode, cons
     if (H.mg9) {
         launch_op0<T>(h->stream, mode, a);  // level scales and the CX floor are in a
     } else if (l == 1 && H.l1_mf) {
-        launch_op1_mf<T>(h->stream, mode, a, H.CX, h->L[1].tmf, kL1Nu);
+        launch_op1_mf<T>(h->stream, mode, a, H.CX, std::is_same<T, double>::value ? h->L[1].tmf64 : h->L[1].tmf,
+                         kL1Nu);
     } else if (l > 0) {
         launch_opA<T>(h->stream, mode, a);
     } else if constexpr (std::is_same<T, float>::value) {
