@@ -55,7 +55,9 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<T> a, Tile t, const T 
     const int nown = t.TJ * t.TK * t.TI;
     const int nel = EJ * EK * EI;
 
-    // halo brick of u, as in k_l1_mf
+    // halo brick of u, one j-row per wave (coalesced rows, wave-uniform row
+    // split); kStageRows rows per wave are loaded before any is stored, with
+    // addresses outside the mesh clamped and their values masked
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
     const int rowlen = 3 * HJ;
     const int nrows = HK * HI;
