@@ -50,7 +50,7 @@ struct LearnState {
     double *partials = nullptr;  // reduction partials
     float *wpart = nullptr;      // weight-gradient partials per position chunk
     // logged level-2 defects (training samples) of the current solve
-    int nsamp_max = 0, nsamp = 0, log_every = 1, vc_count = 0;
+    int nsamp_max = 0, nsamp = 0, vc_count = 0;
     float *samples = nullptr;
     bool trained = false;
     long long bytes = 0;
@@ -60,7 +60,18 @@ struct LearnState {
 void learn_create(gputo_handle *h);
 void learn_destroy(gputo_handle *h);
 void learn_init_params(gputo_handle *h, unsigned seed);
-void learn_forward(gputo_handle *h, const int *ctl);  // y = N(x0), activations cached
+// inputs of the correction stage's forward pass: the feature kernel's
+// partials of sum d^2/ad (norm_n of them, over norm_count DOFs), the
+// diagonal and fixed-DOF mask of the level, and the correction's output
+struct CorrIO {
+    const double *norm;
+    int norm_n;
+    double norm_count;
+    const float *ad;
+    const uint8_t *mask;
+    float *e;
+};
+void learn_forward(gputo_handle *h, const int *ctl, const CorrIO *io);  // y = N(x0), activations cached
 void learn_backward(gputo_handle *h);                  // G += d<gy, y>/dtheta
 // Extra stage after the level-2 post-smoothing of the FP32 V-cycle:
 // d = F - A U;  e = N(d);  gamma = <d,e>/<e,Ae>;  U += gamma e if gamma > 0.

@@ -59,82 +59,278 @@ __device__ __forceinline__ void stage_lds(float *__restrict__ dst, int n, const 
 }
 
 // out[co][p] = act(b[co] + sum_ci sum_t w[co][ci][t] in[ci][p*s + t - 1]).
-// A block owns 256/KS output positions and CT output channels; its
-// CT x cin x 27 weights are staged in LDS as [ci][t][c] (one vector read
-// per tap feeds CT FMAs) and the 27 tap offsets of a position are computed
-// once.  KS > 1 splits the input channels over KS threads per position
-// (small grids have too few positions to fill the device); the KS partial
-// sums are added in a fixed order.
-template <int CT, int KS, bool RELU>
-__global__ __launch_bounds__(256) void k_conv_fwd(const float *__restrict__ in, int cin, Dims3 din,
-                                                 const float *__restrict__ w, const float *__restrict__ b,
-                                                 float *__restrict__ out, Dims3 dout, int stride, const int *ctl)
+// A block of KS waves owns 64 output positions (one per lane) and CT
+// output channels; wave v takes the input channels ci = v, v + KS, ...
+// The block's CT x cin x 27 weights are staged in LDS once as [ci][t][c]:
+// one broadcast read of 4 consecutive c feeds 4 FMAs, and each vector load
+// of an input value feeds CT FMAs.  The grids are small, so a layer has
+// about one block per CU and the staging is paid about once per CU.  The
+// KS partial sums meet in LDS (over the weights, after a barrier) and are
+// added in wave order; wave v then finishes the channels c = v, v + KS, ...
+// (bias, ReLU, epilogue).
+//
+// Options carried in ConvFwd:
+//  * norm (conv 1 of the correction): input channels 0-2 are scaled by
+//    1/sigma, sigma = sqrt(sum(norm[0..norm_n)) / norm_count) from the
+//    partials of k_features; every wave sums them in the same order, and
+//    block (0, 0) stores sigma in sc[0];
+//  * up (convs 4 and 5): the output is also upsampled to the finer grid
+//    dup and added to the skip activation there, up[c][q] = out[c][p] +
+//    skip[c][q] for the up to 8 fine nodes q with parent p;
+//  * corr (conv 6): the output is also turned into the level-2 correction
+//    corr[3p+c] = fixed ? 0 : sigma y[c][p] / sqrt(ad[3p+c]).
+struct ConvFwd {
+    const float *in;
+    Dims3 din;
+    int cin, cout, stride, relu;
+    const float *w, *b;
+    float *out;
+    Dims3 dout;
+    const double *norm = nullptr;
+    int norm_n = 0;
+    double norm_count = 1.0;
+    double *sc = nullptr;
+    const float *skip = nullptr;
+    float *up = nullptr;
+    Dims3 dup{};
+    const float *ad = nullptr;
+    const uint8_t *mask = nullptr;
+    float *corr = nullptr;
+    const int *ctl = nullptr;
+};
+
+// 1/sigma from the feature partials (every wave sums them in the same
+// order); block (0, 0) stores sigma in sc[0].  Called by whole waves.
+__device__ __forceinline__ float conv_inv_sigma(const ConvFwd &a)
 {
-    if (ctl && ctl[0]) return;
-    extern __shared__ float wl[];  // [cin*27][CT], then KS partials [KS][256/KS][CT]
-    constexpr int NP = 256 / KS;
+    double s = 0.0;
+    for (int i = (int)(threadIdx.x & 63); i < a.norm_n; i += 64) s += a.norm[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    double sig = sqrt(s / a.norm_count);
+    sig = sig > 1e-300 ? sig : 1.0;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.sc[0] = sig;
+    return (float)(1.0 / sig);
+}
+
+// bias-added sum r of output channel co at position p = (oi, ok, oj):
+// activation, store, the corr epilogue and (with_up) the up epilogue
+__device__ __forceinline__ void conv_finish(const ConvFwd &a, int co, int p, int nout, int oi, int ok, int oj,
+                                            float r, bool with_up)
+{
+    if (a.relu) r = fmaxf(r, 0.0f);
+    a.out[(long)co * nout + p] = r;
+    if (with_up && a.up) {
+        const int nup = (int)a.dup.size();
+        for (int di = 0; di < 2; ++di)
+            for (int dk = 0; dk < 2; ++dk)
+                for (int dj = 0; dj < 2; ++dj) {
+                    const int fi = 2 * oi + di, fk = 2 * ok + dk, fj = 2 * oj + dj;
+                    if (fi < a.dup.n0 && fk < a.dup.n1 && fj < a.dup.n2) {
+                        const long q = (long)co * nup + (fi * a.dup.n1 + fk) * a.dup.n2 + fj;
+                        a.up[q] = r + a.skip[q];
+                    }
+                }
+    }
+    if (a.corr) {
+        const float sig = (float)a.sc[0];
+        const long q = 3L * p + co;
+        a.corr[q] = ((a.mask[p] >> co) & 1u) ? 0.0f : sig * r / sqrtf(a.ad[q]);
+    }
+}
+
+template <int CT, int KS>
+__global__ __launch_bounds__(64 * KS) void k_conv_fwd(ConvFwd a)
+{
+    if (a.ctl && a.ctl[0]) return;
+    extern __shared__ __align__(16) float lds[];  // weights [cin][27][CT]; then partials [KS][CT][64]
+    constexpr int NT = 64 * KS;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int co0 = blockIdx.y * CT;
-    stage_lds(wl, cin * 27 * CT, w, [=](int q) {
-        const int c = q % CT, k = q / CT;  // k = ci*27 + t
-        return (co0 + c) * cin * 27 + k;
-    });
+    const int nout = (int)a.dout.size(), nin = (int)a.din.size();
+    const int p = blockIdx.x * 64 + lane;
+    const bool live = p < nout;
+    const int nw = a.cin * 27 * CT;
+    for (int q0 = 0; q0 < nw; q0 += 8 * NT) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u * NT + (int)threadIdx.x;
+            v[u] = a.w[q < nw ? (co0 + q % CT) * a.cin * 27 + q / CT : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u * NT + (int)threadIdx.x;
+            if (q < nw) lds[q] = v[u];
+        }
+    }
+    const float inv = a.norm ? conv_inv_sigma(a) : 1.0f;
+    int oi, ok, oj;
+    pos3(live ? p : 0, a.dout, oi, ok, oj);
+    int off[27];
+    unsigned valid = 0;
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        const int ii = oi * a.stride + t / 9 - 1, ik = ok * a.stride + (t / 3) % 3 - 1,
+                  ij = oj * a.stride + t % 3 - 1;
+        const bool v = live && ii >= 0 && ii < a.din.n0 && ik >= 0 && ik < a.din.n1 && ij >= 0 && ij < a.din.n2;
+        valid |= (unsigned)v << t;
+        off[t] = v ? (ii * a.din.n1 + ik) * a.din.n2 + ij : 0;
+    }
     __syncthreads();
-    const int pl = threadIdx.x % NP, ks = threadIdx.x / NP;
-    const int p = blockIdx.x * NP + pl;
-    const int nout = (int)dout.size(), nin = (int)din.size();
     float acc[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[c] = 0.0f;
-    if (p < nout) {
-        int oi, ok, oj;
-        pos3(p, dout, oi, ok, oj);
-        int off[27];
-        unsigned valid = 0;
+    for (int ci = wv; ci < a.cin; ci += KS) {
+        const float *src = a.in + (long)ci * nin;
+        // unconditional loads (off = 0 outside the grid), masked after:
+        // all 27 loads of a channel are in flight together
+        float v[27];
+#pragma unroll
+        for (int t = 0; t < 27; ++t) v[t] = src[off[t]];
+        const float sc = ci < 3 ? inv : 1.0f;
+        const float *wc = lds + ci * 27 * CT;
 #pragma unroll
         for (int t = 0; t < 27; ++t) {
-            const int ii = oi * stride + t / 9 - 1, ik = ok * stride + (t / 3) % 3 - 1, ij = oj * stride + t % 3 - 1;
-            const bool v = ii >= 0 && ii < din.n0 && ik >= 0 && ik < din.n1 && ij >= 0 && ij < din.n2;
-            valid |= (unsigned)v << t;
-            off[t] = v ? (ii * din.n1 + ik) * din.n2 + ij : 0;
-        }
-        if (KS == 1) {
+            const float vt = ((valid >> t) & 1u) ? v[t] * sc : 0.0f;
+            if constexpr (CT % 4 == 0) {
 #pragma unroll
-            for (int c = 0; c < CT; ++c) acc[c] = b[co0 + c];
-        }
-        for (int ci = ks; ci < cin; ci += KS) {
-            const float *src = in + ci * nin;
-            const float *wc = wl + ci * 27 * CT;
-            // unconditional loads (off = 0 outside the grid), masked after:
-            // all 27 loads of a channel are in flight together
-            float v[27];
-#pragma unroll
-            for (int t = 0; t < 27; ++t) v[t] = src[off[t]];
-#pragma unroll
-            for (int t = 0; t < 27; ++t) {
-                const float vt = ((valid >> t) & 1u) ? v[t] : 0.0f;
+                for (int c4 = 0; c4 < CT; c4 += 4) {
+                    const float4 w4 = *reinterpret_cast<const float4 *>(wc + t * CT + c4);
+                    acc[c4] = fmaf(w4.x, vt, acc[c4]);
+                    acc[c4 + 1] = fmaf(w4.y, vt, acc[c4 + 1]);
+                    acc[c4 + 2] = fmaf(w4.z, vt, acc[c4 + 2]);
+                    acc[c4 + 3] = fmaf(w4.w, vt, acc[c4 + 3]);
+                }
+            } else {
 #pragma unroll
                 for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], vt, acc[c]);
             }
         }
     }
+    float *part = lds;
+    if (KS > 1) {
+        __syncthreads();  // every wave is done with the weights
+#pragma unroll
+        for (int c = 0; c < CT; ++c) part[(wv * CT + c) * 64 + lane] = acc[c];
+        __syncthreads();
+    }
+    if (!live) return;
+    auto finish = [&](int c, float r) { conv_finish(a, co0 + c, p, nout, oi, ok, oj, r, true); };
+    if constexpr (KS == 1) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) finish(c, a.b[co0 + c] + acc[c]);
+    } else {
+        for (int c = wv; c < CT; c += KS) {
+            float r = a.b[co0 + c];
+            for (int k = 0; k < KS; ++k) r += part[(k * CT + c) * 64 + lane];
+            finish(c, r);
+        }
+    }
+}
+
+// The same convolution with the input channels split over the lanes: a
+// 256-thread block owns NP = 256/KS positions and CT output channels,
+// thread (ks, pl) takes position pl and the channels ci = ks, ks + KS, ...
+// The coarse grids (a few hundred positions) use it: their blocks are many
+// and small, each input value gathered serves CT outputs, and the KS
+// partials of a position are added in ks order.
+template <int CT, int KS>
+__global__ __launch_bounds__(256) void k_conv_fwd_lanes(ConvFwd a)
+{
+    if (a.ctl && a.ctl[0]) return;
+    extern __shared__ float wl[];  // [cin*27][CT], then KS partials [KS][256/KS][CT]
+    constexpr int NP = 256 / KS;
+    const int co0 = blockIdx.y * CT;
+    const int cin = a.cin;
+    stage_lds(wl, cin * 27 * CT, a.w, [=](int q) {
+        const int c = q % CT, k = q / CT;  // k = ci*27 + t
+        return (co0 + c) * cin * 27 + k;
+    });
+    const float inv = a.norm ? conv_inv_sigma(a) : 1.0f;
+    __syncthreads();
+    const int pl = threadIdx.x % NP, ks = threadIdx.x / NP;
+    const int p = blockIdx.x * NP + pl;
+    const int nout = (int)a.dout.size(), nin = (int)a.din.size();
+    float acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = 0.0f;
+    int oi = 0, ok = 0, oj = 0;
+    if (p < nout) {
+        pos3(p, a.dout, oi, ok, oj);
+        int off[27];
+        unsigned valid = 0;
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const int ii = oi * a.stride + t / 9 - 1, ik = ok * a.stride + (t / 3) % 3 - 1,
+                      ij = oj * a.stride + t % 3 - 1;
+            const bool v = ii >= 0 && ii < a.din.n0 && ik >= 0 && ik < a.din.n1 && ij >= 0 && ij < a.din.n2;
+            valid |= (unsigned)v << t;
+            off[t] = v ? (ii * a.din.n1 + ik) * a.din.n2 + ij : 0;
+        }
+        for (int ci = ks; ci < cin; ci += KS) {
+            const float *src = a.in + (long)ci * nin;
+            const float *wc = wl + ci * 27 * CT;
+            float v[27];
+#pragma unroll
+            for (int t = 0; t < 27; ++t) v[t] = src[off[t]];
+            const float sc = ci < 3 ? inv : 1.0f;
+#pragma unroll
+            for (int t = 0; t < 27; ++t) {
+                const float vt = ((valid >> t) & 1u) ? v[t] * sc : 0.0f;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], vt, acc[c]);
+            }
+        }
+    }
+    const bool own = ks == 0 && p < nout;  // the thread that finishes position p
     if (KS > 1) {
         float *part = wl + cin * 27 * CT;
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < CT; ++c) part[(ks * NP + pl) * CT + c] = acc[c];
         __syncthreads();
-        if (ks != 0 || p >= nout) return;
+        if (own) {
 #pragma unroll
-        for (int c = 0; c < CT; ++c) {
-            float sm = b[co0 + c];
-            for (int k = 0; k < KS; ++k) sm += part[(k * NP + pl) * CT + c];
-            acc[c] = sm;
+            for (int c = 0; c < CT; ++c) {
+                float sm = a.b[co0 + c];
+                for (int k = 0; k < KS; ++k) sm += part[(k * NP + pl) * CT + c];
+                acc[c] = sm;
+            }
         }
-    } else if (p >= nout) {
-        return;
-    }
+    } else if (own) {
 #pragma unroll
-    for (int c = 0; c < CT; ++c) out[(long)(co0 + c) * nout + p] = RELU ? fmaxf(acc[c], 0.0f) : acc[c];
+        for (int c = 0; c < CT; ++c) acc[c] += a.b[co0 + c];
+    }
+    if (own) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) conv_finish(a, co0 + c, p, nout, oi, ok, oj, acc[c], false);
+    }
+    if (a.up) {
+        // the up epilogue over the whole block: item (position, channel,
+        // child) per thread, from the activated outputs in LDS
+        float *res = wl;
+        __syncthreads();  // every thread is done with the weights and partials
+        if (own) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) res[pl * CT + c] = a.relu ? fmaxf(acc[c], 0.0f) : acc[c];
+        }
+        __syncthreads();
+        const int nup = (int)a.dup.size();
+        for (int it = threadIdx.x; it < NP * CT * 8; it += 256) {
+            const int ch = it & 7, c = (it >> 3) % CT, ql = it / (8 * CT);
+            const int pp = blockIdx.x * NP + ql;
+            if (pp >= nout) continue;
+            int pi, pk, pj;
+            pos3(pp, a.dout, pi, pk, pj);
+            const int fi = 2 * pi + (ch >> 2), fk = 2 * pk + ((ch >> 1) & 1), fj = 2 * pj + (ch & 1);
+            if (fi < a.dup.n0 && fk < a.dup.n1 && fj < a.dup.n2) {
+                const long q = (long)(co0 + c) * nup + (fi * a.dup.n1 + fk) * a.dup.n2 + fj;
+                a.up[q] = res[ql * CT + c] + a.skip[q];
+            }
+        }
+    }
 }
 
 // gin[ci][q] = sum_co sum_t w[co][ci][t] gz[co][p] over the outputs p that
@@ -263,21 +459,6 @@ __global__ void k_conv_bwd_w_sum(const float *__restrict__ part, int nch, int np
         gb[pair / cin] += sm;
 }
 
-// out[c][p] = d[c][parent(p)] + e[c][p]  (nearest upsampling by 2 plus skip)
-__global__ void k_up_add(const float *__restrict__ dcoarse, Dims3 dc, const float *__restrict__ efine, Dims3 df,
-                         int C, float *__restrict__ out, const int *ctl)
-{
-    if (ctl && ctl[0]) return;
-    const int n = (int)df.size(), total = n * C, ncs = (int)dc.size();
-    for (int q = blockIdx.x * 256 + threadIdx.x; q < total; q += gridDim.x * 256) {
-        const int c = q / n;
-        const int p = q - c * n;
-        int i, k, j;
-        pos3(p, df, i, k, j);
-        out[q] = dcoarse[c * ncs + ((i >> 1) * dc.n1 + (k >> 1)) * dc.n2 + (j >> 1)] + efine[q];
-    }
-}
-
 // gd[c][pc] = sum over the (up to 8) fine children of pc of gu[c][p]
 __global__ void k_up_bwd(const float *__restrict__ gu, Dims3 df, int C, Dims3 dc, float *__restrict__ gd)
 {
@@ -318,59 +499,38 @@ __device__ double block_total(const double *p, int n)
     return block_sum(s);
 }
 
-// partials of sum_d d^2 / ad
-__global__ __launch_bounds__(kBlock) void k_scaled_norm(const float *__restrict__ d, const float *__restrict__ ad,
-                                                       long n, double *partials, const int *ctl)
+// x[c][p] = d[3p+c] / sqrt(ad[3p+c]) (c < 3), x[3][p] = log10(mean_c ad)/9,
+// and partials[block] = the block's sum of d^2/ad in double: conv 1 turns
+// the kFeatBlocks partials into sigma, the RMS of the Jacobi-scaled defect,
+// and scales channels 0-2 by 1/sigma as it reads them.
+constexpr int kFeatBlocks = 128;
+
+__global__ __launch_bounds__(kBlock) void k_features(const float *__restrict__ d, const float *__restrict__ ad,
+                                                    long nnod, float *__restrict__ x, double *partials,
+                                                    const int *ctl)
 {
     if (ctl && ctl[0]) return;
     double s = 0.0;
-    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < n; q += (long)kRedBlocks * kBlock)
-        s += (double)d[q] * (double)d[q] / (double)ad[q];
+    for (long p = (long)blockIdx.x * kBlock + threadIdx.x; p < nnod; p += (long)kFeatBlocks * kBlock) {
+        float m = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float av = ad[3 * p + c], dv = d[3 * p + c];
+            x[c * nnod + p] = dv / sqrtf(av);
+            s += (double)dv * (double)dv / (double)av;
+            m += av;
+        }
+        x[3 * nnod + p] = log10f(fmaxf(m / 3.0f, 1e-30f)) / 9.0f;
+    }
     s = block_sum(s);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_set_sigma(double *sc, const double *p, long n, const int *ctl)
+// x[c][p] *= 1/sigma for c < 3: the input conv 1 saw, for the backward pass
+__global__ void k_scale_features(float *__restrict__ x, long nnod, const double *sc)
 {
-    if (ctl && ctl[0]) return;
-    const double s = block_total(p, kRedBlocks);
-    if (threadIdx.x == 0) {
-        const double sig = sqrt(s / (double)n);
-        sc[0] = sig > 1e-300 ? sig : 1.0;
-    }
-}
-
-// x[c][p] = d[3p+c] / (sqrt(ad[3p+c]) sigma), x[3][p] = log10(mean_c ad)/9
-__global__ void k_features(const float *__restrict__ d, const float *__restrict__ ad, long nnod,
-                           const double *sc, float *__restrict__ x, const int *ctl)
-{
-    if (ctl && ctl[0]) return;
     const float inv = (float)(1.0 / sc[0]);
-    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < nnod; p += (long)gridDim.x * 256) {
-        float m = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float a = ad[3 * p + c];
-            x[c * nnod + p] = d[3 * p + c] * inv / sqrtf(a);
-            m += a;
-        }
-        x[3 * nnod + p] = log10f(fmaxf(m / 3.0f, 1e-30f)) / 9.0f;
-    }
-}
-
-// e[3p+c] = fixed ? 0 : sigma y[c][p] / sqrt(ad[3p+c])
-__global__ void k_correction(const float *__restrict__ y, const float *__restrict__ ad,
-                             const uint8_t *__restrict__ mask, long nnod, const double *sc, float *__restrict__ e,
-                             const int *ctl)
-{
-    if (ctl && ctl[0]) return;
-    const float sig = (float)sc[0];
-    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < nnod; p += (long)gridDim.x * 256) {
-        const unsigned m = mask[p];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            e[3 * p + c] = ((m >> c) & 1u) ? 0.0f : sig * y[c * nnod + p] / sqrtf(ad[3 * p + c]);
-    }
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < 3 * nnod; q += (long)gridDim.x * 256) x[q] *= inv;
 }
 
 // gy[c][p] += w (Ae - d)[3p+c] / (sqrt(ad) sigma) at free DOFs
@@ -389,30 +549,49 @@ __global__ void k_loss_grad(const float *__restrict__ Ae, const float *__restric
     }
 }
 
-// gamma = <d,e>/<e,Ae> if positive and finite, else 0 (fallback); counters
-__global__ __launch_bounds__(kBlock) void k_gamma(double *sc, const double *p_de, const double *p_eAe, int *count,
-                                                 const int *ctl)
+// partials of <d,e> (first kFeatBlocks) and <e,Ae> (next kFeatBlocks)
+__global__ __launch_bounds__(kBlock) void k_dot2f(const float *__restrict__ d, const float *__restrict__ e,
+                                                 const float *__restrict__ Ae, long n, double *part, const int *ctl)
 {
     if (ctl && ctl[0]) return;
-    const double de = block_total(p_de, kRedBlocks);
-    const double eAe = block_total(p_eAe, kRedBlocks);
+    double s0 = 0.0, s1 = 0.0;
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < n; q += (long)kFeatBlocks * kBlock) {
+        const double ev = e[q];
+        s0 += (double)d[q] * ev;
+        s1 += ev * (double)Ae[q];
+    }
+    s0 = block_sum(s0);
+    s1 = block_sum(s1);
     if (threadIdx.x == 0) {
-        double g = de / eAe;
-        const bool ok = de > 0.0 && eAe > 0.0 && g > 0.0 && g < INFINITY;
+        part[blockIdx.x] = s0;
+        part[kFeatBlocks + blockIdx.x] = s1;
+    }
+}
+
+// gamma = <d,e>/<e,Ae> from the k_dot2f partials; U += gamma e if gamma is
+// positive and finite, else the step is dropped (the plain V-cycle is the
+// fallback).  Every block sums the partials in the same order; block 0
+// records gamma and the dots (sc[1..3]), the accept/reject counters and,
+// when logged > 0, the number of logged samples.
+__global__ __launch_bounds__(kBlock) void k_step(float *__restrict__ U, const float *__restrict__ e, long n,
+                                                const double *part, double *sc, int *count, int logged,
+                                                const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const double de = block_total(part, kFeatBlocks);
+    const double eAe = block_total(part + kFeatBlocks, kFeatBlocks);
+    const double g = de / eAe;
+    const bool ok = de > 0.0 && eAe > 0.0 && g > 0.0 && g < INFINITY;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc[1] = ok ? g : 0.0;
         sc[2] = de;
         sc[3] = eAe;
         count[ok ? 0 : 1] += 1;
+        if (logged > 0) count[2] = logged;
     }
-}
-
-__global__ void k_axpy_gamma(float *__restrict__ U, const float *__restrict__ e, long n, const double *sc,
-                             const int *ctl)
-{
-    if (ctl && ctl[0]) return;
-    const float g = (float)sc[1];
-    if (g == 0.0f) return;
-    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) U[q] += g * e[q];
+    if (!ok) return;
+    const float gf = (float)g;
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) U[q] += gf * e[q];
 }
 
 // energy loss of one sample: (1/2 <e,Ae> - <e,d>) / sigma^2 accumulated in sc[4]
@@ -423,20 +602,11 @@ __global__ __launch_bounds__(kBlock) void k_loss_acc(double *sc, const double *p
     if (threadIdx.x == 0) sc[4] += (0.5 * eAe - ed) / (sc[0] * sc[0]);
 }
 
-__global__ void k_log_sample(const float *__restrict__ d, float *__restrict__ samples, long n, int *count,
-                             int nmax, const int *ctl)
+// number of logged samples of the solve (the defect went to its slot)
+__global__ void k_log_count(int *count, int logged, const int *ctl)
 {
     if (ctl && ctl[0]) return;
-    const int s = count[2];
-    if (s >= nmax) return;
-    float *dst = samples + (long)s * n;
-    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) dst[q] = d[q];
-}
-
-__global__ void k_log_count(int *count, int nmax, const int *ctl)
-{
-    if (ctl && ctl[0]) return;
-    if (threadIdx.x == 0 && blockIdx.x == 0 && count[2] < nmax) count[2] += 1;
+    if (threadIdx.x == 0 && blockIdx.x == 0) count[2] = logged;
 }
 
 __global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
@@ -466,50 +636,91 @@ T *ldalloc(LearnState *L, size_t n)
 
 Dims3 half(const Dims3 &d) { return {(d.n0 - 1) / 2 + 1, (d.n1 - 1) / 2 + 1, (d.n2 - 1) / 2 + 1}; }
 
-template <int CT, int KS>
-void conv_fwd_t(hipStream_t s, const NetLayer &ly, const float *w, const float *b, const float *in, Dims3 din,
-                float *out, Dims3 dout, const int *ctl)
-{
-    const dim3 grid((unsigned)((dout.size() + 256 / KS - 1) / (256 / KS)), ly.cout / CT);
-    const size_t lds = ((size_t)ly.cin * 27 * CT + (KS > 1 ? 256 * CT : 0)) * sizeof(float);
-    if (ly.relu)
-        k_conv_fwd<CT, KS, true><<<grid, 256, lds, s>>>(in, ly.cin, din, w, b, out, dout, ly.stride, ctl);
-    else
-        k_conv_fwd<CT, KS, false><<<grid, 256, lds, s>>>(in, ly.cin, din, w, b, out, dout, ly.stride, ctl);
-}
-
-// Input-channel split: the smallest KS (a power of two, at most cin) that
-// gives at least kConvBlocks blocks, so the small grids fill the device.
-constexpr long kConvBlocks = 1024;
+// Choice of kernel per layer.  A grid with at least kWaveTiles blocks of
+// 64 positions x 16 output channels (3 for the output layer) goes to the
+// wave kernel with KS = min(8, cin) waves; its blocks stage their weights
+// once and each gather feeds 16 FMAs.  Coarser grids go to the lane kernel
+// with CT = 4 (3) and the smallest KS (a power of two, at most 32 and at
+// most cin) that gives kLaneBlocks blocks, so that they fill the device.
+constexpr long kWaveTiles = 128;
+constexpr long kLaneBlocks = 1024;
 
 template <int CT>
-void conv_fwd_ct(hipStream_t s, const NetLayer &ly, const float *w, const float *b, const float *in, Dims3 din,
-                 float *out, Dims3 dout, const int *ctl)
+void conv_wave(hipStream_t s, const ConvFwd &a, int ks)
 {
-    int ks = 1;
-    while (ks < 32 && 2 * ks <= ly.cin &&
-           ((dout.size() + 256 / ks - 1) / (256 / ks)) * (ly.cout / CT) < kConvBlocks)
-        ks *= 2;
+    const dim3 grid((unsigned)((a.dout.size() + 63) / 64), (unsigned)(a.cout / CT));
+    const size_t lds = std::max<size_t>((size_t)a.cin * 27 * CT, ks > 1 ? (size_t)ks * CT * 64 : 0) * sizeof(float);
     switch (ks) {
-    case 1: conv_fwd_t<CT, 1>(s, ly, w, b, in, din, out, dout, ctl); break;
-    case 2: conv_fwd_t<CT, 2>(s, ly, w, b, in, din, out, dout, ctl); break;
-    case 4: conv_fwd_t<CT, 4>(s, ly, w, b, in, din, out, dout, ctl); break;
-    case 8: conv_fwd_t<CT, 8>(s, ly, w, b, in, din, out, dout, ctl); break;
-    case 16: conv_fwd_t<CT, 16>(s, ly, w, b, in, din, out, dout, ctl); break;
-    default: conv_fwd_t<CT, 32>(s, ly, w, b, in, din, out, dout, ctl); break;
+    case 1: k_conv_fwd<CT, 1><<<grid, 64, lds, s>>>(a); break;
+    case 2: k_conv_fwd<CT, 2><<<grid, 128, lds, s>>>(a); break;
+    case 4: k_conv_fwd<CT, 4><<<grid, 256, lds, s>>>(a); break;
+    default: k_conv_fwd<CT, 8><<<grid, 512, lds, s>>>(a); break;
     }
 }
 
-void conv_fwd(hipStream_t s, const NetLayer &ly, const float *P, const float *in, Dims3 din, float *out, Dims3 dout,
-              const int *ctl)
+template <int CT, int KS>
+void conv_lanes_ks(hipStream_t s, const ConvFwd &a)
 {
-    const float *w = P + ly.w_off, *b = P + ly.b_off;
-    if (ly.cout % 4 == 0)
-        conv_fwd_ct<4>(s, ly, w, b, in, din, out, dout, ctl);
-    else if (ly.cout == 3)
-        conv_fwd_ct<3>(s, ly, w, b, in, din, out, dout, ctl);
+    const dim3 grid((unsigned)((a.dout.size() + 256 / KS - 1) / (256 / KS)), (unsigned)(a.cout / CT));
+    // weights and partials; the up epilogue reuses the front for NP x CT outputs
+    const size_t lds = std::max<size_t>((size_t)a.cin * 27 * CT + (KS > 1 ? 256 * CT : 0),
+                                        a.up ? (size_t)(256 / KS) * CT : 0) * sizeof(float);
+    k_conv_fwd_lanes<CT, KS><<<grid, 256, lds, s>>>(a);
+}
+
+template <int CT>
+void conv_lanes(hipStream_t s, const ConvFwd &a)
+{
+    int ks = 1;
+    while (ks < 32 && 2 * ks <= a.cin &&
+           ((a.dout.size() + 256 / ks - 1) / (256 / ks)) * (a.cout / CT) < kLaneBlocks)
+        ks *= 2;
+    switch (ks) {
+    case 1: conv_lanes_ks<CT, 1>(s, a); break;
+    case 2: conv_lanes_ks<CT, 2>(s, a); break;
+    case 4: conv_lanes_ks<CT, 4>(s, a); break;
+    case 8: conv_lanes_ks<CT, 8>(s, a); break;
+    case 16: conv_lanes_ks<CT, 16>(s, a); break;
+    default: conv_lanes_ks<CT, 32>(s, a); break;
+    }
+}
+
+void conv_fwd(hipStream_t s, const ConvFwd &a)
+{
+    const long tiles = (a.dout.size() + 63) / 64;
+    int ks = 1;
+    while (ks < 8 && 2 * ks <= a.cin) ks *= 2;
+    if (a.cout == 3) {
+        if (tiles >= kWaveTiles)
+            conv_wave<3>(s, a, ks);
+        else
+            conv_lanes<3>(s, a);
+        return;
+    }
+    if (a.cout % 16 == 0 && a.cin * 27 * 16 <= 16384 && tiles * (a.cout / 16) >= kWaveTiles)
+        conv_wave<16>(s, a, ks);
+    else if (a.cout % 4 == 0)
+        conv_lanes<4>(s, a);
     else
-        conv_fwd_ct<1>(s, ly, w, b, in, din, out, dout, ctl);
+        throw std::runtime_error("conv: output channels not 3 or a multiple of 4");
+}
+
+ConvFwd conv_args(const NetLayer &ly, const float *P, const float *in, Dims3 din, float *out, Dims3 dout,
+                  const int *ctl)
+{
+    ConvFwd a;
+    a.in = in;
+    a.din = din;
+    a.cin = ly.cin;
+    a.cout = ly.cout;
+    a.stride = ly.stride;
+    a.relu = ly.relu;
+    a.w = P + ly.w_off;
+    a.b = P + ly.b_off;
+    a.out = out;
+    a.dout = dout;
+    a.ctl = ctl;
+    return a;
 }
 
 // gradients of one layer: weights/bias accumulate into G; gin (if non-null)
@@ -634,20 +845,42 @@ void learn_init_params(gputo_handle *h, unsigned seed)
     L->trained = false;
 }
 
-// y = N(x0) with the forward activations cached
-void learn_forward(gputo_handle *h, const int *ctl)
+// y = N(x0) with the forward activations cached.  With io (the correction
+// stage) conv 1 scales the defect channels by 1/sigma from the feature
+// partials and stores sigma, and conv 6 also writes the correction.
+void learn_forward(gputo_handle *h, const int *ctl, const CorrIO *io)
 {
     LearnState *L = h->ln;
     hipStream_t s = h->stream;
     const Dims3 d0 = L->d0, d1 = L->d1, d2 = L->d2;
-    conv_fwd(s, L->L[0], L->P, L->x0, d0, L->e1, d0, ctl);
-    conv_fwd(s, L->L[1], L->P, L->e1, d0, L->e2, d1, ctl);
-    conv_fwd(s, L->L[2], L->P, L->e2, d1, L->e3, d2, ctl);
-    conv_fwd(s, L->L[3], L->P, L->e3, d2, L->a2, d2, ctl);
-    k_up_add<<<grid256(32 * d1.size()), 256, 0, s>>>(L->a2, d2, L->e2, d1, 32, L->u2, ctl);
-    conv_fwd(s, L->L[4], L->P, L->u2, d1, L->a1, d1, ctl);
-    k_up_add<<<grid256(16 * d0.size()), 256, 0, s>>>(L->a1, d1, L->e1, d0, 16, L->u1, ctl);
-    conv_fwd(s, L->L[5], L->P, L->u1, d0, L->y, d0, ctl);
+    ConvFwd c1 = conv_args(L->L[0], L->P, L->x0, d0, L->e1, d0, ctl);
+    if (io) {
+        c1.norm = io->norm;
+        c1.norm_n = io->norm_n;
+        c1.norm_count = io->norm_count;
+        c1.sc = L->sc;
+    }
+    conv_fwd(s, c1);
+    conv_fwd(s, conv_args(L->L[1], L->P, L->e1, d0, L->e2, d1, ctl));
+    conv_fwd(s, conv_args(L->L[2], L->P, L->e2, d1, L->e3, d2, ctl));
+    ConvFwd c4 = conv_args(L->L[3], L->P, L->e3, d2, L->a2, d2, ctl);  // u2 = up(a2) + e2
+    c4.skip = L->e2;
+    c4.up = L->u2;
+    c4.dup = d1;
+    conv_fwd(s, c4);
+    ConvFwd c5 = conv_args(L->L[4], L->P, L->u2, d1, L->a1, d1, ctl);  // u1 = up(a1) + e1
+    c5.skip = L->e1;
+    c5.up = L->u1;
+    c5.dup = d0;
+    conv_fwd(s, c5);
+    ConvFwd c6 = conv_args(L->L[5], L->P, L->u1, d0, L->y, d0, ctl);
+    if (io) {
+        c6.sc = L->sc;
+        c6.ad = io->ad;
+        c6.mask = io->mask;
+        c6.corr = io->e;
+    }
+    conv_fwd(s, c6);
 }
 
 // parameter gradients of <gy, y> accumulated into G (forward cache current)
@@ -672,43 +905,44 @@ void learn_backward(gputo_handle *h)
     conv_bwd(s, L->L[0], P, G, L->wpart, L->x0, d0, L->e1, d0, L->ge1, nullptr);
 }
 
-// e = correction for the defect in L->d (features, forward, output scaling)
-static void correction(gputo_handle *h, const HierView<float> &H, int l, const int *ctl)
+// e = correction for the defect d (features, forward, output scaling)
+static void correction(gputo_handle *h, const HierView<float> &H, int l, const float *d, const int *ctl)
 {
     LearnState *L = h->ln;
-    hipStream_t s = h->stream;
     const Grid &g = h->L[l].g;
-    k_scaled_norm<<<kRedBlocks, kBlock, 0, s>>>(L->d, H.L[l].AD, g.ndof, L->partials, ctl);
-    k_set_sigma<<<1, kBlock, 0, s>>>(L->sc, L->partials, g.ndof, ctl);
-    k_features<<<grid256(g.nnod), 256, 0, s>>>(L->d, H.L[l].AD, g.nnod, L->sc, L->x0, ctl);
-    learn_forward(h, ctl);
-    k_correction<<<grid256(g.nnod), 256, 0, s>>>(L->y, H.L[l].AD, h->L[l].mask, g.nnod, L->sc, L->e, ctl);
+    k_features<<<kFeatBlocks, kBlock, 0, h->stream>>>(d, H.L[l].AD, g.nnod, L->x0, L->partials, ctl);
+    const CorrIO io{L->partials, kFeatBlocks, (double)g.ndof, H.L[l].AD, h->L[l].mask, L->e};
+    learn_forward(h, ctl, &io);
 }
 
+// The first nsamp_max V-cycles of a solve log their defect as training
+// samples: the defect is computed straight into its slot.
 void learn_stage(gputo_handle *h, const HierView<float> &H, int l, const float *F, float *U, const int *ctl)
 {
     LearnState *L = h->ln;
     hipStream_t s = h->stream;
     const Grid &g = h->L[l].g;
+    const int slot = L->vc_count++;
+    const bool logged = slot < L->nsamp_max;
+    if (!L->trained && !logged) return;
+    float *d = logged ? L->samples + (long)slot * g.ndof : L->d;
     OpArgsT<float> a = view_args(h, H, l, ctl);
     a.U = U;
     a.F = F;
-    a.out = L->d;
+    a.out = d;
     level_op(h, H, l, OP_DEFECT, a);
-    if (L->vc_count++ % L->log_every == 0) {
-        k_log_sample<<<grid256(g.ndof), 256, 0, s>>>(L->d, L->samples, g.ndof, L->count, L->nsamp_max, ctl);
-        k_log_count<<<1, 64, 0, s>>>(L->count, L->nsamp_max, ctl);
+    if (!L->trained) {
+        k_log_count<<<1, 64, 0, s>>>(L->count, slot + 1, ctl);
+        return;
     }
-    if (!L->trained) return;
-    correction(h, H, l, ctl);
+    correction(h, H, l, d, ctl);
     OpArgsT<float> b = view_args(h, H, l, ctl);
     b.U = L->e;
     b.out = L->Ae;
     level_op(h, H, l, OP_MATVEC, b);
-    launch_dot_partial<float, float>(s, L->d, L->e, g.ndof, L->partials, ctl);
-    launch_dot_partial<float, float>(s, L->e, L->Ae, g.ndof, L->partials + kRedBlocks, ctl);
-    k_gamma<<<1, kBlock, 0, s>>>(L->sc, L->partials, L->partials + kRedBlocks, L->count, ctl);
-    k_axpy_gamma<<<grid256(g.ndof), 256, 0, s>>>(U, L->e, g.ndof, L->sc, ctl);
+    k_dot2f<<<kFeatBlocks, kBlock, 0, s>>>(d, L->e, L->Ae, g.ndof, L->partials, ctl);
+    k_step<<<grid256(g.ndof), 256, 0, s>>>(U, L->e, g.ndof, L->partials, L->sc, L->count, logged ? slot + 1 : 0,
+                                           ctl);
 }
 
 void learn_begin_solve(gputo_handle *h)
@@ -745,7 +979,8 @@ int learn_train(gputo_handle *h, int steps, double lr, double *loss_first, doubl
         const int smp = it % L->nsamp;
         launch_copy<float, float>(s, L->samples + (long)smp * g.ndof, L->d, g.ndof, nullptr);
         k_zero<<<grid256(L->nparam), 256, 0, s>>>(L->G, L->nparam);
-        correction(h, H, l, nullptr);
+        correction(h, H, l, L->d, nullptr);
+        k_scale_features<<<grid256(3 * g.nnod), 256, 0, s>>>(L->x0, g.nnod, L->sc);
         OpArgsT<float> b = view_args(h, H, l, nullptr);
         b.U = L->e;
         b.out = L->Ae;
@@ -860,7 +1095,7 @@ int gputo_learn_forward(gputo_handle *h, const double *x, double *y)
         LearnState *L = h->ln;
         const long n0 = L->d0.size();
         to_float(h->stream, x, L->x0, 4 * n0);
-        learn_forward(h, nullptr);
+        learn_forward(h, nullptr, nullptr);
         to_double(h->stream, L->y, y, 3 * n0);
     });
 }
@@ -874,7 +1109,7 @@ int gputo_learn_grad(gputo_handle *h, const double *x, const double *gy, double 
         const long n0 = L->d0.size();
         hipStream_t s = h->stream;
         to_float(s, x, L->x0, 4 * n0);
-        learn_forward(h, nullptr);
+        learn_forward(h, nullptr, nullptr);
         to_float(s, gy, L->gy, 3 * n0);
         k_zero<<<grid256(L->nparam), 256, 0, s>>>(L->G, L->nparam);
         learn_backward(h);

@@ -55,6 +55,31 @@ def test_network_forward_and_gradient_match_cpu_oracle(net_solver, seed):
         off += n
 
 
+@pytest.mark.parametrize("shape,nl", [((96, 48, 48), 4), ((192, 96, 96), 5)])
+def test_network_on_larger_grids(gputo, shape, nl):
+    """The level-2 grids of larger meshes (25x13x13 and 49x25x25 nodes),
+    where the convolutions split their positions over many blocks and their
+    input channels over fewer waves than on the small grid."""
+    s = gputo.Solver(*shape, nl, precision=1, learned=True)
+    try:
+        rng = np.random.default_rng(7)
+        p = rng.normal(size=learn_oracle.nparams()) * 0.05
+        s.learn_params(p)
+        n0, n1, n2 = s.learn_info()["grid"]
+        assert (n0, n1, n2) == (shape[0] // 4 + 1, shape[2] // 4 + 1, shape[1] // 4 + 1)
+        x = rng.normal(size=(4, n0, n1, n2))
+        gy = rng.normal(size=(3, n0, n1, n2))
+        p32 = s.learn_params()
+        y_ref, g_ref = learn_oracle.run(p32, x.astype(np.float32).astype(np.float64), gy.astype(np.float32))
+        y = s.learn_forward(x)
+        print(shape, "forward", rel(y, y_ref))
+        assert rel(y, y_ref) < 1e-5
+        g = s.learn_grad(x, gy)
+        assert rel(g, g_ref) < 1e-4
+    finally:
+        s.close()
+
+
 def test_training_lowers_the_level2_energy(gputo):
     nelx, nely, nelz, nl = 48, 24, 24, 4
     Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=41, kind="smooth")
