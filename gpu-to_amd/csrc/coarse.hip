@@ -460,8 +460,12 @@ __device__ __forceinline__ bool potrf_tile(double *T, long ld)
             bad = true;
             d = 1.0;
         }
-        d = sqrt(d);
-        const double lj = r == jj ? d : (r > jj ? L[jj] / d : 0.0);
+        // y = 1/sqrt(d): v_rsq_f64 (2^-29 relative) and one Newton step;
+        // the pivot is d y and the column is scaled by y, so the chain of a
+        // column has no square root or division in it
+        double y = __builtin_amdgcn_rsq(d);
+        y = y * fma(-0.5 * d * y, y, 1.5);
+        const double lj = r == jj ? d * y : (r > jj ? L[jj] * y : 0.0);
         L[jj] = lj;
         if (jj + 1 < NB) {
             col[lane] = lj;
@@ -492,13 +496,18 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double *A, int N, int k0, int
     if (bad && threadIdx.x == 0) flag[0] = 1;
 }
 
+// Panel rows solve x L^T = p: x[jj] = (p[jj] - sum_{m<jj} x[m] L[jj][m]) rd[jj]
+// with rd[jj] = 1/L[jj][jj] computed once per tile.
 __global__ __launch_bounds__(256) void k_trsm_panel(double *A, int N, int k0, int rows)
 {
     __shared__ double L[NB][NB + 1];
+    __shared__ double rd[NB];
     for (int q = threadIdx.x; q < NB * NB; q += 256) {
         const int r = q / NB, c = q % NB;
         L[r][c] = c <= r ? A[(long)(k0 + r) * N + k0 + c] : 0.0;
     }
+    __syncthreads();
+    if (threadIdx.x < NB) rd[threadIdx.x] = 1.0 / L[threadIdx.x][threadIdx.x];
     __syncthreads();
     const long row = (long)k0 + NB + (long)blockIdx.x * 256 + threadIdx.x;
     if (row >= (long)k0 + NB + rows) return;
@@ -510,7 +519,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double *A, int N, int k0, in
         double s = x[jj];
 #pragma unroll
         for (int m = 0; m < jj; ++m) s -= x[m] * L[jj][m];
-        x[jj] = s / L[jj][jj];
+        x[jj] = s * rd[jj];
     }
 #pragma unroll
     for (int c = 0; c < NB; ++c) A[row * N + k0 + c] = x[c];
@@ -597,6 +606,7 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_band(double *A, int N, in
 {
     __shared__ double T[NB][NB + 1];
     __shared__ double P[kBandRowsMax][NB + 1];
+    __shared__ double rdg[NB];  // 1/T[jj][jj]
     const int tid = threadIdx.x;
     const int nb = N / NB;
     bool bad = false;
@@ -631,29 +641,36 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_band(double *A, int N, in
         __syncthreads();
         if (tid < 64) {
             bad |= potrf_tile(&T[0][0], NB + 1);
-            if (tid < NB)
-                for (int c = 0; c <= tid; ++c) A[(long)(k0 + tid) * N + k0 + c] = T[tid][c];
+            if (tid < NB) rdg[tid] = 1.0 / T[tid][tid];
         }
         __syncthreads();
+        // The panel solve (one row per thread) and the coalesced store of
+        // the factored tile (row-contiguous, by the threads past the panel
+        // rows) run side by side.  Column-oriented substitution: x[m]
+        // receives its updates in the same order (and roundings) as the
+        // row-oriented panel kernel.
         if (tid < rows) {
             double x[NB];
 #pragma unroll
             for (int c = 0; c < NB; ++c) x[c] = P[tid][c];
-            // column-oriented substitution: x[m] receives its updates in the
-            // same order (and roundings) as the row-oriented panel kernel
 #pragma unroll
             for (int jj = 0; jj < NB; ++jj) {
-                x[jj] = x[jj] / T[jj][jj];
+                x[jj] *= rdg[jj];
 #pragma unroll
                 for (int m = jj + 1; m < NB; ++m) x[m] -= x[jj] * T[m][jj];
             }
 #pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                P[tid][c] = x[c];
-                A[(long)(k0 + NB + tid) * N + k0 + c] = x[c];
+            for (int c = 0; c < NB; ++c) P[tid][c] = x[c];
+        } else if (tid >= kBandRowsMax) {
+            for (int q = tid - kBandRowsMax; q < NB * NB; q += kCholThreads - kBandRowsMax) {
+                const int r = q / NB, c = q % NB;
+                if (c <= r) A[(long)(k0 + r) * N + k0 + c] = T[r][c];
             }
         }
         __syncthreads();
+        // coalesced store of the solved panel; the trailing update below
+        // reads it from LDS and writes only columns past it
+        for (int q = tid; q < rows * NB; q += kCholThreads) A[(long)(k0 + NB + q / NB) * N + k0 + q % NB] = P[q / NB][q % NB];
         const int nt = rows / NB;
         const int nsub = nt * (nt + 1) / 2 * 64;  // 64 4x4 blocks per tile
         for (int w = tid; w < nsub; w += kCholThreads) {
