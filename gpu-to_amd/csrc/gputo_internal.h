@@ -65,6 +65,7 @@ struct OpArgsT {
     const T *Ke;           // 24x24 row-major (level 0)
     const T *CX;           // element scale (level 0)
     const T *A;            // packed 300 x nel entry-major element matrices (levels >= 1)
+    const T *S;            // assembled node stencils [27][3][3][nnod] of the same operator, or null
     const T *U;            // input vector
     const T *F;            // right-hand side (defect / Jacobi)
     const T *AD;           // diagonal (Jacobi)
@@ -107,6 +108,7 @@ struct Level {
     double *CXl = nullptr;        // element scales of a matrix-free coarse level (mgcg9)
     double *U = nullptr, *F = nullptr, *R = nullptr, *AD = nullptr, *T = nullptr;
     double *A = nullptr;          // packed element matrices (levels >= 1)
+    double *S = nullptr;          // the level's operator as node stencils (stored levels below the coarsest)
     uint8_t *mask = nullptr;      // fixed-DOF node mask
     int *fixed_d = nullptr;       // fixed list (0-based DOF) on the device
     std::vector<int> fixed;       // fixed list (0-based DOF), reference order
@@ -179,6 +181,9 @@ template <class T> void launch_op0(hipStream_t s, int mode, const OpArgsT<T> &a)
 template <class T> void launch_opA(hipStream_t s, int mode, const OpArgsT<T> &a);
 void launch_diag0(hipStream_t s, const Grid &g, const double *Ke, const double *CX, double *AD);
 void launch_diagA(hipStream_t s, const Grid &g, const double *A, double *AD);
+// S = the operator of the packed element matrices A as node stencils
+void launch_stencil_build(hipStream_t s, const Grid &g, const double *A, double *S);
+constexpr int kStencilEntries = 243;  // 27 neighbours x 3 x 3 per node
 template <class T>
 void launch_restrict(hipStream_t s, const Grid &gc, const T *Rf, T *Fc, const uint8_t *maskc, const int *ctl);
 template <class T>

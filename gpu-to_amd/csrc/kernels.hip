@@ -418,9 +418,149 @@ __global__ __launch_bounds__(kBlock) void k_stored(OpArgsT<T> a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same operator assembled per node: block b = 9 (di+1) + 3 (dk+1) +
+// (dj+1) of node (i, k, j) couples it to node (i+di, k+dk, j+dj) and holds
+// the sum of the 3x3 sub-blocks of every element containing both nodes,
+// S[(9 b + 3 r + c) nnod + node].  Zero blocks stand for neighbours outside
+// the grid.  Built once per setup from the element matrices (elements
+// summed in a fixed order), applied by k_stencil with 81 coalesced matrix
+// loads and 27 neighbour loads per node, where the element form reads 576
+// matrix entries and 192 neighbour values per node.
+// ---------------------------------------------------------------------------
+// one thread per (node, block b = blockIdx.y)
+__global__ void k_stencil_build(Grid g, const double *__restrict__ A, double *__restrict__ S)
+{
+    const long nnod = g.nnod;
+    const long nod = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nod >= nnod) return;
+    const int b = blockIdx.y;
+    const int j = (int)(nod % g.nny), k = (int)((nod / g.nny) % g.nnz), i = (int)(nod / ((long)g.nny * g.nnz));
+    const int di = b / 9 - 1, dk = (b / 3) % 3 - 1, dj = b % 3 - 1;
+    double sb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) sb[q] = 0.0;
+#pragma unroll
+    for (int ei = 0; ei < 2; ++ei)
+#pragma unroll
+        for (int ek = 0; ek < 2; ++ek)
+#pragma unroll
+            for (int ej = 0; ej < 2; ++ej) {
+                const int ex = i - 1 + ei, ez = k - 1 + ek, ey = j - 1 + ej;
+                const int bx = di + 1 - ei, bz = dk + 1 - ek, by = dj + 1 - ej;
+                if (ex < 0 || ex >= g.nx || ey < 0 || ey >= g.ny || ez < 0 || ez >= g.nz) continue;
+                if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+                const long e = (long)g.ny * g.nz * ex + (long)g.ny * ez + ey;
+                const int la = lidx(1 - ei, 1 - ej, 1 - ek), lb = lidx(bx, by, bz);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) sb[3 * r + c] += A[(long)pidx(3 * la + r, 3 * lb + c) * g.nel + e];
+            }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) S[(long)(9 * b + q) * nnod + nod] = sb[q];
+}
+
+void launch_stencil_build(hipStream_t s, const Grid &g, const double *A, double *S)
+{
+    k_stencil_build<<<dim3((unsigned)((g.nnod + 255) / 256), 27), 256, 0, s>>>(g, A, S);
+}
+
+// Three waves per 64 consecutive nodes: wave w sums the 9 blocks of the
+// neighbour plane di = w - 1 (FP32: all 108 loads in flight together; FP64:
+// one dk row of 36 at a time), the three plane sums meet in LDS and wave 0
+// adds them in plane order and finishes as k_stored does.
+template <class T, int MODE>
+__global__ __launch_bounds__(192) void k_stencil(OpArgsT<T> a)
+{
+    if (a.ctl && a.ctl[0]) return;
+    __shared__ T red[3][3][64];
+    const Grid g = a.g;
+    const long nnod = g.nnod;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long p = (long)blockIdx.x * 64 + lane;
+    const bool live = p < nnod;
+    const long pc = live ? p : 0;
+    const int j = (int)(pc % g.nny), k = (int)((pc / g.nny) % g.nnz), i = (int)(pc / ((long)g.nny * g.nnz));
+    const int ii = i + w - 1;
+    T y0 = T(0), y1 = T(0), y2 = T(0);
+    if (live && ii >= 0 && ii < g.nnx) {
+        const T *__restrict__ Sp = a.S + (long)(81 * w) * nnod + p;  // blocks 9w .. 9w+8
+        constexpr int NR = sizeof(T) == 8 ? 1 : 3;  // dk rows per batch
+#pragma unroll
+        for (int r0 = 0; r0 < 3; r0 += NR) {
+            T sv[NR][3][9], uu[NR][3][3];
+#pragma unroll
+            for (int rr = 0; rr < NR; ++rr) {
+                const int dk = r0 + rr - 1;
+#pragma unroll
+                for (int c3 = 0; c3 < 3; ++c3) {
+                    const int dj = c3 - 1;
+                    const int kk = k + dk, jj = j + dj;
+                    const bool ok = kk >= 0 && kk < g.nnz && jj >= 0 && jj < g.nny;
+                    // outside the grid the stencil block is zero: read any valid node
+                    const long nb = ok ? (long)g.nny * g.nnz * ii + (long)g.nny * kk + jj : pc;
+                    const int bl = 3 * (r0 + rr) + c3;
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) sv[rr][c3][q] = Sp[(long)(9 * bl + q) * nnod];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) uu[rr][c3][c] = stored_u<T, MODE>(a, 3 * nb + c);
+                }
+            }
+#pragma unroll
+            for (int rr = 0; rr < NR; ++rr)
+#pragma unroll
+                for (int c3 = 0; c3 < 3; ++c3) {
+                    const T *m = sv[rr][c3];
+                    const T *u = uu[rr][c3];
+                    y0 += m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
+                    y1 += m[3] * u[0] + m[4] * u[1] + m[5] * u[2];
+                    y2 += m[6] * u[0] + m[7] * u[1] + m[8] * u[2];
+                }
+        }
+    }
+    red[w][0][lane] = y0;
+    red[w][1][lane] = y1;
+    red[w][2][lane] = y2;
+    __syncthreads();
+    if (w != 0 || !live) return;
+    const long dof = 3 * p;
+    const unsigned msk = a.mask[p];
+    const T om = T(a.omega), one = T(1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T yc = red[0][c][lane] + red[1][c][lane] + red[2][c][lane];
+        const bool fixed = (msk >> c) & 1u;
+        const T u = stored_u<T, MODE>(a, dof + c);
+        if (MODE == OP_MATVEC) {
+            a.out[dof + c] = fixed ? T(0) : yc;
+        } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
+            a.out[dof + c] = fixed ? T(0) : a.F[dof + c] - yc;
+            if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = u;
+        } else {
+            const T f = a.F[dof + c];
+            const T rv = fixed ? T(0) : f - yc;
+            const T ad = a.AD[dof + c];
+            a.out[dof + c] = (one - om) * u + om * (rv + ad * u) / ad;
+        }
+    }
+}
+
 template <class T>
 void launch_opA(hipStream_t s, int mode, const OpArgsT<T> &a)
 {
+    if (a.S && !a.partials) {
+        // the stencil form (its launches carry no fused dot)
+        const dim3 sg((unsigned)((a.g.nnod + 63) / 64)), sb(192);
+        switch (mode) {
+        case OP_MATVEC: k_stencil<T, OP_MATVEC><<<sg, sb, 0, s>>>(a); return;
+        case OP_DEFECT: k_stencil<T, OP_DEFECT><<<sg, sb, 0, s>>>(a); return;
+        case OP_JACOBI: k_stencil<T, OP_JACOBI><<<sg, sb, 0, s>>>(a); return;
+        case OP_PRE_JACOBI: k_stencil<T, OP_PRE_JACOBI><<<sg, sb, 0, s>>>(a); return;
+        case OP_PRE_DEFECT: k_stencil<T, OP_PRE_DEFECT><<<sg, sb, 0, s>>>(a); return;
+        default: return;
+        }
+    }
     const dim3 grid((unsigned)a.t.blocks()), block(kBlock);
     switch (mode) {
     case OP_MATVEC: k_stored<T, OP_MATVEC><<<grid, block, 0, s>>>(a); break;

@@ -14,6 +14,7 @@ struct MfmaState;
 struct MixedState {
     HierView<float> H;                 // FP32 hierarchy view
     float *A32[kMaxLevels] = {};       // FP32 packed element matrices
+    float *S32[kMaxLevels] = {};       // FP32 node stencils of the levels that have them
     float *CXl32[kMaxLevels] = {};     // FP32 element scales of matrix-free coarse levels
     float *Ke32 = nullptr, *CX32 = nullptr;
     float *X = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr, *Z = nullptr;  // inner CG vectors

@@ -44,6 +44,7 @@ void mixed_create(gputo_handle *h)
             float *A = malloc_dev<float>(m, 300 * g.nel);
             m->A32[l] = A;
             v.A = A;
+            if (h->L[l].S) v.S = m->S32[l] = malloc_dev<float>(m, (long)kStencilEntries * g.nnod);
             v.CX = m->CXl32[l] = malloc_dev<float>(m, g.nel);
         }
     }
@@ -113,7 +114,11 @@ static void mixed_prepare(gputo_handle *h)
         if (m->H.mg9) {
             if (l > 0) launch_copy<double, float>(s, v.CXl, m->CXl32[l], v.g.nel, nullptr);
         } else if (l > 0 && !(l == 1 && m->H.l1_mf)) {
-            launch_copy<double, float>(s, v.A, m->A32[l], 300 * v.g.nel, nullptr);
+            // the stencil replaces the element matrices where it exists
+            if (m->S32[l])
+                launch_copy<double, float>(s, v.S, m->S32[l], (long)kStencilEntries * v.g.nnod, nullptr);
+            else
+                launch_copy<double, float>(s, v.A, m->A32[l], 300 * v.g.nel, nullptr);
         }
     }
     m->H.l0 = l0_kind(h->prm, h->prm.precision);

@@ -223,6 +223,7 @@ HierView<double> f64_view(const gputo_handle *h)
         H.L[l].AD = v.AD;
         H.L[l].Tmp = v.T;
         H.L[l].A = v.A;
+        H.L[l].S = v.S;
         H.L[l].CX = l == 0 ? h->CX : v.CXl;
     }
     H.Ke = h->Ke;
@@ -377,6 +378,7 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
         }
         launch_diagA(s, v.g, v.A, v.AD);
         if (per) launch_per_fold<double>(s, v.g, v.AD, 1, nullptr);
+        if (v.S) launch_stencil_build(s, v.g, v.A, v.S);
     }
     HIPCHK(hipGetLastError());
     // overlaps the solve (lagged) or the first V-cycle's descent (exact);
@@ -610,6 +612,9 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
                 v.F = dalloc<double>(h, v.g.ndof);
                 v.A = dalloc<double>(h, 300 * v.g.nel);
                 v.CXl = dalloc<double>(h, v.g.nel);
+                // node stencils of the stored levels the V-cycle smooths
+                if (l < nl - 1 && !(l == 1 && use_l1_mf(h->prm, nl)))
+                    v.S = dalloc<double>(h, (long)kStencilEntries * v.g.nnod);
             }
         }
         const Grid &g0 = h->L[0].g;

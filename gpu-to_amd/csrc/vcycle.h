@@ -38,6 +38,7 @@ template <class T>
 struct LevelView {
     T *U = nullptr, *F = nullptr, *R = nullptr, *AD = nullptr, *Tmp = nullptr;
     const T *A = nullptr;   // packed element matrices (levels >= 1)
+    const T *S = nullptr;   // node stencils of a stored level, or null
     const T *CX = nullptr;  // element scales of a matrix-free level (mgcg9)
 };
 
@@ -87,6 +88,7 @@ OpArgsT<T> view_args(const gputo_handle *h, const HierView<T> &H, int l, const i
     a.Ke = H.Ke;
     a.CX = H.CX;
     a.A = H.L[l].A;
+    a.S = H.L[l].S;
     a.mask = h->L[l].mask;
     a.AD = H.L[l].AD;
     a.ctl = ctl;
