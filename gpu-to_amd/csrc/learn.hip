@@ -79,6 +79,10 @@ __device__ __forceinline__ void stage_lds(float *__restrict__ dst, int n, const 
 //    skip[c][q] for the up to 8 fine nodes q with parent p;
 //  * corr (conv 6): the output is also turned into the level-2 correction
 //    corr[3p+c] = fixed ? 0 : sigma y[c][p] / sqrt(ad[3p+c]).
+// input channels whose gathers a thread issues together (one memory
+// latency per group instead of per channel)
+constexpr int kConvGroup = 4;
+
 struct ConvFwd {
     const float *in;
     Dims3 din;
@@ -182,13 +186,22 @@ __global__ __launch_bounds__(64 * KS) void k_conv_fwd(ConvFwd a)
     float acc[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[c] = 0.0f;
-    for (int ci = wv; ci < a.cin; ci += KS) {
-        const float *src = a.in + (long)ci * nin;
-        // unconditional loads (off = 0 outside the grid), masked after:
-        // all 27 loads of a channel are in flight together
-        float v[27];
+    for (int cg = wv; cg < a.cin; cg += kConvGroup * KS) {
+        // unconditional loads (off = 0 outside the grid), masked after: the
+        // 27 loads of up to kConvGroup channels are in flight together
+        float vg[kConvGroup][27];
 #pragma unroll
-        for (int t = 0; t < 27; ++t) v[t] = src[off[t]];
+        for (int u = 0; u < kConvGroup; ++u) {
+            const int ci = cg + u * KS;
+            const float *src = a.in + (long)(ci < a.cin ? ci : 0) * nin;
+#pragma unroll
+            for (int t = 0; t < 27; ++t) vg[u][t] = src[off[t]];
+        }
+#pragma unroll
+        for (int u = 0; u < kConvGroup; ++u) {
+        const int ci = cg + u * KS;
+        if (ci >= a.cin) break;
+        const float *v = vg[u];
         const float sc = ci < 3 ? inv : 1.0f;
         const float *wc = lds + ci * 27 * CT;
 #pragma unroll
@@ -207,6 +220,7 @@ __global__ __launch_bounds__(64 * KS) void k_conv_fwd(ConvFwd a)
 #pragma unroll
                 for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], vt, acc[c]);
             }
+        }
         }
     }
     float *part = lds;
@@ -269,18 +283,28 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lanes(ConvFwd a)
             valid |= (unsigned)v << t;
             off[t] = v ? (ii * a.din.n1 + ik) * a.din.n2 + ij : 0;
         }
-        for (int ci = ks; ci < cin; ci += KS) {
-            const float *src = a.in + (long)ci * nin;
-            const float *wc = wl + ci * 27 * CT;
-            float v[27];
+        for (int cg = ks; cg < cin; cg += kConvGroup * KS) {
+            // the 27 loads of up to kConvGroup channels in flight together
+            float vg[kConvGroup][27];
 #pragma unroll
-            for (int t = 0; t < 27; ++t) v[t] = src[off[t]];
-            const float sc = ci < 3 ? inv : 1.0f;
+            for (int u = 0; u < kConvGroup; ++u) {
+                const int ci = cg + u * KS;
+                const float *src = a.in + (long)(ci < cin ? ci : 0) * nin;
 #pragma unroll
-            for (int t = 0; t < 27; ++t) {
-                const float vt = ((valid >> t) & 1u) ? v[t] * sc : 0.0f;
+                for (int t = 0; t < 27; ++t) vg[u][t] = src[off[t]];
+            }
 #pragma unroll
-                for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], vt, acc[c]);
+            for (int u = 0; u < kConvGroup; ++u) {
+                const int ci = cg + u * KS;
+                if (ci >= cin) break;
+                const float *wc = wl + ci * 27 * CT;
+                const float sc = ci < 3 ? inv : 1.0f;
+#pragma unroll
+                for (int t = 0; t < 27; ++t) {
+                    const float vt = ((valid >> t) & 1u) ? vg[u][t] * sc : 0.0f;
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) acc[c] = fmaf(wc[t * CT + c], vt, acc[c]);
+                }
             }
         }
     }
