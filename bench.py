@@ -26,7 +26,8 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "level0_operator_cantilever_192x96x
 # gputo_params.l0_kernel (0 = FP64 node-gather for the plain path)
 # name: (precision, level-0 kernel, sweeps before / after the coarse
 # correction on levels 0 and 1 of the FP32 V-cycle (0 = the preset's 3 / 3),
-# coarse_lag: V-cycles that use the previous design's coarsest factor)
+# coarse_lag: V-cycles that use the previous design's coarsest factor[,
+# level-1 sweeps before / after when they differ from level 0's])
 VARIANTS = {
     "f64": (0, 0, 0, 0, 0),
     "mixed_gather": (1, 1, 0, 0, 0),  # FP32 inner solve, FP32 node-gather level-0 operator
@@ -36,7 +37,18 @@ VARIANTS = {
     "mixed_mfma1": (2, 5, 0, 0, 0),   # MFMA level-0 operator, one BF16 product
     "mixed_mfma3s": (2, 3, 1, 3, 0),  # as mixed_mfma3 with 1 + 3 sweeps on levels 0 and 1
     "mixed_mfma3s3": (2, 3, 1, 3, 3),  # and the previous coarsest factor in the first 3 V-cycles
+    "mixed_mfma3s11": (2, 3, 1, 1, 0),  # 1 + 1 sweeps on levels 0 and 1
+    "mixed_mfma3s22": (2, 3, 2, 2, 0),  # 2 + 2 sweeps on levels 0 and 1
+    "mixed_mfma3s11l3": (2, 3, 1, 1, 3),  # 1 + 1, previous coarsest factor in the first 3 V-cycles
+    "mixed_mfma3s1113": (2, 3, 1, 1, 0, 1, 3),  # 1 + 1 on level 0, 1 + 3 on level 1
+    "mixed_mfma3s1122": (2, 3, 1, 1, 0, 2, 2),  # 1 + 1 on level 0, 2 + 2 on level 1
 }
+
+
+def level1_sweeps(v):
+    """(pre, post) sweeps of level 1 of a variant tuple: its own entries when
+    it has them, else level 0's."""
+    return (v[5], v[6]) if len(v) > 5 else (v[2], v[3])
 
 
 def pmc_traffic(kernel):
@@ -58,10 +70,13 @@ def pmc_traffic(kernel):
 def level0_sweeps(name):
     """'pre+post' Jacobi sweeps on levels 0 and 1 of a mixed variant's FP32
     V-cycle (None for the FP64 variant, whose V-cycle is the preset's)."""
-    prec, _, npre0, npost0, _ = VARIANTS[name.replace("+learned", "")]
-    if prec == 0:
+    v = VARIANTS[name.replace("+learned", "")]
+    if v[0] == 0:
         return None
-    return "%d+%d" % (npre0 or 3, npost0 or 3)
+    n1 = level1_sweeps(v)
+    s0 = "%d+%d" % (v[2] or 3, v[3] or 3)
+    s1 = "%d+%d" % (n1[0] or 3, n1[1] or 3)
+    return s0 if s0 == s1 else "%s on level 0, %s on level 1" % (s0, s1)
 
 
 def log(*a):
@@ -69,11 +84,13 @@ def log(*a):
 
 
 def run_variant(gputo, name, args, Ke, F, fixed, x0):
-    prec, kernel, npre0, npost0, lag = VARIANTS[name.replace("+learned", "")]
+    v = VARIANTS[name.replace("+learned", "")]
+    prec, kernel, npre0, npost0, lag = v[:5]
+    npre1, npost1 = level1_sweeps(v)
     learned = name.endswith("+learned")
     s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=learned,
-                     l0_kernel=kernel, mixed_npre0=npre0, mixed_npost0=npost0, mixed_npre1=npre0,
-                     mixed_npost1=npost0, coarse_lag=lag)
+                     l0_kernel=kernel, mixed_npre0=npre0, mixed_npost0=npost0, mixed_npre1=npre1,
+                     mixed_npost1=npost1, coarse_lag=lag)
     run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
                        cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
     train_s = 0.0
@@ -188,7 +205,7 @@ def main():
     ap.add_argument("--nl", type=int, default=5)
     ap.add_argument("--cg-tol", type=float, default=1e-8)
     ap.add_argument("--cg-maxit", type=int, default=300)
-    ap.add_argument("--variants", default="f64,mixed_mfma3s,mixed_mfma3s+learned")
+    ap.add_argument("--variants", default="f64,mixed_mfma3s11,mixed_mfma3s11+learned")
     ap.add_argument("--op-reps", type=int, default=50)
     ap.add_argument("--learn-steps", type=int, default=300)
     ap.add_argument("--learn-lr", type=float, default=1e-3)
