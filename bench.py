@@ -27,7 +27,8 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "level0_operator_cantilever_192x96x
 # name: (precision, level-0 kernel, sweeps before / after the coarse
 # correction on levels 0 and 1 of the FP32 V-cycle (0 = the preset's 3 / 3),
 # coarse_lag: V-cycles that use the previous design's coarsest factor[,
-# level-1 sweeps before / after when they differ from level 0's])
+# level-1 sweeps before / after when they differ from level 0's[, sweeps
+# of the levels from 2 on]])
 VARIANTS = {
     "f64": (0, 0, 0, 0, 0),
     "mixed_gather": (1, 1, 0, 0, 0),  # FP32 inner solve, FP32 node-gather level-0 operator
@@ -42,7 +43,17 @@ VARIANTS = {
     "mixed_mfma3s11l3": (2, 3, 1, 1, 3),  # 1 + 1, previous coarsest factor in the first 3 V-cycles
     "mixed_mfma3s1113": (2, 3, 1, 1, 0, 1, 3),  # 1 + 1 on level 0, 1 + 3 on level 1
     "mixed_mfma3s1122": (2, 3, 1, 1, 0, 2, 2),  # 1 + 1 on level 0, 2 + 2 on level 1
+    "mixed_mfma3s111": (2, 3, 1, 1, 0, 1, 1, 1, 1),  # 1 + 1 on every level
+    "mixed_mfma3s1112": (2, 3, 1, 1, 0, 1, 1, 2, 2),  # 1 + 1 on levels 0 and 1, 2 + 2 below
+    "mixed_mfma3s1114": (2, 3, 1, 1, 0, 1, 1, 4, 4),  # 1 + 1 on levels 0 and 1, 4 + 4 below
+    "mixed_mfma3s1116": (2, 3, 1, 1, 0, 1, 1, 6, 6),  # 1 + 1 on levels 0 and 1, 6 + 6 below
+    "mixed_mfma3s1118": (2, 3, 1, 1, 0, 1, 1, 8, 8),  # 1 + 1 on levels 0 and 1, 8 + 8 below
 }
+
+
+def level2_sweeps(v):
+    """(pre, post) sweeps of the levels from 2 on (0, 0: the preset's)."""
+    return (v[7], v[8]) if len(v) > 7 else (0, 0)
 
 
 def level1_sweeps(v):
@@ -76,7 +87,11 @@ def level0_sweeps(name):
     n1 = level1_sweeps(v)
     s0 = "%d+%d" % (v[2] or 3, v[3] or 3)
     s1 = "%d+%d" % (n1[0] or 3, n1[1] or 3)
-    return s0 if s0 == s1 else "%s on level 0, %s on level 1" % (s0, s1)
+    out = s0 if s0 == s1 else "%s on level 0, %s on level 1" % (s0, s1)
+    n2 = level2_sweeps(v)
+    if n2[0] or n2[1]:
+        out += ", %d+%d on the levels from 2" % (n2[0] or 3, n2[1] or 3)
+    return out
 
 
 def log(*a):
@@ -87,10 +102,11 @@ def run_variant(gputo, name, args, Ke, F, fixed, x0):
     v = VARIANTS[name.replace("+learned", "")]
     prec, kernel, npre0, npost0, lag = v[:5]
     npre1, npost1 = level1_sweeps(v)
+    npre2, npost2 = level2_sweeps(v)
     learned = name.endswith("+learned")
     s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=learned,
                      l0_kernel=kernel, mixed_npre0=npre0, mixed_npost0=npost0, mixed_npre1=npre1,
-                     mixed_npost1=npost1, coarse_lag=lag)
+                     mixed_npost1=npost1, mixed_npre2=npre2, mixed_npost2=npost2, coarse_lag=lag)
     run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
                        cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
     train_s = 0.0
@@ -205,7 +221,7 @@ def main():
     ap.add_argument("--nl", type=int, default=5)
     ap.add_argument("--cg-tol", type=float, default=1e-8)
     ap.add_argument("--cg-maxit", type=int, default=300)
-    ap.add_argument("--variants", default="f64,mixed_mfma3s11,mixed_mfma3s11+learned")
+    ap.add_argument("--variants", default="f64,mixed_mfma3s1116,mixed_mfma3s1116+learned")
     ap.add_argument("--op-reps", type=int, default=50)
     ap.add_argument("--learn-steps", type=int, default=300)
     ap.add_argument("--learn-lr", type=float, default=1e-3)

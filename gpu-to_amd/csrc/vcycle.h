@@ -185,11 +185,12 @@ inline int mixed_sweeps(const gputo_params &prm, int l, bool post)
 {
     if (l == 0) return post ? prm.mixed_npost0 : prm.mixed_npre0;
     if (l == 1) return post ? prm.mixed_npost1 : prm.mixed_npre1;
-    return 0;
+    return post ? prm.mixed_npost2 : prm.mixed_npre2;
 }
 
 // Sweeps before / after the coarse correction on level l: the preset's,
-// or on levels 0 and 1 of the FP32 V-cycle the mixed_npre/npost params.
+// or in the FP32 V-cycle the mixed_npre/npost params of level 0, level 1
+// and the levels from 2 on.
 template <class T>
 int sweeps_of(const gputo_handle *h, int l, bool post)
 {

@@ -55,6 +55,8 @@ class GputoParams(ctypes.Structure):
         ("mixed_npost0", ctypes.c_int),
         ("mixed_npre1", ctypes.c_int),
         ("mixed_npost1", ctypes.c_int),
+        ("mixed_npre2", ctypes.c_int),
+        ("mixed_npost2", ctypes.c_int),
     ]
 
 

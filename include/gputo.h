@@ -107,6 +107,8 @@ typedef struct gputo_params {
     int mixed_npost0;
     int mixed_npre1;   /* the same for level 1 */
     int mixed_npost1;
+    int mixed_npre2;   /* the same for the levels from 2 down to the last above the coarsest */
+    int mixed_npost2;
 } gputo_params;
 
 /* Fill *p with the defaults (REF_C, F64, no learned stage, operator

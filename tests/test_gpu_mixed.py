@@ -74,19 +74,22 @@ def test_mixed_solve_matches_f64(gputo, oracle, prec, kernel, mesh):
 
 
 @pytest.mark.parametrize("kernel", [3, 1])
-@pytest.mark.parametrize("sweeps", [(1, 3), (2, 2), (1, 1)])
+@pytest.mark.parametrize("sweeps", [(1, 3), (2, 2), (1, 1), (1, 1, 6, 6), (1, 1, 2, 2)])
 def test_level0_sweep_schedule(gputo, oracle, sweeps, kernel):
-    """Fewer level-0 and level-1 sweeps in the FP32 V-cycle change the preconditioner,
-    not the solution: the FP64 true residual meets the tolerance and the
-    compliance matches the FP64 solve."""
+    """Other sweep counts on levels 0 and 1 (and, with four entries, on the
+    levels from 2) of the FP32 V-cycle change the preconditioner, not the
+    solution: the FP64 true residual meets the tolerance and the compliance
+    matches the FP64 solve."""
     nelx, nely, nelz, nl = 48, 24, 24, 4
     print("case", sweeps, kernel, flush=True)
     Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=23, kind="perturbed")
     s64 = gputo.Solver(nelx, nely, nelz, nl)
     U64, r64, n64, rc = s64.solve(Ke, F, CX, fixed, 1e-10, 300)
     assert rc == 0
+    below = sweeps[2:] if len(sweeps) > 2 else (0, 0)
     sm = gputo.Solver(nelx, nely, nelz, nl, precision=2, l0_kernel=kernel, mixed_npre0=sweeps[0],
-                      mixed_npost0=sweeps[1], mixed_npre1=sweeps[0], mixed_npost1=sweeps[1])
+                      mixed_npost0=sweeps[1], mixed_npre1=sweeps[0], mixed_npost1=sweeps[1],
+                      mixed_npre2=below[0], mixed_npost2=below[1])
     Um, rm, nm, rc = sm.solve(Ke, F, CX, fixed, 1e-10, 300)
     print("iterations", n64, nm, flush=True)
     assert rc == 0 and rm < 1e-10 and nm <= n64 + 10
