@@ -234,8 +234,10 @@ void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const
 void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
 void launch_cg_rho0(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
 void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
+// U += alpha P, R -= alpha Q, partials of R.R; R32 (if not null) gets R in FP32
 template <class T>
-void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials);
+void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials,
+                         float *R32 = nullptr);
 void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit);
 void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit);
 template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc);

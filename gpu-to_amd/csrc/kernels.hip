@@ -898,7 +898,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars *sc, const double
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ P,
                                                         const T *__restrict__ Q, long n, const CgScalars *sc,
-                                                        double *partials)
+                                                        double *partials, float *__restrict__ R32)
 {
     if (sc->done) return;
     const T al = (T)sc->alpha;
@@ -907,6 +907,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *_
         U[i] += al * P[i];
         const T r = R[i] - al * Q[i];
         R[i] = r;
+        if (R32) R32[i] = (float)r;  // the FP32 V-cycle's right-hand side
         s += (double)r * (double)r;
     }
     s = block_sum(s);
@@ -1014,14 +1015,15 @@ void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int n
     k_cg_alpha<<<1, kBlock, 0, s>>>(sc, partials, nparts);
 }
 template <class T>
-void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials)
+void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials,
+                         float *R32)
 {
-    k_cg_update_ur<T><<<kRedBlocks, kBlock, 0, s>>>(U, R, P, Q, n, sc, partials);
+    k_cg_update_ur<T><<<kRedBlocks, kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
 }
 template void launch_cg_update_ur<double>(hipStream_t, double *, double *, const double *, const double *, long,
-                                          CgScalars *, double *);
+                                          CgScalars *, double *, float *);
 template void launch_cg_update_ur<float>(hipStream_t, float *, float *, const float *, const float *, long,
-                                         CgScalars *, double *);
+                                         CgScalars *, double *, float *);
 void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit)
 {
     k_cg_check<<<1, kBlock, 0, s>>>(sc, partials, nparts, tol, maxit);

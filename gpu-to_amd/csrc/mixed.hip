@@ -130,15 +130,17 @@ static void mixed_prepare(gputo_handle *h)
 // z = V(r) with the FP32 V-cycle: r (FP64) is rounded to FP32, the whole
 // cycle runs on the FP32 copies of the hierarchy (FP64 coarsest Cholesky
 // inside), and z is widened back to FP64 for the FP64 Krylov recurrences.
-void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl)
+void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, bool r32_ready)
 {
     MixedState *m = h->mx;
     hipStream_t s = h->stream;
     const long n = h->L[0].g.ndof;
-    launch_copy<double, float>(s, r, m->R, n, ctl);
+    if (!r32_ready) launch_copy<double, float>(s, r, m->R, n, ctl);
     vcycle_t<float>(h, m->H, m->R, m->Z, ctl);
     launch_copy<float, double>(s, m->Z, z, n, ctl);
 }
+
+float *mixed_r32(gputo_handle *h) { return h->mx->R; }
 
 // Start of a mixed-precision solve: FP32 copies of the current hierarchy.
 void mixed_begin(gputo_handle *h)

@@ -416,12 +416,12 @@ static bool custom_schedule(const gputo_params &p)
 
 // Device-resident PCG (mgcg1.cu:235-280); U, F on the device.
 // z = M r: the FP64 V-cycle, or the FP32 V-cycle of the mixed precisions
-static void precond(gputo_handle *h, const double *r, double *z)
+static void precond(gputo_handle *h, const double *r, double *z, bool r32_ready = false)
 {
     if (h->prm.precision == GPUTO_PREC_F64)
         vcycle(h, r, z);
     else
-        mixed_vcycle(h, r, z, &h->sc->done);
+        mixed_vcycle(h, r, z, &h->sc->done, r32_ready);
 }
 
 int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit)
@@ -437,11 +437,13 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     // coarse_lag = n >= 2: the first n V-cycles use the previous setup's
     // coarsest factor, later ones the factor of this setup
     int vcycles = 0;
-    auto precond_n = [&](const double *r, double *z) {
+    auto precond_n = [&](const double *r, double *z, bool r32_ready) {
         if (h->prm.coarse_lag >= 2 && vcycles == h->prm.coarse_lag) coarse_switch(h->C, s);
-        precond(h, r, z);
+        precond(h, r, z, r32_ready);
         ++vcycles;
     };
+    // the residual update also writes the FP32 right-hand side of the mixed V-cycle
+    float *r32 = h->prm.precision != GPUTO_PREC_F64 ? mixed_r32(h) : nullptr;
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
@@ -452,7 +454,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     if (per) launch_per_fold<double>(s, L0.g, h->R, 0, nullptr);  // images' share to the masters
     launch_dot_partial(s, h->R, h->R, n, h->partials, nullptr);
     launch_cg_init(s, h->sc, h->partials, kRedBlocks);
-    precond_n(h->R, h->P);
+    precond_n(h->R, h->P, false);
     launch_copy(s, h->P, h->Z, n, ctl);
     launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
     launch_cg_rho0(s, h->sc, h->partials, kRedBlocks);
@@ -470,9 +472,9 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
             // (p is equal on images); the fold completes q for the update
             if (per) launch_per_fold<double>(s, L0.g, h->Q, 0, ctl);
             launch_cg_alpha(s, h->sc, h->partials, nmv);
-            launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials);
+            launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials, r32);
             launch_cg_check(s, h->sc, h->partials, kRedBlocks, tol, maxit);
-            precond_n(h->R, h->Z);
+            precond_n(h->R, h->Z, r32 != nullptr);
             if (flex) {
                 launch_cg_beta_flex(s, h->R, h->Z, h->Q, n, h->sc, h->partials, maxit);
             } else {
