@@ -11,7 +11,7 @@ from ._lib import check, f64, ptr
 class TopOpt:
     """Runs the design loop of the Cantilever drivers on one libgputo handle.
 
-    cfg keys (defaults are the BASELINE cantilever case): volfrac, penal,
+    cfg keys (defaults are the 192x96x96 cantilever benchmark's): volfrac, penal,
     rmin, eta, beta, move, Emin, E0, ft (1: density filter, 3: filter +
     projection), cg_tol, cg_maxit.
     """
