@@ -899,15 +899,28 @@ int coarse_check(Coarse &C)
     return flag ? GPUTO_ERR_NOT_SPD : 0;
 }
 
+void coarse_invalidate(Coarse &C)
+{
+    C.have = false;
+    C.invalid = true;
+}
+
 void coarse_adopt(Coarse &C)
 {
-    if (!C.pending) return;
+    const bool invalid = C.invalid;
+    C.invalid = false;
+    if (!C.pending) {
+        if (invalid) C.have = false;
+        return;
+    }
     int flag = 0;
     HIPCHK(hipMemcpyAsync(&flag, C.flag, sizeof(int), hipMemcpyDeviceToHost, C.side));
     HIPCHK(hipStreamSynchronize(C.side));  // launched a solve ago: normally finished
     C.pending = false;
-    if (flag) {
-        C.have = false;  // factor again, exactly, at this setup
+    if (flag || invalid) {
+        // a failed factor, or one of an earlier configuration: factor
+        // again, exactly, at this setup
+        C.have = false;
         return;
     }
     C.cur = C.target;

@@ -399,6 +399,17 @@ int gputo_to_begin(gputo_handle *h, const double *Ke0, const double *x0, const d
     if (!h || !Ke0 || !x0 || !F || !cfg || nfd < 0 || (nfd && !fixed)) return GPUTO_ERR_ARGUMENT;
     if (!(cfg->rmin > 0) || cfg->rmin > 8 || (cfg->ft != 1 && cfg->ft != 3)) return GPUTO_ERR_ARGUMENT;
     to_release(h);
+    // the fixed list is validated (finite, 1 <= v <= ndof) before any state
+    // is published: a rejected call leaves no design loop on the handle
+    std::vector<int> fx((size_t)nfd);
+    {
+        const double ndof = (double)h->L[0].g.ndof;
+        for (int n = 0; n < nfd; ++n) {
+            const double v = fixed[n];
+            if (!(v >= 1.0 && v <= ndof)) return GPUTO_ERR_ARGUMENT;  // NaN fails too
+            fx[n] = (int)v - 1;
+        }
+    }
     try {
         const Grid &g = h->L[0].g;
         ToState *t = new ToState();
@@ -410,11 +421,7 @@ int gputo_to_begin(gputo_handle *h, const double *Ke0, const double *x0, const d
         t->beta = cfg->beta;
         t->eta = cfg->eta;
         t->Ke.assign(Ke0, Ke0 + 576);
-        t->fixed.resize(nfd);
-        for (int n = 0; n < nfd; ++n) {
-            t->fixed[n] = (int)fixed[n] - 1;
-            if (t->fixed[n] < 0 || t->fixed[n] >= g.ndof) return GPUTO_ERR_ARGUMENT;
-        }
+        t->fixed = std::move(fx);
         const long ne = g.nel;
         t->x = talloc<double>(t, ne);
         t->xPhys = talloc<double>(t, ne);

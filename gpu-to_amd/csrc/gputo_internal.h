@@ -125,6 +125,8 @@ struct Coarse {
     int cur = 0;                // buffer of Linv/LinvT
     int target = 0;             // buffer the factorisation in flight writes
     bool have = false;          // buffers[cur] hold a factor of an earlier setup
+    bool invalid = false;       // set by a parameter or operator change: the next setup
+                                // drops a lagged factor still in flight and factors exactly
     bool solve_waits = true;    // solves wait for the factorisation in flight
     double *y = nullptr;     // work vector
     int *flag = nullptr;     // SPD failure flag
@@ -224,6 +226,9 @@ void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ);
 void coarse_factor(hipStream_t s, Coarse &C, bool lag);
 int coarse_check(Coarse &C);  // waits for an exact factorisation; GPUTO_ERR_NOT_SPD on a bad pivot
 void coarse_adopt(Coarse &C);  // before a setup: take a finished lagged factor
+// after a parameter or operator change: no factor of an earlier setup
+// (current or still in flight) is used again
+void coarse_invalidate(Coarse &C);
 // mid-solve: make the solves on s wait for the factorisation in flight and
 // use it from now on (no-op without a lagged factorisation)
 void coarse_switch(Coarse &C, hipStream_t s);

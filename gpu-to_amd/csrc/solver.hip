@@ -494,7 +494,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     if (maxit <= 0) *nit = 0;
     if (int frc = coarse_check(h->C)) {  // the solve used a failed factorisation
         h->hier_ready = false;
-        h->C.have = false;
+        coarse_invalidate(h->C);
         *nit = frc;
         if (h->prm.precision != GPUTO_PREC_F64) mixed_end(h, 0, 0);
         if (h->ln) {
@@ -700,7 +700,7 @@ int gputo_set_params(gputo_handle *h, const gputo_params *params)
         return GPUTO_ERR_ARGUMENT;  // the learned stage runs in the FP32 V-cycle
     if (!h->C.AQ && params->preset != GPUTO_PRESET_MGCG9) return GPUTO_ERR_WORKSPACE;  // no dense coarsest
     if (params->preset != h->prm.preset) h->hier_ready = false;
-    h->C.have = false;  // the next setup factors exactly
+    coarse_invalidate(h->C);  // the next setup factors exactly
     h->prm = *params;
     try {
         if (h->prm.precision != GPUTO_PREC_F64) mixed_create(h);
@@ -732,7 +732,7 @@ int gputo_setup(gputo_handle *h, const double *Ke0, const double *CX, const doub
         if (int rc = setup_hierarchy(h, fx)) return rc;
         if (int rc = coarse_check(h->C)) {
             h->hier_ready = false;
-            h->C.have = false;
+            coarse_invalidate(h->C);
             return rc;
         }
         return 0;
@@ -832,7 +832,7 @@ static int host_op(gputo_handle *h, int mode, const double *Ke0, const double *C
         HIPCHK(hipMemcpyAsync(y, h->Q, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         h->hier_ready = false;  // the level-0 mask was replaced
-        h->C.have = false;
+        coarse_invalidate(h->C);
         return 0;
     } catch (...) {
         return map_exception();
@@ -863,7 +863,7 @@ int gputo_matvec_mixed(gputo_handle *h, int kernel, const double *Ke0, const dou
         HIPCHK(hipMemcpyAsync(y, h->Q, g0.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         h->hier_ready = false;
-        h->C.have = false;
+        coarse_invalidate(h->C);
         return 0;
     } catch (...) {
         return map_exception();
@@ -1041,20 +1041,22 @@ namespace {
 // Launch one level-`level` operator application of mode `mode` (0 matvec,
 // 1 defect, 2 damped Jacobi) with operator `op` on device vectors.
 template <class T>
-void level_apply_t(gputo_handle *h, HierView<T> H, int level, int op, int mode, const T *u, const T *f, T *y)
+void level_apply_t(gputo_handle *h, HierView<T> H, int level, int op, int mode, const T *u, const T *f, T *y,
+                   T *tmp = nullptr)
 {
     H.l1_mf = (level == 1 && op == 2);
     OpArgsT<T> a = view_args(h, H, level, nullptr);
     a.U = u;
     a.F = f;
     a.out = y;
-    const int m = mode == 0 ? OP_MATVEC : mode == 1 ? OP_DEFECT : OP_JACOBI;
-    level_op(h, H, level, m, a);
+    a.out2 = tmp;
+    static const int kModes[5] = {OP_MATVEC, OP_DEFECT, OP_JACOBI, OP_PRE_JACOBI, OP_PRE_DEFECT};
+    level_op(h, H, level, kModes[mode], a);
 }
 
 int level_args_ok(const gputo_handle *h, int level, int op, int mode)
 {
-    if (!h || !h->hier_ready || level < 0 || level >= h->nl - 1 || mode < 0 || mode > 2) return 0;
+    if (!h || !h->hier_ready || level < 0 || level >= h->nl - 1 || mode < 0 || mode > 4) return 0;
     if (op == 2) return level == 1 && !preset_of(h->prm).l1_galerkin && !preset_of(h->prm).mg9;
     return op == 1;
 }
@@ -1077,10 +1079,10 @@ int gputo_level_apply(gputo_handle *h, int level, int op, int mode, int fp32, co
             MixedState *m = h->mx;
             launch_copy<double, float>(s, h->P, m->P, n, nullptr);
             launch_copy<double, float>(s, h->Z, m->X, n, nullptr);
-            level_apply_t<float>(h, H, level, op, mode, m->P, m->X, m->Q);
+            level_apply_t<float>(h, H, level, op, mode, m->P, m->X, m->Q, m->Z);
             launch_copy<float, double>(s, m->Q, h->Q, n, nullptr);
         } else {
-            level_apply_t<double>(h, f64_view(h), level, op, mode, h->P, h->Z, h->Q);
+            level_apply_t<double>(h, f64_view(h), level, op, mode, h->P, h->Z, h->Q, h->R);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(y, h->Q, n * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -163,8 +163,9 @@ class Solver:
 
     def level_apply(self, level, op, mode, u, f=None, fp32=False):
         """One level-`level` operator application (gputo_level_apply): mode 0
-        y = A u, 1 y = f - A u, 2 one damped-Jacobi sweep; op 1 stored, 2
-        matrix-free (level 1)."""
+        y = A u, 1 y = f - A u, 2 one damped-Jacobi sweep, 3 one sweep from
+        prerelax*f/AD, 4 f - A (prerelax*f/AD); op 1 stored (level 0: the
+        level-0 kernel), 2 matrix-free (level 1)."""
         y = np.zeros(len(u))
         check(self.lib.gputo_level_apply(self.h, level, op, mode, int(bool(fp32)), ptr(f64(u)),
                                          None if f is None else ptr(f64(f)), ptr(y)), "gputo_level_apply")

@@ -193,11 +193,16 @@ int gputo_bench_operator(gputo_handle *h, int kernel, int reps, double *avg_ms);
 /* One application of the operator of level `level` (0 <= level < nl-1) of
  * the hierarchy of the last solve / setup: y = A u (mode 0, masked; multAV,
  * Cantilever/omp/mgcg1.c:667-715), y = f - A u (mode 1; defectc,
- * Cantilever/TOP3D125MGCGCUDA/mgcg1.cu:1266-1324) or one damped-Jacobi
+ * Cantilever/TOP3D125MGCGCUDA/mgcg1.cu:1266-1324), one damped-Jacobi
  * sweep from u with the preset's omega (mode 2; defectc then Jacobc,
- * mgcg1.cu:1199-1206).  op 1: stored element matrices (level 0: the fine
- * node-gather); op 2: matrix-free level-1 operator (rediscretised level 1
- * only).  fp32 != 0 runs the FP32 kernels of the mixed path.  Host vectors. */
+ * mgcg1.cu:1199-1206), the same sweep from the prerelaxed iterate
+ * prerelax*f/AD instead of u (mode 3; preRelc then one sweep,
+ * mgcg1.cu:1190-1206) or the defect f - A (prerelax*f/AD) (mode 4; u is
+ * ignored in modes 3 and 4).  op 1: stored element matrices or node
+ * stencils (level 0: the level-0 kernel of the precision, for fp32 the
+ * handle's l0_kernel); op 2: matrix-free level-1 operator (rediscretised
+ * level 1 only).  fp32 != 0 runs the FP32 kernels of the mixed path.
+ * Host vectors. */
 int gputo_level_apply(gputo_handle *h, int level, int op, int mode, int fp32, const double *u, const double *f,
                       double *y);
 /* Mean duration (ms, HIP events on the handle's stream) of `reps`

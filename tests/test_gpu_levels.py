@@ -119,3 +119,90 @@ def test_level_apply_rejects_bad_arguments(gputo):
     with pytest.raises(gputo.GputoError):
         s.level_apply(2, 1, 0, np.zeros(3 * 3 * 2 * 2))  # the coarsest level is solved directly
     s.close()
+
+
+PRERELAX_REF_C = 0.7  # preRelc, mgcg1.cu:1190-1197
+
+
+def level_reference(oracle, A, ad, fd, nx, ny, nz, u, f, mode, omega=OMEGA_REF_C, prerelax=PRERELAX_REF_C):
+    """The oracle's product for gputo_level_apply's mode (A None: level 0)."""
+    if mode in (3, 4):
+        u = prerelax * f / ad
+    y = oracle.matvec_el(A, u, fd, nx, ny, nz) if A is not None else None
+    return y, u
+
+
+@pytest.mark.parametrize("mesh", [(32, 16, 16, 4), (32, 16, 16, 5), (48, 24, 24, 5)])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_stencil_levels_vs_oracle(gputo, oracle, mesh, fp32):
+    """The node-stencil operator that every V-cycle runs on levels 2..nl-2
+    (k_stencil, assembled by k_stencil_build from the Galerkin element
+    matrices) against the oracle's element matrices of the same level
+    applied element by element (defectc, mgcg1.cu:1266-1324), in all five
+    modes.  Tolerances as for level 1: 1e-13 FP64, 2e-6 FP32 (rel. to max|y|)."""
+    nelx, nely, nelz, nl = mesh
+    print("case", mesh, fp32, flush=True)
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=39, kind="smooth")
+    s = gputo.Solver(nelx, nely, nelz, nl, precision=1 if fp32 else 0)
+    s.setup(Ke, CX, fixed)
+    rng = np.random.default_rng(40)
+    tol = 2e-6 if fp32 else 1e-13
+    for l in range(2, nl - 1):
+        A, ad, fd = oracle.level_data(Ke, CX, fixed, nelx, nely, nelz, nl, l)
+        nx, ny, nz = nelx >> l, nely >> l, nelz >> l
+        u, f = rng.uniform(-1, 1, ad.size), rng.uniform(-1, 1, ad.size)
+        fx = fd.astype(np.int64) - 1
+        for mode in range(5):
+            uu = PRERELAX_REF_C * f / ad if mode in (3, 4) else u
+            y_or = oracle.matvec_el(A, uu, fd, nx, ny, nz)
+            if mode in (1, 4):
+                ref = f - y_or
+                ref[fx] = 0.0
+            elif mode in (2, 3):
+                r = f - y_or
+                r[fx] = 0.0
+                ref = (1 - OMEGA_REF_C) * uu + OMEGA_REF_C * (r + ad * uu) / ad
+            else:
+                ref = y_or
+            y = s.level_apply(l, 1, mode, u, f, fp32=fp32)
+            assert rel(y, ref) < tol, (l, mode, rel(y, ref))
+            if mode == 0:
+                assert np.array_equal(y, s.level_apply(l, 1, mode, u, f, fp32=fp32))  # deterministic
+    s.close()
+
+
+@pytest.mark.parametrize("kernel,tol", [(1, 2e-6), (2, 2e-6), (3, 1e-4), (4, 2e-6)])
+@pytest.mark.parametrize("mesh", [(16, 8, 8, 3), (40, 20, 12, 3)])
+def test_level0_fp32_kernel_modes_vs_oracle(gputo, oracle, kernel, tol, mesh):
+    """Every mode of the FP32 level-0 kernels (node-gather and the MFMA
+    flavours) against the FP64 oracle: matvec, defect, damped-Jacobi sweep,
+    sweep from the prerelaxed iterate, defect of the prerelaxed iterate
+    (multAV0c / defect0c / preRelc / Jacobc, mgcg1.cu:941-997,1190-1264).
+    Tolerances relative to max|y|: FP32 rounding (2e-6); the hi+lo split-BF16
+    flavour drops lo*lo (1e-4)."""
+    nelx, nely, nelz, nl = mesh
+    print("case", mesh, kernel, flush=True)
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=41, kind="smooth")
+    s = gputo.Solver(nelx, nely, nelz, nl, precision=2, l0_kernel=kernel)
+    s.setup(Ke, CX, fixed)
+    rng = np.random.default_rng(42)
+    ad = oracle.prepad0(Ke, CX, nelx, nely, nelz)
+    u, f = rng.uniform(-1, 1, ad.size), rng.uniform(-1, 1, ad.size)
+    fx = (fixed - 1).astype(np.int64)
+    for mode in range(5):
+        uu = PRERELAX_REF_C * f / ad if mode in (3, 4) else u
+        Ku = oracle.matvec0(Ke, CX, uu, np.zeros(0), nelx, nely, nelz)
+        if mode in (1, 4):
+            ref = f - Ku
+            ref[fx] = 0.0
+        elif mode in (2, 3):
+            r = f - Ku
+            r[fx] = 0.0
+            ref = (1 - OMEGA_REF_C) * uu + OMEGA_REF_C * (r + ad * uu) / ad
+        else:
+            ref = Ku
+            ref[fx] = 0.0
+        y = s.level_apply(0, 1, mode, u, f, fp32=True)
+        assert rel(y, ref) < tol, (mode, rel(y, ref))
+        assert np.array_equal(y, s.level_apply(0, 1, mode, u, f, fp32=True))  # deterministic
+    s.close()

@@ -39,11 +39,21 @@ def test_mgcg9_solve_matches_oracle(gputo, oracle, mesh, kind, nswp):
     nelx, nely, nelz, nl = mesh
     print("case", mesh, kind, nswp, flush=True)
     Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=52, kind=kind)
-    Uo, ro, no, rco = oracle.mgcg(Ke, F, CX, fixed, nelx, nely, nelz, nl, 1e-9, 400, preset=MGCG9, nswp=nswp)
     s = gputo.Solver(nelx, nely, nelz, nl, preset=MGCG9, nswp=nswp)
-    U, r, n, rc = s.solve(Ke, F, CX, fixed, 1e-9, 400)
-    assert rc == 0 and rco == 0
-    if no < 400:
+    if nswp == 1 and kind == "perturbed":
+        # one sweep each way with omega 0.33 leaves this preconditioner
+        # indefinite: the oracle stalls (relres 2.3e-4 after 400 iterations),
+        # so the two are compared over the same 20 capped iterations
+        Uo, ro, no, rco = oracle.mgcg(Ke, F, CX, fixed, nelx, nely, nelz, nl, 1e-12, 20, preset=MGCG9, nswp=nswp)
+        U, r, n, rc = s.solve(Ke, F, CX, fixed, 1e-12, 20)
+        assert rc == 0 and rco == 0 and n == no == 20
+        assert abs(r - ro) <= 1e-7 * ro
+        assert rel(U, Uo) < 1e-7
+    else:
+        Uo, ro, no, rco = oracle.mgcg(Ke, F, CX, fixed, nelx, nely, nelz, nl, 1e-9, 400, preset=MGCG9, nswp=nswp)
+        U, r, n, rc = s.solve(Ke, F, CX, fixed, 1e-9, 400)
+        assert rc == 0 and rco == 0
+        assert no < 400, "the oracle converges on every other case (16-86 iterations)"
         assert abs(n - no) <= 1 and r < 1e-9
         assert rel(U, Uo) < 1e-7
         assert abs(F @ U - F @ Uo) <= 1e-8 * abs(F @ Uo)

@@ -130,3 +130,28 @@ def test_lagged_coarse_factor(gputo, oracle, lag_mode):
         assert nlag <= ne + 4
     for s in (lag, exact, s64):
         s.close()
+
+
+def test_lagged_factor_dropped_after_parameter_change(gputo):
+    """A lagged coarsest factorisation still in flight when the parameters
+    change is not adopted: the first solve after the change factors exactly,
+    so it matches a fresh handle with the new parameters bit for bit."""
+    nelx, nely, nelz, nl = 32, 16, 16, 4
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=61, kind="perturbed")
+    x2 = gputo.problems.perturbed_density(nelx, nely, nelz, 0.15, seed=62)
+    CX2 = 1e-9 + x2 ** 3 * (1 - 1e-9)
+    s = gputo.Solver(nelx, nely, nelz, nl, precision=1, coarse_lag=1)
+    for cx in (CX, CX2):  # the second setup leaves a lagged factor pending
+        U, r, n, rc = s.solve(Ke, F, cx, fixed, 1e-10, 300)
+        assert rc == 0
+    # still lagged after the change: without the drop, this setup would use
+    # the pending factor of the old preset's hierarchy
+    s.set_params(precision=1, coarse_lag=1, preset=gputo.PRESET_MATLAB_CFG0)
+    U1, r1, n1, rc = s.solve(Ke, F, CX2, fixed, 1e-10, 300)
+    assert rc == 0
+    fresh = gputo.Solver(nelx, nely, nelz, nl, precision=1, coarse_lag=1, preset=gputo.PRESET_MATLAB_CFG0)
+    U2, r2, n2, rc = fresh.solve(Ke, F, CX2, fixed, 1e-10, 300)
+    assert rc == 0
+    assert n1 == n2 and np.array_equal(U1, U2)
+    s.close()
+    fresh.close()

@@ -175,14 +175,19 @@ def test_coarse_solve(solvers, oracle):
     assert rel(x, xr) < 1e-9
 
 
-@pytest.mark.parametrize("preset", [0, 1])
-def test_vcycle(solvers, oracle, preset):
-    nelx, nely, nelz, nl = 16, 8, 8, 3
+@pytest.mark.parametrize("preset", [0, 1, 2])
+@pytest.mark.parametrize("mesh", [(16, 8, 8, 3), (32, 16, 16, 4), (32, 16, 16, 5), (48, 24, 24, 4)])
+def test_vcycle(solvers, oracle, preset, mesh):
+    """One V-cycle of the FP64 hierarchy against the oracle's, including the
+    node-stencil levels 2..nl-2 (nl >= 4) and the mgcg9 engine (nswp = 2)."""
+    nelx, nely, nelz, nl = mesh
+    print("case", mesh, preset, flush=True)
     Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=10)
     s = solvers(nelx, nely, nelz, nl, preset)
     r = np.random.default_rng(11).uniform(-1, 1, s.ndof)
     r[(fixed - 1).astype(int)] = 0.0
     z = s.vcycle(Ke, CX, fixed, r)
+    oracle.lib().or_set_nswp(2)
     zo = oracle.vcycle_once(Ke, CX, fixed, nelx, nely, nelz, nl, r, preset=preset)
     assert rel(z, zo) < 1e-9
 

@@ -58,3 +58,30 @@ def test_design_loop_log(gputo, tmp_path):
     for k, (l, h) in enumerate(zip(lines, hist)):
         assert l.startswith("It.:%5i C:%6.5e Time:" % (k + 1, h["compliance"]))
         assert "cgiters: %3d MemUsed:" % h["cg_iters"] in l
+
+
+@pytest.mark.parametrize("bad", [0.0, -3.0, float("nan"), 1e12])
+def test_to_begin_rejects_bad_fixed_list(gputo, bad):
+    """A fixed list with an entry outside 1..ndof (or NaN) is rejected before
+    any design-loop state exists, and a following iterate call is rejected
+    too instead of launching kernels on unset buffers."""
+    import ctypes
+    from gputo import _lib
+    nelx, nely, nelz, nl = 8, 4, 4, 2
+    Ke, F, fixed = gputo.cantilever(nelx, nely, nelz)
+    s = gputo.Solver(nelx, nely, nelz, nl)
+    lib = s.lib
+    fx = np.array(fixed, dtype=np.float64)
+    fx[len(fx) // 2] = bad
+    cfg = _lib.ToConfig(volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), eta=0.4, beta=2.0, move=0.2, Emin=1e-9, E0=1.0,
+                        ft=1, cg_tol=1e-8, cg_maxit=100)
+    x0 = np.full(nelx * nely * nelz, 0.12)
+    rc = lib.gputo_to_begin(s.h, _lib.ptr(Ke), _lib.ptr(x0), _lib.ptr(F), _lib.ptr(fx), len(fx), ctypes.byref(cfg))
+    assert rc == _lib.GPUTO_ERR_ARGUMENT
+    st = _lib.ToStats()
+    assert lib.gputo_to_iterate(s.h, ctypes.byref(st)) == _lib.GPUTO_ERR_ARGUMENT
+    # the handle stays usable: a valid loop starts and iterates
+    run = gputo.TopOpt(s, Ke, x0, F, fixed, cg_tol=1e-8, cg_maxit=100)
+    assert run.iterate()["status"] == 0
+    run.close()
+    s.close()
