@@ -98,52 +98,129 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def pretrain(gputo, s, args, Ke, fixed, name):
+    """Training of the learned level-2 stage before the design loop, after the
+    config-3 protocol: the network learns from the level-2 defects that plain
+    MGCG solves of other cantilevers log (volume fraction ~ U[0.08, 0.3] with
+    a +-0.01 uniform perturbation, a unit downward line load along j at a
+    random height k of the free end face x = nelx); after each solve,
+    `pretrain_steps` Adam steps on its logged defects with its own level-2
+    operator.  A one-off cost, reported beside the design iterations."""
+    rng = np.random.default_rng(args.seed + 1000)
+    nel = args.nelx * args.nely * args.nelz
+    nn = (args.nely + 1) * (args.nelz + 1)
+    t0 = time.perf_counter()
+    samples = 0
+    loss = []
+    for p in range(args.pretrain_problems):
+        vf = rng.uniform(0.08, 0.3)
+        x = np.clip(vf + rng.uniform(-0.01, 0.01, nel), 1e-3, 1.0)
+        CX = 1e-9 + x ** 3 * (1 - 1e-9)
+        k = int(rng.integers(0, args.nelz + 1))
+        F = np.zeros(3 * (args.nelx + 1) * nn)
+        nodes = args.nelx * nn + (args.nely + 1) * k + np.arange(args.nely + 1)
+        F[3 * nodes + 2] = -1.0 / (args.nely + 1)
+        U, r, n, rc = s.solve(Ke, F, CX, fixed, args.cg_tol, args.cg_maxit)
+        if rc != 0:
+            raise RuntimeError("pretraining solve failed: %d" % rc)
+        samples += s.learn_info()["nsamples"]
+        l0, l1 = s.learn_train(args.pretrain_steps, args.learn_lr)
+        loss.append((l0, l1))
+    s.learn_logging(False)
+    s.lib.gputo_device_synchronize()
+    dt = time.perf_counter() - t0
+    log("[%s] pretraining: %d problems, %d logged level-2 defects, %d Adam steps, %.2fs" % (
+        name, args.pretrain_problems, samples, args.pretrain_problems * args.pretrain_steps, dt))
+    return {"seconds": dt, "problems": args.pretrain_problems, "samples": samples,
+            "adam_steps": args.pretrain_problems * args.pretrain_steps,
+            "energy_first_last": [loss[0][0], loss[-1][1]] if loss else None}
+
+
 def run_variant(gputo, name, args, Ke, F, fixed, x0):
+    """One design run of variant `name`: `warmup` untimed design iterations,
+    `steps` timed ones (bracketed by device synchronisation), then untimed
+    ones up to the `horizon` (each still timed by the library's clock).
+    The learned variant trains after each warm-up iteration on the level-2
+    defects its solve logged; that training time is counted into the timed
+    steps' cost (`s_per_iter` includes it, amortised over them).  The state
+    (xPhys, U, compliance) is kept at design iterations 1, 50 and 100."""
     v = VARIANTS[name.replace("+learned", "")]
     prec, kernel, npre0, npost0, lag = v[:5]
     npre1, npost1 = level1_sweeps(v)
     npre2, npost2 = level2_sweeps(v)
     learned = name.endswith("+learned")
-    s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=learned,
+    s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=2 if learned else 0,
                      l0_kernel=kernel, mixed_npre0=npre0, mixed_npost0=npost0, mixed_npre1=npre1,
                      mixed_npost1=npost1, mixed_npre2=npre2, mixed_npost2=npost2, coarse_lag=lag)
+    pre = pretrain(gputo, s, args, Ke, fixed, name) if learned else None
     run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
                        cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
-    train_s = 0.0
+    keep = {}
+
+    def snap(it, st):
+        if it + 1 in REPORT_ITERS:
+            x, xp, U = run.state()
+            keep[it + 1] = {"xPhys": xp, "U": U, "C": st["compliance"]}
+
+    train_s = 0.0  # training inside the design loop (none: the network is trained beforehand)
+    hist = []
     for w in range(args.warmup):
         st = run.iterate()
+        hist.append(st)
+        snap(len(hist) - 1, st)
         log("[%s] warmup %d: C=%.6e nit=%d solve=%.3fs total=%.3fs" % (name, w, st["compliance"], st["cg_iters"],
                                                                        st["seconds_solve"], st["seconds_total"]))
-        if learned:
-            # train on the level-2 defects this solve logged (outside the timed steps)
-            t0 = time.perf_counter()
-            l0, l1 = s.learn_train(args.learn_steps, args.learn_lr)
-            s.lib.gputo_device_synchronize()
-            train_s += time.perf_counter() - t0
-            log("[%s] training %d steps: energy %.4g -> %.4g (%.2fs)" % (name, args.learn_steps, l0, l1,
-                                                                        time.perf_counter() - t0))
     s.lib.gputo_device_synchronize()
     t0 = time.perf_counter()
-    hist = [run.iterate() for _ in range(args.steps)]
+    timed = []
+    for _ in range(args.steps):
+        st = run.iterate()
+        timed.append(st)
     s.lib.gputo_device_synchronize()
     wall = time.perf_counter() - t0
-    for k, st in enumerate(hist):
+    for st in timed:
+        hist.append(st)
+        snap(len(hist) - 1, st)
+    for k, st in enumerate(timed):
         log("[%s] step %d: C=%.8e nit=%d relres=%.2e solve=%.3fs total=%.3fs" % (
             name, k, st["compliance"], st["cg_iters"], st["relres"], st["seconds_solve"], st["seconds_total"]))
+    while len(hist) < args.horizon:
+        st = run.iterate()
+        hist.append(st)
+        snap(len(hist) - 1, st)
+        if len(hist) % 10 == 0:
+            log("[%s] design iteration %d: C=%.8e nit=%d total=%.3fs" % (name, len(hist), st["compliance"],
+                                                                         st["cg_iters"], st["seconds_total"]))
     x, xp, U = run.state()
+    if learned:
+        on = [h for h in hist if h["learned_accepted"] + h["learned_rejected"] > 0]
+        log("[%s] the learned stage ran in %d of %d design iterations" % (name, len(on), len(hist)))
     out = {
-        "s_per_iter": wall / args.steps,
-        "solve_s_per_iter": float(np.mean([h["seconds_solve"] for h in hist])),
-        "cg_iters": [h["cg_iters"] for h in hist],
-        "compliance": [h["compliance"] for h in hist],
-        "relres": [h["relres"] for h in hist],
-        "learned_accepted": [h["learned_accepted"] for h in hist],
-        "learned_rejected": [h["learned_rejected"] for h in hist],
+        "pretraining": pre,
+        "learned_on_iters": [k + 1 for k, h in enumerate(hist) if h["learned_accepted"] + h["learned_rejected"] > 0],
+        "s_per_iter": (wall + train_s) / args.steps,
+        "s_per_iter_excl_training": wall / args.steps,
         "train_s": train_s,
+        "solve_s_per_iter": float(np.mean([h["seconds_solve"] for h in timed])),
+        "cg_iters": [h["cg_iters"] for h in timed],
+        "compliance": [h["compliance"] for h in timed],
+        "relres": [h["relres"] for h in timed],
+        "learned_accepted": [h["learned_accepted"] for h in timed],
+        "learned_rejected": [h["learned_rejected"] for h in timed],
         "U": U,
+        "keep": keep,
+        "warmup_s": [h["seconds_total"] for h in hist[:args.warmup]],
+        "warmup_cg_iters": [h["cg_iters"] for h in hist[:args.warmup]],
     }
+    if len(hist) > args.warmup + args.steps:
+        out["horizon"] = {
+            "design_iters": len(hist),
+            "s_per_iter": float(np.mean([h["seconds_total"] for h in hist])),
+            "cg_iters_total": int(sum(h["cg_iters"] for h in hist)),
+            "cg_iters_every_10th": [h["cg_iters"] for h in hist[9::10]],
+        }
     # dominant kernel: the level-0 operator, timed with HIP events on the
-    # solver's stream on the hierarchy of the last timed solve
+    # solver's stream on the hierarchy of the last solve
     avg = ctypes.c_double()
     rc = s.lib.gputo_bench_operator(s.h, kernel, args.op_reps, ctypes.byref(avg))
     out["op_ms"] = avg.value if rc == 0 else None
@@ -152,32 +229,102 @@ def run_variant(gputo, name, args, Ke, F, fixed, x0):
     return out
 
 
-def cpu_baseline(args, Ke, F, fixed, x0, nit_avg):
-    """Oracle (C/OpenMP restatement of mgcg1.c + top3D125mod.m) on the host:
-    two design iterations from x0 with the CG capped at 1 and 3 iterations
-    give the fixed cost and the per-CG-iteration cost, extrapolated to the
-    GPU's measured CG count (same algorithm, same iteration counts)."""
+REPORT_ITERS = (1, 50, 100)
+
+
+def fp64_errors(gputo, args, Ke, F, fixed, res, base):
+    """Error of u and compliance of every reported result of a mixed variant
+    against the FP64 plain MGCG solve of the same design (per result), and
+    the drift of its design trajectory from the FP64 run's (trajectory)."""
+    out = {}
+    s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl)
+    for it, k in sorted(res["keep"].items()):
+        CX = 1e-9 + k["xPhys"] ** 3 * (1 - 1e-9)
+        U64, r64, n64, rc = s.solve(Ke, F, CX, fixed, args.cg_tol, args.cg_maxit)
+        U64[1::3] *= -1  # the MATLAB frame of gputo_mgcg1 back to the C frame of the loop state
+        C64 = float(F @ U64)
+        e = {"u_rel_err": float(np.linalg.norm(k["U"] - U64) / np.linalg.norm(U64)),
+             "compliance_rel_err": abs(k["C"] - C64) / abs(C64), "fp64_cg_iters": int(n64)}
+        if base is not None and it in base["keep"]:
+            b = base["keep"][it]
+            e["trajectory_compliance_rel_diff"] = abs(k["C"] - b["C"]) / abs(b["C"])
+            e["trajectory_max_abs_dxPhys"] = float(np.abs(k["xPhys"] - b["xPhys"]).max())
+        out[str(it)] = e
+    s.close()
+    return out
+
+
+def physical_cores():
+    """Physical cores of the host (lscpu's core,socket pairs), or None."""
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        return len({l for l in txt.splitlines() if l and not l.startswith("#")}) or None
+    except Exception:
+        return None
+
+
+def cpu_baseline(gputo, args, Ke, F, fixed, x0, base):
+    """The oracle (C/OpenMP restatement of Cantilever/omp/mgcg1.c and the
+    design steps of Cantilever/omp/top3D125mod.m) on the host cores, measured:
+    the first `cpu_iters` design iterations of the 192x96x96 run from the same
+    x0 (whole iterations, CG to the same tolerance), and the 20 design
+    iterations of the 48x24x24 cantilever (config 0: 4 levels, the MATLAB
+    omega 0.6 1+1 smoother, CG tolerance 1e-10) beside the GPU's FP64 run of
+    the same 20 iterations.  Threads: the reference's 16 (mgcg1.c:120-121),
+    fewer if the process may use fewer CPUs."""
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    threads = min(16, len(os.sched_getaffinity(0)))
     oracle.set_num_threads(threads)
-    times = {}
-    for cap in (1, 3):
-        st = oracle.DesignState(args.nelx, args.nely, args.nelz, x0)
+    st = oracle.DesignState(args.nelx, args.nely, args.nelz, x0)
+    times, nits = [], []
+    for it in range(args.cpu_iters):
         t0 = time.perf_counter()
-        o = oracle.design_iter(st, Ke, F, fixed, args.nl, args.cg_tol, cap, 0.12, 3.0, np.sqrt(3.0), 1, 2.0, 0.2)
-        times[cap] = time.perf_counter() - t0
-        log("[cpu] design iteration with CG capped at %d: %.2fs (rc %d)" % (cap, times[cap], o["rc"]))
-    t_it = (times[3] - times[1]) / 2
-    t_fixed = times[1] - t_it
+        o = oracle.design_iter(st, Ke, F, fixed, args.nl, args.cg_tol, args.cg_maxit, 0.12, 3.0, np.sqrt(3.0), 1,
+                               2.0, 0.2)
+        times.append(time.perf_counter() - t0)
+        nits.append(int(o["cgiters"]) + 1)
+        log("[cpu] design iteration %d at %dx%dx%d: %.2fs, CG %d (rc %d)" % (it + 1, args.nelx, args.nely, args.nelz,
+                                                                             times[-1], nits[-1], o["rc"]))
+    # config 0 on both sides
+    n0 = (48, 24, 24, 4)
+    K0, F0, fx0 = gputo.cantilever(*n0[:3])
+    xc0 = np.full(n0[0] * n0[1] * n0[2], 0.12)
+    st0 = oracle.DesignState(*n0[:3], xc0)
+    t0 = time.perf_counter()
+    for it in range(20):
+        o = oracle.design_iter(st0, K0, F0, fx0, n0[3], 1e-10, 500, 0.12, 3.0, np.sqrt(3.0), 1, 2.0, 0.2,
+                               preset=oracle.PRESET_MATLAB_CFG0)
+    cpu0 = (time.perf_counter() - t0) / 20
+    s0 = gputo.Solver(*n0, preset=gputo.PRESET_MATLAB_CFG0)
+    r0 = gputo.TopOpt(s0, K0, xc0, F0, fx0, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2, cg_tol=1e-10,
+                      cg_maxit=500)
+    s0.lib.gputo_device_synchronize()
+    t0 = time.perf_counter()
+    for it in range(20):
+        g = r0.iterate()
+    s0.lib.gputo_device_synchronize()
+    gpu0 = (time.perf_counter() - t0) / 20
+    r0.close()
+    s0.close()
+    log("[cpu] 48x24x24 config 0: oracle %.3fs, GPU FP64 %.4fs per design iteration" % (cpu0, gpu0))
+    n_same = min(args.cpu_iters, len(base["warmup_s"])) if base else 0
     return {
-        "value": t_fixed + nit_avg * t_it,
+        "value": float(np.mean(times)),
         "unit": "s/iter",
-        "cores": oracle.num_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": ("oracle design iteration at %dx%dx%d from x0 with CG capped at 1 and 3 iterations "
-                   "(%.1fs + %.1fs); per-CG-iteration %.3fs and fixed %.3fs extrapolated to the GPU's "
-                   "mean %.1f CG iterations" % (args.nelx, args.nely, args.nelz, times[1], times[3], t_it,
-                                                 t_fixed, nit_avg)),
+        "sample": ("oracle (C/OpenMP restatement of mgcg1.c + top3D125mod.m), design iterations 1-%d of the "
+                   "%dx%dx%d run from the same x0, each whole (CG %s to tol %g), %d threads; the GPU window's "
+                   "designs need more CG iterations, so this is a lower bound on the CPU's time there"
+                   % (args.cpu_iters, args.nelx, args.nely, args.nelz, "/".join(map(str, nits)), args.cg_tol,
+                      threads)),
+        "physical_cores": physical_cores(),
+        "threads": threads,
+        "design_s": times,
+        "cg_iters": nits,
+        "gpu_f64_same_designs_s_per_iter": float(np.mean(base["warmup_s"][:n_same])) if n_same else None,
+        "config0_48x24x24": {"iters": 20, "cpu_s_per_iter": cpu0, "gpu_f64_s_per_iter": gpu0},
     }
 
 
@@ -195,7 +342,7 @@ def probe_operator(args):
     for k in [int(v) for v in args.probe_op.split(",")]:
         rc = s.lib.gputo_bench_operator(s.h, k, 3, ctypes.byref(avg))
         log("level-0 kernel %d: %.1f us (rc %d)" % (k, avg.value * 1e3, rc))
-    hip = ctypes.CDLL("libamdhip64.so")
+    hip = ctypes.CDLL("libamdhip64.so.7")
     n = 256 << 20
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
@@ -223,14 +370,19 @@ def main():
     ap.add_argument("--cg-maxit", type=int, default=300)
     ap.add_argument("--variants", default="f64,mixed_mfma3s1116,mixed_mfma3s1116+learned")
     ap.add_argument("--op-reps", type=int, default=50)
-    ap.add_argument("--learn-steps", type=int, default=300)
+    ap.add_argument("--pretrain-problems", type=int, default=16)
+    ap.add_argument("--pretrain-steps", type=int, default=50)
     ap.add_argument("--learn-lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=3, help="whole oracle design iterations at the bench size")
+    ap.add_argument("--horizon", type=int, default=100,
+                    help="design iterations per variant run (the timed steps are inside it; 0: only those)")
     ap.add_argument("--probe-op", default="",
                     help="counter probe: a few launches of these level-0 kernels (ids as gputo_bench_operator) "
                          "and two 256 MiB device copies, then exit")
     args = ap.parse_args()
+    args.horizon = max(args.horizon, args.warmup + args.steps)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -275,8 +427,6 @@ def main():
                 "unit": "GB/s", "traffic": traffic}
     roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
 
-    # the CPU baseline runs the FP64 reference algorithm: price it at the FP64 CG count
-    nit_avg = float(np.mean((base or r)["cg_iters"]))
     out = {
         "metric": METRIC,
         "value": r["s_per_iter"],
@@ -309,8 +459,10 @@ def main():
         },
         "cg_iters": r["cg_iters"],
         "solve_s_per_iter": r["solve_s_per_iter"],
-        "variants": {v: {k: res[v][k] for k in ("s_per_iter", "solve_s_per_iter", "cg_iters", "op_ms",
-                                               "learned_accepted", "learned_rejected", "train_s")}
+        "variants": {v: {k: res[v][k] for k in ("s_per_iter", "s_per_iter_excl_training", "solve_s_per_iter",
+                                               "cg_iters", "op_ms", "learned_accepted", "learned_rejected",
+                                               "train_s", "pretraining", "learned_on_iters", "horizon")
+                         if k in res[v] and res[v][k] is not None}
                      for v in variants},
     }
     if base is not None and head != "f64":
@@ -319,8 +471,14 @@ def main():
         out["u_rel_err_vs_f64"] = float(np.linalg.norm(Uh - Ub) / np.linalg.norm(Ub))
         out["compliance_rel_err_vs_f64"] = float(abs(r["compliance"][-1] - base["compliance"][-1])
                                                  / abs(base["compliance"][-1]))
+        out["speedup_vs_f64_mixed_without_learned"] = {
+            v: base["s_per_iter"] / res[v]["s_per_iter"] for v in variants if v not in ("f64", head)}
+    # error of u and compliance against the FP64 plain solve of the same
+    # design, for every reported result (design iterations 1, 50, 100)
+    out["fp64_error_per_result"] = {v: fp64_errors(gputo, args, Ke, F, fixed, res[v], base)
+                                    for v in variants if v != "f64"}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, Ke, F, fixed, x0, nit_avg)
+        out["cpu_baseline"] = cpu_baseline(gputo, args, Ke, F, fixed, x0, base)
     print(json.dumps(out), flush=True)
 
 

@@ -52,7 +52,13 @@ struct LearnState {
     // logged level-2 defects (training samples) of the current solve
     int nsamp_max = 0, nsamp = 0, vc_count = 0;
     float *samples = nullptr;
+    bool logging = true;         // log the defects of the first nsamp_max V-cycles of a solve
     bool trained = false;
+    // whether the current solve applies the stage, and the solve-level
+    // fallback of params.learned = 2 (LearnPolicy in learn.hip)
+    bool use = true;
+    int pol_state = 0, pol_off_left = 0, pol_backoff = 0, pol_on_count = 0;
+    double pol_t_plain = 0.0;
     long long bytes = 0;
     std::vector<void *> allocs;
 };
@@ -60,6 +66,7 @@ struct LearnState {
 void learn_create(gputo_handle *h);
 void learn_destroy(gputo_handle *h);
 void learn_init_params(gputo_handle *h, unsigned seed);
+void learn_policy_reset(LearnState *L);  // the fallback policy starts over (REF)
 // inputs of the correction stage's forward pass: the feature kernel's
 // partials of sum d^2/ad (norm_n of them, over norm_count DOFs), the
 // diagonal and fixed-DOF mask of the level, and the correction's output
@@ -77,8 +84,9 @@ void learn_backward(gputo_handle *h);                  // G += d<gy, y>/dtheta
 // d = F - A U;  e = N(d);  gamma = <d,e>/<e,Ae>;  U += gamma e if gamma > 0.
 // Logs d as a training sample when logging is on.
 void learn_stage(gputo_handle *h, const HierView<float> &H, int l, const float *F, float *U, const int *ctl);
-void learn_begin_solve(gputo_handle *h);                   // reset counters and the sample log
-void learn_end_solve(gputo_handle *h, int *accepted, int *rejected);
+void learn_begin_solve(gputo_handle *h);                   // reset counters and the sample log; policy
+// after a solve of `seconds` (wall): counters, logged samples, policy update
+void learn_end_solve(gputo_handle *h, int *accepted, int *rejected, double seconds = -1.0);
 int learn_train(gputo_handle *h, int steps, double lr, double *loss_first, double *loss_last);
 
 }  // namespace gputo

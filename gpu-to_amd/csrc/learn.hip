@@ -837,6 +837,7 @@ void learn_create(gputo_handle *h)
     check_hip(hipMemset(L->count, 0, 4 * sizeof(int)), "memset");
     check_hip(hipMemset(L->sc, 0, 8 * sizeof(double)), "memset");
     learn_init_params(h, 1234u);
+    learn_policy_reset(L);
     h->bytes += L->bytes;
 }
 
@@ -947,15 +948,16 @@ void learn_stage(gputo_handle *h, const HierView<float> &H, int l, const float *
     hipStream_t s = h->stream;
     const Grid &g = h->L[l].g;
     const int slot = L->vc_count++;
-    const bool logged = slot < L->nsamp_max;
-    if (!L->trained && !logged) return;
+    const bool logged = L->logging && slot < L->nsamp_max;
+    const bool apply = L->trained && L->use;
+    if (!apply && !logged) return;
     float *d = logged ? L->samples + (long)slot * g.ndof : L->d;
     OpArgsT<float> a = view_args(h, H, l, ctl);
     a.U = U;
     a.F = F;
     a.out = d;
     level_op(h, H, l, OP_DEFECT, a);
-    if (!L->trained) {
+    if (!apply) {
         k_log_count<<<1, 64, 0, s>>>(L->count, slot + 1, ctl);
         return;
     }
@@ -969,14 +971,42 @@ void learn_stage(gputo_handle *h, const HierView<float> &H, int l, const float *
                                            ctl);
 }
 
+// Solve-level fallback of params.learned = 2.  The stage is worth its cost
+// only if it shortens the whole solve, which a per-V-cycle test cannot
+// see, so successive solves of a design loop are compared by wall time:
+//   REF    a solve without the stage; its time is the reference -> TRIAL
+//   TRIAL  a solve with the stage: faster than the reference -> ON, else OFF
+//   ON     solves with the stage; after kPolRecheck of them -> REF
+//   OFF    solves without the stage (each refreshing the reference) for a
+//          backoff that doubles after every failed trial (4 .. 64) -> TRIAL
+// Neighbouring designs of a loop differ a little, so a trial compares two
+// slightly different problems; a harder next design counts against the
+// stage (the bias is towards the plain V-cycle).  A new training (or new
+// parameters) restarts at REF.
+enum LearnPolicy { POL_REF = 0, POL_TRIAL = 1, POL_ON = 2, POL_OFF = 3 };
+constexpr int kPolRecheck = 8, kPolBackoff0 = 4, kPolBackoffMax = 64;
+
+void learn_policy_reset(LearnState *L)
+{
+    L->pol_state = POL_REF;
+    L->pol_off_left = 0;
+    L->pol_backoff = kPolBackoff0;
+    L->pol_on_count = 0;
+    L->pol_t_plain = 0.0;
+}
+
 void learn_begin_solve(gputo_handle *h)
 {
     LearnState *L = h->ln;
     check_hip(hipMemsetAsync(L->count, 0, 4 * sizeof(int), h->stream), "memset");
     L->vc_count = 0;
+    if (h->prm.learned == 2 && L->trained)
+        L->use = L->pol_state == POL_TRIAL || L->pol_state == POL_ON;
+    else
+        L->use = true;
 }
 
-void learn_end_solve(gputo_handle *h, int *accepted, int *rejected)
+void learn_end_solve(gputo_handle *h, int *accepted, int *rejected, double seconds)
 {
     LearnState *L = h->ln;
     int c[4];
@@ -985,6 +1015,31 @@ void learn_end_solve(gputo_handle *h, int *accepted, int *rejected)
     *accepted = c[0];
     *rejected = c[1];
     L->nsamp = c[2];
+    if (h->prm.learned != 2 || !L->trained || seconds < 0.0) return;
+    switch (L->pol_state) {
+    case POL_REF:
+        L->pol_t_plain = seconds;
+        L->pol_state = POL_TRIAL;
+        break;
+    case POL_TRIAL:
+        if (seconds < L->pol_t_plain) {
+            L->pol_state = POL_ON;
+            L->pol_on_count = 0;
+            L->pol_backoff = kPolBackoff0;
+        } else {
+            L->pol_state = POL_OFF;
+            L->pol_off_left = L->pol_backoff;
+            L->pol_backoff = std::min(2 * L->pol_backoff, kPolBackoffMax);
+        }
+        break;
+    case POL_ON:
+        if (++L->pol_on_count >= kPolRecheck) L->pol_state = POL_REF;
+        break;
+    default:  // POL_OFF
+        L->pol_t_plain = seconds;
+        if (--L->pol_off_left <= 0) L->pol_state = POL_TRIAL;
+        break;
+    }
 }
 
 // Adam steps on the logged defects of the last solve (one sample per step,
@@ -1032,6 +1087,7 @@ int learn_train(gputo_handle *h, int steps, double lr, double *loss_first, doubl
     }
     check_hip(hipGetLastError(), "launch");
     L->trained = true;
+    learn_policy_reset(L);
     if (loss_first) *loss_first = lf;
     if (loss_last) *loss_last = ll;
     return 0;
@@ -1099,10 +1155,18 @@ int gputo_learn_params(gputo_handle *h, int set, double *buf)
         if (set) {
             to_float(h->stream, buf, h->ln->P, h->ln->nparam);
             h->ln->trained = true;
+            learn_policy_reset(h->ln);
         } else {
             to_double(h->stream, h->ln->P, buf, h->ln->nparam);
         }
     });
+}
+
+int gputo_learn_logging(gputo_handle *h, int on)
+{
+    if (int rc = learn_guard(h)) return rc;
+    h->ln->logging = on != 0;
+    return 0;
 }
 
 int gputo_learn_reset(gputo_handle *h, unsigned seed)

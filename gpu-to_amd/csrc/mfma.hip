@@ -1,27 +1,30 @@
 // Level-0 operator of the FP32 V-cycle on the matrix cores.
 //
-// Element form: the shared element matrix Ke (24 x 24) multiplies the 24
-// nodal values of 32 elements at a time with v_mfma_f32_32x32x16_bf16 (Ke
-// rows and columns padded to 32, FP32 accumulation), each result column is
-// scaled by its element's E, and every node then sums the contributions of
-// its 8 elements in a fixed order.  FP32 operands are split into BF16
-// parts x = hi + mid + lo; the flavours differ in the products they keep:
+// Element form: the shared element matrix Ke (24 x 24, rows and columns
+// padded to 32) multiplies the 24 nodal values of 32 elements at a time
+// with v_mfma_f32_32x32x16_bf16 (FP32 accumulation); each result column is
+// scaled by its element's E and added into the sums of the element's 8
+// nodes.  FP32 operands are split into BF16 parts x = hi + mid + lo; the
+// flavours differ in the products they keep:
 //   MF_F32    all nine products (hi/mid/lo x hi/mid/lo)
 //   MF_BF16X3 the six products of order >= 2^-16 (FP32-level accuracy)
 //   MF_BF16   hi*hi + hi*lo + lo*hi on a two-way split
 //   MF_BF16X1 hi*hi only
 //
-// One block owns a TJ x TK column of nodes and streams TI node layers along
-// i.  LDS holds a ring of three halo node planes (each node as BF16 parts,
-// 4 per node with a zero pad, so a pair of j-neighbours is one 16-byte MFMA
-// operand fragment) and a ring of two element layers of results (32 floats
-// per element: 8 nodes x (3 components + pad)).  Per node layer: the waves
-// multiply the next element layer, a barrier, each thread sums the 8 element
-// results of one node and applies the operator epilogue, the next node plane
-// is staged, a barrier.  No atomics, no colouring, one fixed summation
-// order: bitwise deterministic.
+// Work of one wave: a strip of 31 nodes along j (element columns j0-1 ..
+// j0+30 on the 32 MFMA columns), KR node rows along k and TI node layers
+// along i, streamed layer by layer.  Lane (h, c), h = lane / 32, holds for
+// element column c the results of the element's four nodes with y offset
+// h: the (x, z) offsets.  The z offsets of consecutive element rows and the
+// x offsets of consecutive element layers meet in the lane's registers; the
+// two y halves of a node meet by one lane permutation per value.  The
+// operands come straight from global memory (a node is read by two lanes
+// per layer, the repeats hit L1/L2), so the kernel has no LDS staging and
+// no barriers, and the waves of a block are independent.  No atomics; every
+// node's sum has one fixed order, so results are bitwise deterministic.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <stdexcept>
 #include <vector>
 
@@ -35,51 +38,28 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 enum MfFlavor { MF_F32 = 0, MF_BF16 = 1, MF_BF16X3 = 2, MF_BF16X1 = 3 };
 
-struct MfmaState {
-    Tile t;  // node brick of the MFMA kernel on level 0
+constexpr int kMfStrip = 31;  // nodes per wave along j (32 element columns)
+constexpr int kMfWaves = 4;   // independent waves per block
+
+// Work decomposition: strips of kMfStrip nodes along j, KR node rows along
+// k, TI node layers along i; one unit per wave, strips fastest.
+struct MfWork {
+    int KR, TI;
+    int ns, nkb, nib;
+    __host__ __device__ long units() const { return (long)ns * nkb * nib; }
+    __host__ __device__ long blocks() const { return (units() + kMfWaves - 1) / kMfWaves; }
 };
 
-// q / d for 0 <= q < 2^22 with inv = 1/d (float quotient corrected both ways)
-__device__ __forceinline__ int mf_udiv(int q, int d, float inv)
-{
-    int r = (int)((float)q * inv);
-    r -= (r * d > q) ? 1 : 0;
-    r += ((r + 1) * d <= q) ? 1 : 0;
-    return r;
-}
+struct MfmaState {
+    MfWork w;
+};
 
 // local node number of offset (x, y, z) = (i, j, k) in {0,1}^3 in the element
 // order ie={0,1,1,0,0,1,1,0}, je={0,0,1,1,0,0,1,1}, ke={0,0,0,0,1,1,1,1}
 __host__ __device__ constexpr int mf_lidx(int x, int y, int z) { return 4 * z + (y ? (x ? 2 : 3) : (x ? 1 : 0)); }
 
-// slot of local node ln in element q's 8 result slots (16 bytes each, 128
-// bytes per element): permuted by q so that 8 consecutive elements touch
-// the 8 different 16-byte bank groups of 128 bytes that a 16-byte LDS access
-// of 8 lanes is served from (the counters show a third fewer bank-conflict
-// cycles than with groups of 256 bytes assumed)
-__device__ __forceinline__ int mf_swz(int q, int ln) { return ln ^ (q & 7); }
-
 // number of BF16 parts per operand for a flavour
 __host__ __device__ constexpr int mf_parts(int fl) { return fl == MF_BF16X1 ? 1 : (fl == MF_BF16 ? 2 : 3); }
-
-struct MfGeom {
-    int HJ, HK, NPL;  // halo plane (TK+2) x (TJ+2) nodes
-    int EJ, EK, NEL;  // element layer (TK+1) x (TJ+1)
-};
-
-__device__ __forceinline__ MfGeom mf_geom(const Tile &t)
-{
-    MfGeom m;
-    m.HJ = t.TJ + 2;
-    m.HK = t.TK + 2;
-    m.NPL = m.HJ * m.HK;
-    m.EJ = t.TJ + 1;
-    m.EK = t.TK + 1;
-    m.NEL = m.EJ * m.EK;
-    return m;
-}
-
-__device__ __forceinline__ uint32_t bf16_bits(__bf16 v) { return (uint32_t) * reinterpret_cast<unsigned short *>(&v); }
 
 // the NP BF16 parts of v: part 0 = bf16(v), part 1 = bf16(v - part0), ...
 template <int NP>
@@ -88,438 +68,504 @@ __device__ __forceinline__ void mf_split(float v, float out[NP])
     float r = v;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        const __bf16 h = (__bf16)r;
-        out[p] = (float)h;
+        const __bf16 hb = (__bf16)r;
+        out[p] = (float)hb;
         r -= out[p];
     }
 }
 
-// Stage node plane gi (halo coordinates: j from j0-1, k from k0-1) as BF16
-// parts: slot layout [part][node] of uint2 = (c0 | c1 << 16, c2 | 0 << 16).
-// Values are first loaded into registers (vals) so the load can be issued
-// early and the LDS write done later.
-template <int MODE>
-__device__ __forceinline__ void mf_plane_load(const OpArgsT<float> &a, const MfGeom &m, int gi, int j0, int k0,
-                                             float vals[2][3])
-{
-    const Grid &g = a.g;
-#pragma unroll
-    for (int rr = 0; rr < 2; ++rr) {
-        const int node = threadIdx.x + 256 * rr;
-        vals[rr][0] = vals[rr][1] = vals[rr][2] = 0.0f;
-        if (node >= m.NPL) continue;
-        const int hk = node / m.HJ, hj = node - hk * m.HJ;
-        const int gj = j0 - 1 + hj, gk = k0 - 1 + hk;
-        if (gi < 0 || gi >= g.nnx || gj < 0 || gj >= g.nny || gk < 0 || gk >= g.nnz) continue;
-        const long dof = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + gj);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
-                vals[rr][c] = (float)a.prerelax * a.F[dof + c] / a.AD[dof + c];
-            else
-                vals[rr][c] = a.U[dof + c];
-        }
-    }
-}
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
+// (a, b) rounded to BF16 and packed, a in the low half
+__device__ __forceinline__ unsigned mf_pk(float a, float b)
+{
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+__device__ __forceinline__ float mf_lo16(unsigned w) { return __builtin_bit_cast(float, w << 16); }
+__device__ __forceinline__ float mf_hi16(unsigned w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
+
+// B operand fragments of one node row: the 6 values of the lane's node pair
+// (y = 0: v[0..2], y = 1: v[3..5]) as NP BF16 parts, slots (3 components,
+// zero) per node; part p + 1 splits what parts 0..p leave.
 template <int NP>
-__device__ __forceinline__ void mf_plane_store(uint2 *su_slot, const MfGeom &m, const float vals[2][3])
+__device__ __forceinline__ void mf_frag(const float (&v)[6], bf16x8 (&f)[NP])
 {
+    float r0 = v[0], r1 = v[1], r2 = v[2], r3 = v[3], r4 = v[4], r5 = v[5];
 #pragma unroll
-    for (int rr = 0; rr < 2; ++rr) {
-        const int node = threadIdx.x + 256 * rr;
-        if (node >= m.NPL) continue;
-        float p0[NP], p1[NP], p2[NP];
-        mf_split<NP>(vals[rr][0], p0);
-        mf_split<NP>(vals[rr][1], p1);
-        mf_split<NP>(vals[rr][2], p2);
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            uint2 w;
-            w.x = bf16_bits((__bf16)p0[p]) | (bf16_bits((__bf16)p1[p]) << 16);
-            w.y = bf16_bits((__bf16)p2[p]);
-            su_slot[p * m.NPL + node] = w;
+    for (int p = 0; p < NP; ++p) {
+        const unsigned w0 = mf_pk(r0, r1), w1 = mf_pk(r2, 0.0f), w2 = mf_pk(r3, r4), w3 = mf_pk(r5, 0.0f);
+        const u32x4 w = {w0, w1, w2, w3};
+        f[p] = __builtin_bit_cast(bf16x8, w);
+        if (p + 1 < NP) {
+            r0 -= mf_lo16(w0);
+            r1 -= mf_hi16(w0);
+            r2 -= mf_lo16(w1);
+            r3 -= mf_lo16(w2);
+            r4 -= mf_hi16(w2);
+            r5 -= mf_lo16(w3);
         }
     }
 }
 
-constexpr int kMfBatchesPerWave = 3;  // element layers of at most 12 batches of 32
+// Buffer resources: loads past the end (and, with the offset taken as
+// unsigned, before the start) return 0, stores there are dropped.
+typedef __amdgpu_buffer_rsrc_t MfBuf;
+__device__ __forceinline__ MfBuf mf_buf(const void *p, long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float mf_ld(MfBuf b, int off, int soff = 0)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b, off, soff, 0));
+}
+// Three consecutive floats as three dword loads.  (The vector-valued load
+// builtins are read through a bit cast of the whole vector: a bit cast of
+// one component of their result came out wrong in this toolchain.)
+__device__ __forceinline__ void mf_ld3(MfBuf b, int off, float *v)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[i] = mf_ld(b, off, 4 * i);
+}
+// Six consecutive floats: one 16-byte and one 8-byte load.
+__device__ __forceinline__ void mf_ld6(MfBuf b, int off, float (&v)[6])
+{
+    const f32x4 x = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(b, off, 0, 0));
+    const f32x2 y = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(b, off + 16, 0, 0));
+    v[0] = x[0];
+    v[1] = x[1];
+    v[2] = x[2];
+    v[3] = x[3];
+    v[4] = y[0];
+    v[5] = y[1];
+}
+constexpr int kMfNoStore = 0x7ffffff0;  // a store offset past every buffer: dropped
 
-// Element coordinates of the batches a lane works on (batch b = wave + 4 kb,
-// element q = 32 b + lane % 32), the same in every element layer: first-node
-// position in the halo plane (np), result slot (qd: NEL for the padding lanes
-// of the last batch) and the element's (k, j) offset in the mesh (eo, -1 when
-// outside it or a padding lane); np = -1 for batches past the last.
-struct MfBatch {
-    int np[kMfBatchesPerWave], qd[kMfBatchesPerWave], eo[kMfBatchesPerWave];
+// Buffers of one launch.
+struct MfBufs {
+    MfBuf U, F, AD, CX, M, O, O2;
 };
 
-__device__ __forceinline__ void mf_batches(const Grid &g, const MfGeom &m, int j0, int k0, MfBatch &B)
-{
-    const int n = threadIdx.x & 31, wave = threadIdx.x >> 6;
-    const int nb = (m.NEL + 31) / 32;
-    const float inv_ej = 1.0f / (float)m.EJ;
-#pragma unroll
-    for (int kb = 0; kb < kMfBatchesPerWave; ++kb) {
-        const int b = wave + 4 * kb;
-        const int q = 32 * b + n;
-        const bool valid = q < m.NEL;
-        const int qq = valid ? q : 0;
-        const int ek = mf_udiv(qq, m.EJ, inv_ej), ej = qq - ek * m.EJ;
-        const int gj = j0 - 1 + ej, gk = k0 - 1 + ek;
-        B.np[kb] = b < nb ? ek * m.HJ + ej : -1;
-        B.qd[kb] = valid ? q : m.NEL;
-        B.eo[kb] = (valid && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz) ? g.ny * gk + gj : -1;
-    }
-}
-
-// Element scales of element layer ge_i for the lane's batches (zero outside
-// the mesh and for padding lanes).
-__device__ __forceinline__ void mf_load_E(const OpArgsT<float> &a, int ge_i, const MfBatch &B,
-                                          float (&E)[kMfBatchesPerWave])
+// The node pair (je, je+1) of node row kr in node plane pi as a lane's 6
+// operand values: U, or prerelax*F/AD in the prerelaxed modes (preRelc,
+// mgcg1.cu:1190-1197).  Pairs outside the mesh only feed elements outside
+// it, whose scale is 0: the loads need no bounds test (they read other
+// nodes' values, or 0 past the ends of the vector).
+template <int MODE>
+__device__ __forceinline__ void mf_row(const OpArgsT<float> &a, const MfBufs &B, int pi, int kr, int je, float (&v)[6])
 {
     const Grid &g = a.g;
-    const bool in = ge_i >= 0 && ge_i < g.nx;
+    const int off = 12 * (g.nny * g.nnz * pi + g.nny * kr + je);
+    if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT) {
+        float f[6], ad[6];
 #pragma unroll
-    for (int kb = 0; kb < kMfBatchesPerWave; ++kb)
-        E[kb] = (in && B.eo[kb] >= 0) ? a.CX[(long)g.ny * g.nz * ge_i + B.eo[kb]] : 0.0f;
-}
-
-// Multiply one element layer: element (ej, ek) of the layer has its first
-// node at halo plane position ek*HJ + ej of planes sx0 (x = 0) and sx1
-// (x = 1).  Results E_e * (Ke u_e) go to T[q] (q = ek*EJ + ej), rows in
-// local-node order with a pad: T[q][slot(ln)][0..3].
-template <int FL>
-__device__ __forceinline__ void mf_layer(const MfGeom &m, const uint2 *__restrict__ sx0,
-                                         const uint2 *__restrict__ sx1, float4 *__restrict__ T,
-                                         const bf16x8 (&Ka)[3][2], const MfBatch &B,
-                                         const float (&E)[kMfBatchesPerWave])
-{
-    constexpr int NP = mf_parts(FL);
-    const int lane = threadIdx.x & 63;
-    const int h = lane >> 5;
-#pragma unroll
-    for (int kb = 0; kb < kMfBatchesPerWave; ++kb) {
-        const int np = B.np[kb];
-        if (np < 0) break;  // wave-uniform
-        // operand fragments: K step s covers node-pair groups 2s and 2s+1,
-        // lane half h takes group 2s + h = (x, z) = (h, s)
-        bf16x8 Bf[NP][2];
-#pragma unroll
-        for (int st = 0; st < 2; ++st) {
-            const uint2 *src = (h ? sx1 : sx0) + np + st * m.HJ;
-#pragma unroll
-            for (int p = 0; p < NP; ++p) {
-                const uint2 w0 = src[p * m.NPL], w1 = src[p * m.NPL + 1];
-                union {
-                    uint32_t u[4];
-                    bf16x8 v;
-                } cv;
-                cv.u[0] = w0.x;
-                cv.u[1] = w0.y;
-                cv.u[2] = w1.x;
-                cv.u[3] = w1.y;
-                Bf[p][st] = cv.v;
-            }
-        }
-        // one accumulation chain, smallest products first
-        f32x16 d = {};
-#pragma unroll
-        for (int st = 0; st < 2; ++st) {
-            if (FL == MF_F32) {
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[2][st], Bf[2][st], d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[2][st], Bf[1][st], d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[1][st], Bf[2][st], d, 0, 0, 0);
-            }
-            if (FL == MF_F32 || FL == MF_BF16X3) {
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[2][st], Bf[0][st], d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[1][st], Bf[1][st], d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[0][st], Bf[2][st], d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[1][st], Bf[0][st], d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[0][st], Bf[1][st], d, 0, 0, 0);
-            }
-            if (FL == MF_BF16) {
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[1][st], Bf[0][st], d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[0][st], Bf[1][st], d, 0, 0, 0);
-            }
+        for (int i = 0; i < 6; ++i) {
+            f[i] = mf_ld(B.F, off, 4 * i);
+            ad[i] = mf_ld(B.AD, off, 4 * i);
         }
 #pragma unroll
-        for (int st = 0; st < 2; ++st) d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[0][st], Bf[0][st], d, 0, 0, 0);
-        // rows 8j + 4h + r: node pair group j = (x, z) = (j & 1, j >> 1), the
-        // pair member y = h, component r (r = 3: pad row, zero)
-        const float e = E[kb];
-        const int qd = B.qd[kb];
-        float4 *dst = T + (long)qd * 8;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int ln = h ? mf_lidx(j & 1, 1, j >> 1) : mf_lidx(j & 1, 0, j >> 1);
-            dst[mf_swz(qd, ln)] = make_float4(e * d[4 * j], e * d[4 * j + 1], e * d[4 * j + 2], e * d[4 * j + 3]);
-        }
+        for (int i = 0; i < 6; ++i) v[i] = ad[i] != 0.0f ? (float)a.prerelax * f[i] / ad[i] : 0.0f;
+    } else {
+        mf_ld6(B.U, off, v);
     }
 }
 
-// Operands of the epilogue of one owned node (loaded a layer ahead).
-struct MfNodeIn {
+// scale * E of element (ie, je, ke), zero outside the mesh (jin: je inside)
+__device__ __forceinline__ float mf_E(const OpArgsT<float> &a, const MfBufs &B, int ie, int ke, int je, bool jin,
+                                      float scale)
+{
+    const Grid &g = a.g;
+    const bool in = jin && ie >= 0 && ie < g.nx && ke >= 0 && ke < g.nz;
+    const float v = mf_ld(B.CX, 4 * (g.ny * g.nz * ie + g.ny * ke + je));
+    return in ? v * scale : 0.0f;
+}
+
+// Operands of the epilogue of one output node.
+struct MfOut {
     float u[3], f[3], ad[3];
     unsigned msk;
-    bool ok;
+    int off;  // byte offset of the node's first DOF, or kMfNoStore
 };
 
+// (ni is inside the mesh; `want` says whether the lane owns column nj)
 template <int MODE>
-__device__ __forceinline__ void mf_load_node(const OpArgsT<float> &a, const Tile &t, int gi, int j0, int k0,
-                                             MfNodeIn &in)
+__device__ __forceinline__ void mf_out_load(const OpArgsT<float> &a, const MfBufs &B, int ni, int nk, int nj,
+                                            bool want, MfOut &o)
 {
     const Grid &g = a.g;
-    const int own = threadIdx.x;
-    in.ok = false;
-    if (own >= t.TJ * t.TK || gi >= g.nnx) return;
-    const int tk = own / t.TJ, tj = own - tk * t.TJ;
-    const int gj = j0 + tj, gk = k0 + tk;
-    if (gj >= g.nny || gk >= g.nnz) return;
-    in.ok = true;
-    const long nod = (long)g.nny * g.nnz * gi + (long)g.nny * gk + gj;
-    const long dof = 3 * nod;
-    in.msk = a.mask[nod];
+    const bool ok = want && nk < g.nnz && nj < g.nny;
+    const int node = g.nny * g.nnz * ni + g.nny * nk + nj;
+    const int off = 12 * node;
+    o.off = ok ? off : kMfNoStore;
+    o.msk = __builtin_amdgcn_raw_buffer_load_b8(B.M, node, 0, 0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o.u[c] = o.f[c] = o.ad[c] = 0.0f;
+    if (MODE != OP_MATVEC) mf_ld3(B.F, off, o.f);
+    if (MODE == OP_JACOBI || MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT) mf_ld3(B.AD, off, o.ad);
+    if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o.u[c] = o.ad[c] != 0.0f ? (float)a.prerelax * o.f[c] / o.ad[c] : 0.0f;
+    } else if (MODE == OP_JACOBI) {
+        mf_ld3(B.U, off, o.u);
+    }
+}
+
+// The operator epilogue of one node from y = (K u) at the node
+// (multAV0c / defect0c / Jacobc, mgcg1.cu:941-997,1199-1264; ZeroBC fused).
+// Lanes that own no node store past the end of the buffer (dropped).
+template <int MODE>
+__device__ __forceinline__ void mf_out_store(const OpArgsT<float> &a, const MfBufs &B, const MfOut &o,
+                                             const float (&y)[3], float om)
+{
+    float r[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        if (MODE != OP_MATVEC) in.f[c] = a.F[dof + c];
-        if (MODE == OP_JACOBI || MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT) in.ad[c] = a.AD[dof + c];
-        if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
-            in.u[c] = (float)a.prerelax * in.f[c] / in.ad[c];
-        else if (MODE != OP_DEFECT)
-            in.u[c] = a.U[dof + c];
+        const bool fixed = (o.msk >> c) & 1u;
+        if (MODE == OP_MATVEC) {
+            r[c] = fixed ? 0.0f : y[c];
+        } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
+            r[c] = fixed ? 0.0f : o.f[c] - y[c];
+        } else {
+            const float f = o.f[c], u = o.u[c], ad = o.ad[c];
+            const float rv = fixed ? 0.0f : f - y[c];
+            // U = (1-w) U + w (R + AD U)/AD  (mgcg1.cu:1199-1206)
+            r[c] = (1.0f - om) * u + om * (rv + ad * u) * __builtin_amdgcn_rcpf(ad);
+        }
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, r[c]), B.O, o.off, 4 * c, 0);
+        if (MODE == OP_PRE_DEFECT)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, o.u[c]), B.O2, o.off, 4 * c, 0);
     }
 }
 
-template <int FL, int MODE>
-__global__ __launch_bounds__(256, 3) void k_mfma_op(OpArgsT<float> a, Tile t)
-{
-    if (a.ctl && a.ctl[0]) return;
-    constexpr int NP = mf_parts(FL);
-    extern __shared__ __align__(16) unsigned char smem[];
-    const Grid &g = a.g;
-    const long tb = xcd_tile(blockIdx.x, t.blocks());
-    if (tb >= t.blocks()) {  // padding block of the XCD order
-        if (a.partials && threadIdx.x == 0) a.partials[blockIdx.x] = 0.0;
-        return;
-    }
-    int b = (int)tb;
-    const int bj = b % t.ntj;
-    b /= t.ntj;
-    const int bk = b % t.ntk;
-    const int bi = b / t.ntk;
-    const int j0 = bj * t.TJ, k0 = bk * t.TK, i0 = bi * t.TI;
-    const MfGeom m = mf_geom(t);
-    float4 *T = reinterpret_cast<float4 *>(smem);                     // [2][NEL+1][8]
-    uint2 *su = reinterpret_cast<uint2 *>(T + 2 * (m.NEL + 1) * 8);  // [3][NP][NPL]
-    const int lane = threadIdx.x & 63;
-    const int n = lane & 31, h = lane >> 5;
+// A operand parts: the BF16 split of Ke, or with KX (Ke = S Kint, Kint
+// integer and exact in BF16) Kint alone.
+template <int FL, bool KX>
+struct MfA {
+    static constexpr int n = KX ? 1 : mf_parts(FL);
+};
 
-    // A fragments: lane row r = n (output row 8j + 4hm + cm), K step s,
-    // columns 16s + 8h + 4yy + c -> Ke column 3*lidx(h, yy, s) + c
-    bf16x8 Ka[3][2];
-    {
-        const int jm = n >> 3, hm = (n >> 2) & 1, cm = n & 3;
-        const int lnr = mf_lidx(jm & 1, hm, jm >> 1);
+// D = Ke u_e (KX: Kint u_e) for the 32 elements of one element row: K step
+// 0 is the z = 0 node row (fragments f0), step 1 the z = 1 row (f1); one
+// accumulation chain, smallest products first.
+template <int FL, bool KX>
+__device__ __forceinline__ f32x16 mf_chain(const bf16x8 (&Ka)[MfA<FL, KX>::n][2], const bf16x8 (&f0)[mf_parts(FL)],
+                                           const bf16x8 (&f1)[mf_parts(FL)])
+{
+    constexpr int NP = mf_parts(FL);
+    f32x16 d = {};
+    if constexpr (KX) {
+        // exact A: every product of the split of u, lo parts first
+#pragma unroll
+        for (int p = NP - 1; p >= 0; --p)
+#pragma unroll
+            for (int st = 0; st < 2; ++st)
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[0][st], st ? f1[p] : f0[p], d, 0, 0, 0);
+        return d;
+    } else {
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+            const bf16x8(&B)[NP] = st ? f1 : f0;
+            if constexpr (FL == MF_F32) {
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[2][st], B[2], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[2][st], B[1], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[1][st], B[2], d, 0, 0, 0);
+            }
+            if constexpr (FL == MF_F32 || FL == MF_BF16X3) {
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[2][st], B[0], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[1][st], B[1], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[0][st], B[2], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[1][st], B[0], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[0][st], B[1], d, 0, 0, 0);
+            }
+            if constexpr (FL == MF_BF16) {
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[1][st], B[0], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[0][st], B[1], d, 0, 0, 0);
+            }
+        }
 #pragma unroll
         for (int st = 0; st < 2; ++st)
+            d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ka[0][st], st ? f1[0] : f0[0], d, 0, 0, 0);
+        return d;
+    }
+}
+
+// Operands of one element layer: the lane's node rows k0-1 .. k0+KR in node
+// plane layer + h, and the (scaled) element scales of element rows
+// k0-1 .. k0+KR-1.
+template <int KR>
+struct MfLayer {
+    float u[KR + 2][6];
+    float e[KR + 1];
+};
+
+template <int MODE, int KR>
+__device__ __forceinline__ void mf_load_layer(const OpArgsT<float> &a, const MfBufs &B, int ie, int k0, int je,
+                                              bool jin, int h, float scale, MfLayer<KR> &L)
+{
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int yy = i >> 2, c = i & 3;
-                const int lnc = mf_lidx(h, yy, st);
-                const float v = (cm < 3 && c < 3) ? a.Ke[24 * (3 * lnr + cm) + 3 * lnc + c] : 0.0f;
-                float pv[3] = {0.0f, 0.0f, 0.0f};
-                mf_split<NP>(v, pv);
+    for (int t = 0; t < KR + 2; ++t) mf_row<MODE>(a, B, ie + h, k0 - 1 + t, je, L.u[t]);
 #pragma unroll
-                for (int p = 0; p < 3; ++p) Ka[p][st][i] = (__bf16)pv[p];
+    for (int q = 0; q <= KR; ++q) L.e[q] = mf_E(a, B, ie, k0 - 1 + q, je, jin, scale);
+}
+
+// Element layer ie: products of its KR+1 element rows added into the node
+// sums of node layer ie (cur: x = 0 offsets; it already holds the x = 1
+// offsets of layer ie-1) and started for node layer ie+1 (nxt: x = 1
+// offsets; each row's first product is assigned, so nxt needs no clearing).
+// Accumulator 4j + r holds node group j = (x, z) = (j & 1, j >> 1),
+// component r (r = 3 is the zero pad row).
+template <int FL, int KR, bool KX>
+__device__ __forceinline__ void mf_products(const bf16x8 (&Ka)[MfA<FL, KX>::n][2], const MfLayer<KR> &L,
+                                            float (&cur)[KR][3], float (&nxt)[KR][3])
+{
+    constexpr int NP = mf_parts(FL);
+    bf16x8 f0[NP], f1[NP];
+    mf_frag<NP>(L.u[0], f0);
+#pragma unroll
+    for (int q = 0; q <= KR; ++q) {
+        mf_frag<NP>(L.u[q + 1], f1);
+        const f32x16 d = mf_chain<FL, KX>(Ka, f0, f1);
+        const float e = L.e[q];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (q >= 1) {  // z = 0 offsets: node row k0-1+q
+                cur[q - 1][c] += e * d[c];
+                nxt[q - 1][c] += e * d[4 + c];
             }
-    }
-    const int nlay = min(t.TI, g.nnx - i0);
-    MfBatch B;
-    mf_batches(g, m, j0, k0, B);
-
-    // prologue: planes i0-1, i0, i0+1 into slots 0, 1, 2; element layer i0-1.
-    // Software pipeline of the layer loop: plane q = tt + 3 is loaded two
-    // layers ahead (at layer tt - 1) and stored at the end of layer tt; the
-    // epilogue operands of node layer tt and the element scales of element
-    // layer tt are loaded one layer ahead.
-    float En[kMfBatchesPerWave];
-    float pv[2][3];  // plane loaded one layer earlier, stored at the end of this layer
-    MfNodeIn in;
-    {
-        float E0[kMfBatchesPerWave];
-        mf_load_E(a, i0 - 1, B, E0);
-        float vals[2][3];
-#pragma unroll 1
-        for (int q = 0; q < 3; ++q) {
-            mf_plane_load<MODE>(a, m, i0 - 1 + q, j0, k0, vals);
-            mf_plane_store<NP>(su + q * NP * m.NPL, m, vals);
+            if (q < KR) {  // z = 1 offsets: node row k0+q
+                cur[q][c] += e * d[8 + c];
+                nxt[q][c] = e * d[12 + c];
+            }
         }
-        mf_load_E(a, i0, B, En);
-        mf_plane_load<MODE>(a, m, i0 + 2, j0, k0, pv);
-        mf_load_node<MODE>(a, t, i0, j0, k0, in);
-        __syncthreads();
-        mf_layer<FL>(m, su, su + NP * m.NPL, T, Ka, B, E0);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) f0[p] = f1[p];
     }
+}
 
-    double part = 0.0;
-    const float om = (float)a.omega;
-    for (int tt = 0; tt < nlay; ++tt) {
-        const int gi = i0 + tt;
-        // loads for the next layer first
-        float pn[2][3];
-        mf_plane_load<MODE>(a, m, gi + 3, j0, k0, pn);
-        MfNodeIn inn;
-        mf_load_node<MODE>(a, t, gi + 1, j0, k0, inn);
-        float Enn[kMfBatchesPerWave];
-        mf_load_E(a, gi + 1, B, Enn);
-        mf_layer<FL>(m, su + ((tt + 1) % 3) * NP * m.NPL, su + ((tt + 2) % 3) * NP * m.NPL,
-                     T + ((tt + 1) & 1) * (m.NEL + 1) * 8, Ka, B, En);
-        __syncthreads();
-        // node layer gi: element layers gi-1 (node at x = 1) and gi (x = 0)
-        if (in.ok) {
-            const int own = threadIdx.x;
-            const int tk = own / t.TJ, tj = own - tk * t.TJ;
-            float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+template <int FL, int MODE, int KR, bool KX>
+__global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, MfWork w, float kscale)
+{
+    if (a.ctl && a.ctl[0]) return;
+    constexpr int NA = MfA<FL, KX>::n;
+    constexpr int KH = KR / 2;  // output rows per lane half
+    const Grid &g = a.g;
+    const long nb = w.blocks();
+    const long tb = xcd_tile(blockIdx.x, nb);
+    const int lane = threadIdx.x & 63, c = lane & 31, h = lane >> 5;
+    const long unit = tb * kMfWaves + (threadIdx.x >> 6);
+    if (tb < nb && unit < w.units()) {  // wave-uniform
+        long un = unit;
+        const int s = (int)(un % w.ns);
+        un /= w.ns;
+        const int kb = (int)(un % w.nkb);
+        const int ib = (int)(un / w.nkb);
+        const int j0 = kMfStrip * s, k0 = KR * kb, i0 = w.TI * ib;
+        const int je = j0 - 1 + c;  // element column of this lane
+        const bool jin = je >= 0 && je < g.ny;
+        const float scale = KX ? kscale : 1.0f;
+        const long vb = 4 * g.ndof;
+        MfBufs B;
+        B.U = mf_buf(a.U, a.U ? vb : 0);
+        B.F = mf_buf(a.F, a.F ? vb : 0);
+        B.AD = mf_buf(a.AD, a.AD ? vb : 0);
+        B.CX = mf_buf(a.CX, 4 * g.nel);
+        B.M = mf_buf(a.mask, g.nnod);
+        B.O = mf_buf(a.out, vb);
+        B.O2 = mf_buf(a.out2, a.out2 ? vb : 0);
+
+        // A fragments: lane row r = c (output row 8j + 4hm + cm), K step st,
+        // columns 16 st + 8 h + 4 yy + cc -> Ke column 3 lidx(h, yy, st) + cc
+        bf16x8 Ka[NA][2];
+        {
+            const int jm = c >> 3, hm = (c >> 2) & 1, cm = c & 3;
+            const int lnr = mf_lidx(jm & 1, hm, jm >> 1);
 #pragma unroll
-            for (int x = 1; x >= 0; --x) {
-                const float4 *Tl = T + ((x ? tt : tt + 1) & 1) * (m.NEL + 1) * 8;
+            for (int st = 0; st < 2; ++st)
 #pragma unroll
-                for (int z = 1; z >= 0; --z)
+                for (int i = 0; i < 8; ++i) {
+                    const int yy = i >> 2, cc = i & 3;
+                    const int lnc = mf_lidx(h, yy, st);
+                    const float v = (cm < 3 && cc < 3) ? a.Ke[24 * (3 * lnr + cm) + 3 * lnc + cc] : 0.0f;
+                    if constexpr (KX) {
+                        Ka[0][st][i] = (__bf16)rintf(v / kscale);  // an integer of at most 8 bits: exact
+                    } else {
+                        float pv[NA];
+                        mf_split<NA>(v, pv);
 #pragma unroll
-                    for (int y = 1; y >= 0; --y) {
-                        // element with first node at halo (tj + 1 - y, tk + 1 - z)
-                        const int q = (tk + 1 - z) * m.EJ + (tj + 1 - y);
-                        const float4 v = Tl[q * 8 + mf_swz(q, mf_lidx(x, y, z))];
-                        y0 += v.x;
-                        y1 += v.y;
-                        y2 += v.z;
+                        for (int p = 0; p < NA; ++p) Ka[p][st][i] = (__bf16)pv[p];
                     }
-            }
-            const long dof = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * (k0 + tk) + (j0 + tj));
-            const float yv[3] = {y0, y1, y2};
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const bool fixed = (in.msk >> c) & 1u;
-                if (MODE == OP_MATVEC) {
-                    const float v = fixed ? 0.0f : yv[c];
-                    a.out[dof + c] = v;
-                    part += (double)in.u[c] * (double)v;
-                } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
-                    const float rv = fixed ? 0.0f : in.f[c] - yv[c];
-                    a.out[dof + c] = rv;
-                    if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = in.u[c];
-                } else {
-                    const float f = in.f[c], u = in.u[c], ad = in.ad[c];
-                    const float rv = fixed ? 0.0f : f - yv[c];
-                    // U = (1-w) U + w (R + AD U)/AD  (mgcg1.cu:1199-1206)
-                    const float un = (1.0f - om) * u + om * (rv + ad * u) / ad;
-                    a.out[dof + c] = un;
-                    part += (double)f * (double)un;
                 }
-            }
         }
-        mf_plane_store<NP>(su + (tt % 3) * NP * m.NPL, m, pv);  // plane gi + 2
+
+        // output node of the lane: the low half owns nodes j0 + c - 1 of node
+        // rows k0 .. k0+KH-1, the high half nodes j0 + c of rows k0+KH ..
+        const int jo = h ? j0 + c : j0 + c - 1;
+        const bool jok = h ? (c < kMfStrip) : (c >= 1);
+        const int src = h ? lane - 31 : lane + 31;  // the lane with the other y half
+        const float om = (float)a.omega;
+        const int nlay = min(w.TI, g.nnx - i0);
+
+        float accA[KR][3], accB[KR][3];
 #pragma unroll
-        for (int kb = 0; kb < kMfBatchesPerWave; ++kb) En[kb] = Enn[kb];
+        for (int r = 0; r < KR; ++r)
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+            for (int cc = 0; cc < 3; ++cc) accA[r][cc] = accB[r][cc] = 0.0f;
+
+        // element layer ie = i0 - 1 + tt; after it node layer ie is complete.
+        // Software pipeline: the operands of element layer ie + 1 are loaded
+        // while layer ie computes (two register sets, swapped every layer);
+        // the epilogue operands of node layer ie are loaded at the start of
+        // layer ie, whose products cover their latency.
+        auto step = [&](int tt, float (&cur)[KR][3], float (&nxt)[KR][3], const MfLayer<KR> &L, MfLayer<KR> &N) {
+            const int ie = i0 - 1 + tt;
+            // issue order matters: loads complete in order for the wait
+            // counter, so the epilogue operands go before the prefetch
+            MfOut O[KH];
+            if (tt >= 1) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) pv[r][c] = pn[r][c];
-        in = inn;
-        __syncthreads();
-    }
-    if (a.partials) {
-        const double sm = block_sum(part);
-        if (threadIdx.x == 0) a.partials[blockIdx.x] = sm;
+                for (int rr = 0; rr < KH; ++rr) mf_out_load<MODE>(a, B, ie, k0 + rr + h * KH, jo, jok, O[rr]);
+            }
+            if (tt < nlay) mf_load_layer<MODE, KR>(a, B, ie + 1, k0, je, jin, h, scale, N);
+            mf_products<FL, KR, KX>(Ka, L, cur, nxt);
+            if (tt >= 1) {
+                float y[KH][3];
+#pragma unroll
+                for (int r = 0; r < KR; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) {
+                        const float full = cur[r][cc] + __shfl(cur[r][cc], src, 64);
+                        if (r < KH) {
+                            if (!h) y[r][cc] = full;
+                        } else {
+                            if (h) y[r - KH][cc] = full;
+                        }
+                    }
+#pragma unroll
+                for (int rr = 0; rr < KH; ++rr) mf_out_store<MODE>(a, B, O[rr], y[rr], om);
+            }
+        };
+        MfLayer<KR> LA, LB;
+        mf_load_layer<MODE, KR>(a, B, i0 - 1, k0, je, jin, h, scale, LA);
+        for (int tt = 0; tt <= nlay; tt += 2) {
+            step(tt, accA, accB, LA, LB);
+            if (tt + 1 <= nlay) step(tt + 1, accB, accA, LB, LA);
+        }
     }
 }
 
-static int mf_balance(int n, int tmax)
+// S with Ke = S Kint, Kint integer with |entries| <= 256 (exact in BF16),
+// from the entry classes of a unit-H8 Ke0; 0 when there is none.  For the
+// top3D125 Ke0 (nu = 0.3), Kint = 1440 (1+nu)(2nu-1) Ke0 has the entries
+// -176 .. 80 (top3D125mod.m:50-76 in integer form).
+static float ke_int_scale(const OpArgsT<float> &a)
 {
-    if (tmax >= n) return n;
-    const int nt = (n + tmax - 1) / tmax;
-    return (n + nt - 1) / nt;
-}
-
-template <int NP>
-static size_t mf_lds_bytes_t(const Tile &t)
-{
-    const size_t NEL = (size_t)(t.TJ + 1) * (t.TK + 1), NPL = (size_t)(t.TJ + 2) * (t.TK + 2);
-    return 2 * (NEL + 1) * 8 * sizeof(float4) + 3 * NP * NPL * sizeof(uint2);
-}
-
-constexpr size_t kMfLdsMax = 52 * 1024;  // three blocks per CU
-
-static Tile mfma_tile(const Grid &g)
-{
-    Tile t;
-    // 20 x 5 node columns: with two BF16 parts the block needs 39.9 KB of
-    // LDS, so four blocks fit a CU (measured at 192x96x96: 55 us against
-    // 62 us for 20 x 6 at three blocks per CU)
-    t.TJ = mf_balance(g.nny, 20);
-    t.TK = mf_balance(g.nnz, 5);
-    t.TI = 8;
-    // LDS, one owned node per thread, at most kMfBatchesPerWave batches per wave
-    while (mf_lds_bytes_t<3>(t) > kMfLdsMax || t.TJ * t.TK > 256 ||
-           ((t.TJ + 1) * (t.TK + 1) + 31) / 32 > 4 * kMfBatchesPerWave) {
-        if (t.TK > 1)
-            t.TK--;
-        else
-            t.TJ--;
+    if (!a.kc) return 0.0f;
+    float vmin = 0.0f;
+    for (int k = 0; k < kKeMaxClasses; ++k)
+        if (a.kv[k] != 0.0f && (vmin == 0.0f || std::fabs(a.kv[k]) < vmin)) vmin = std::fabs(a.kv[k]);
+    if (vmin == 0.0f) return 0.0f;
+    for (int m = 1; m <= 256; ++m) {
+        const double S = (double)vmin / m;
+        bool ok = true;
+        for (int k = 0; k < kKeMaxClasses && ok; ++k) {
+            const double q = a.kv[k] / S, r = std::nearbyint(q);
+            ok = std::fabs(r) <= 256.0 && std::fabs(q - r) <= 1e-4;
+        }
+        if (ok) return (float)S;
     }
-    t.ntj = (g.nny + t.TJ - 1) / t.TJ;
-    t.ntk = (g.nnz + t.TK - 1) / t.TK;
-    t.nti = (g.nnx + t.TI - 1) / t.TI;
-    // a halo plane is staged two nodes per thread
-    if ((t.TJ + 2) * (t.TK + 2) > 512) throw std::runtime_error("MFMA level-0 tile: halo plane too large");
-    return t;
+    return 0.0f;
 }
 
-template <int FL>
-static void launch_flavor(hipStream_t s, int mode, const OpArgsT<float> &a, const Tile &t)
+// Tiling: KR node rows per wave (a template argument) and TI node layers.
+// A wave streams its TI layers one after another, so the grid is sized to
+// be resident at once: TI is the smallest layer count whose units fit the
+// waves the device holds of the kernel (its occupancy times the CUs; one
+// round, no tail).  GPUTO_MF_TILE="KR,TI" overrides it (KR 4 or 6).
+static MfWork mfma_work(const Grid &g)
 {
-    const dim3 grid((unsigned)xcd_launch(t.blocks())), block(256);
-    const size_t lds = mf_lds_bytes_t<mf_parts(FL)>(t);
-    static bool attr = false;
-    if (!attr) {  // dynamic LDS above 64 KB
-        (void)hipFuncSetAttribute((const void *)k_mfma_op<FL, OP_MATVEC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMfLdsMax);
-        (void)hipFuncSetAttribute((const void *)k_mfma_op<FL, OP_DEFECT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMfLdsMax);
-        (void)hipFuncSetAttribute((const void *)k_mfma_op<FL, OP_JACOBI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMfLdsMax);
-        (void)hipFuncSetAttribute((const void *)k_mfma_op<FL, OP_PRE_JACOBI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMfLdsMax);
-        (void)hipFuncSetAttribute((const void *)k_mfma_op<FL, OP_PRE_DEFECT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMfLdsMax);
-        attr = true;
+    MfWork w;
+    w.KR = 4;
+    w.TI = 0;
+    if (const char *e = std::getenv("GPUTO_MF_TILE")) {
+        int kr = 0, ti = 0;
+        if (std::sscanf(e, "%d,%d", &kr, &ti) == 2 && (kr == 4 || kr == 6) && ti >= 1) {
+            w.KR = kr;
+            w.TI = ti;
+        }
     }
+    w.ns = (g.nny + kMfStrip - 1) / kMfStrip;
+    w.nkb = (g.nnz + w.KR - 1) / w.KR;
+    if (w.TI == 0) {
+        int dev = 0, ncu = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mfma_op<MF_BF16, OP_JACOBI, 4, true>,
+                                                         kMfWaves * 64, 0) != hipSuccess || per_cu < 1)
+            per_cu = 2;
+        const long waves = (long)per_cu * ncu * kMfWaves;
+        const long nib = std::max(1L, waves / ((long)w.ns * w.nkb));
+        w.TI = (int)((g.nnx + nib - 1) / nib);
+    }
+    w.nib = (g.nnx + w.TI - 1) / w.TI;
+    return w;
+}
+
+template <int FL, int KR, bool KX>
+static void launch_kr(hipStream_t s, int mode, const OpArgsT<float> &a, const MfWork &w, float S)
+{
+    const dim3 grid((unsigned)xcd_launch(w.blocks())), block(kMfWaves * 64);
     switch (mode) {
-    case OP_MATVEC: k_mfma_op<FL, OP_MATVEC><<<grid, block, lds, s>>>(a, t); break;
-    case OP_DEFECT: k_mfma_op<FL, OP_DEFECT><<<grid, block, lds, s>>>(a, t); break;
-    case OP_JACOBI: k_mfma_op<FL, OP_JACOBI><<<grid, block, lds, s>>>(a, t); break;
-    case OP_PRE_JACOBI: k_mfma_op<FL, OP_PRE_JACOBI><<<grid, block, lds, s>>>(a, t); break;
-    case OP_PRE_DEFECT: k_mfma_op<FL, OP_PRE_DEFECT><<<grid, block, lds, s>>>(a, t); break;
+    case OP_MATVEC: k_mfma_op<FL, OP_MATVEC, KR, KX><<<grid, block, 0, s>>>(a, w, S); break;
+    case OP_DEFECT: k_mfma_op<FL, OP_DEFECT, KR, KX><<<grid, block, 0, s>>>(a, w, S); break;
+    case OP_JACOBI: k_mfma_op<FL, OP_JACOBI, KR, KX><<<grid, block, 0, s>>>(a, w, S); break;
+    case OP_PRE_JACOBI: k_mfma_op<FL, OP_PRE_JACOBI, KR, KX><<<grid, block, 0, s>>>(a, w, S); break;
+    case OP_PRE_DEFECT: k_mfma_op<FL, OP_PRE_DEFECT, KR, KX><<<grid, block, 0, s>>>(a, w, S); break;
     default: break;
     }
 }
 
-void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const void *ctx, int flavor)
+template <int FL, bool KX>
+static void launch_fx(hipStream_t s, int mode, const OpArgsT<float> &a, const MfWork &w, float S)
 {
-    const MixedState *m = static_cast<const MixedState *>(ctx);
-    const Tile &t = m->mf->t;
-    if (flavor == MF_F32)
-        launch_flavor<MF_F32>(s, mode, a, t);
-    else if (flavor == MF_BF16X3)
-        launch_flavor<MF_BF16X3>(s, mode, a, t);
-    else if (flavor == MF_BF16X1)
-        launch_flavor<MF_BF16X1>(s, mode, a, t);
+    if (w.KR == 6)
+        launch_kr<FL, 6, KX>(s, mode, a, w, S);
     else
-        launch_flavor<MF_BF16>(s, mode, a, t);
+        launch_kr<FL, 4, KX>(s, mode, a, w, S);
 }
 
-long mfma_tile_blocks(const Grid &g) { return xcd_launch(mfma_tile(g).blocks()); }
+template <int FL>
+static void launch_flavor(hipStream_t s, int mode, const OpArgsT<float> &a, const MfWork &w)
+{
+    const float S = ke_int_scale(a);
+    if (S > 0.0f)
+        launch_fx<FL, true>(s, mode, a, w, S);
+    else
+        launch_fx<FL, false>(s, mode, a, w, 1.0f);
+}
+
+void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const void *ctx, int flavor)
+{
+    if (a.partials) throw std::runtime_error("MFMA level-0 operator: no fused dot product");
+    const MixedState *m = static_cast<const MixedState *>(ctx);
+    const MfWork &w = m->mf->w;
+    if (flavor == MF_F32)
+        launch_flavor<MF_F32>(s, mode, a, w);
+    else if (flavor == MF_BF16X3)
+        launch_flavor<MF_BF16X3>(s, mode, a, w);
+    else if (flavor == MF_BF16X1)
+        launch_flavor<MF_BF16X1>(s, mode, a, w);
+    else
+        launch_flavor<MF_BF16>(s, mode, a, w);
+}
+
+long mfma_tile_blocks(const Grid &g) { return xcd_launch(mfma_work(g).blocks()); }
 
 void mfma_create(gputo_handle *h, MixedState *m)
 {
     m->mf = new MfmaState();
-    m->mf->t = mfma_tile(h->L[0].g);
+    m->mf->w = mfma_work(h->L[0].g);
 }
 
 void mfma_destroy(gputo_handle *, MixedState *m)
@@ -528,7 +574,7 @@ void mfma_destroy(gputo_handle *, MixedState *m)
     m->mf = nullptr;
 }
 
-int mfma_blocks(const gputo_handle *h) { return (int)xcd_launch(h->mx->mf->t.blocks()); }
+int mfma_blocks(const gputo_handle *h) { return (int)xcd_launch(h->mx->mf->w.blocks()); }
 
 // ---------------------------------------------------------------------------
 // Lane-layout self-checks of the two MFMA instructions the level-0 kernel

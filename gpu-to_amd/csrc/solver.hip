@@ -2,6 +2,7 @@
 // the V-cycle and the device-resident FP64 PCG, and the C-ABI entry points
 // that replace the mgcg1 MEX solve (Cantilever/TOP3D125MGCGCUDA/mgcg1.cu).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -427,6 +428,7 @@ static void precond(gputo_handle *h, const double *r, double *z, bool r32_ready 
 int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit)
 {
     hipStream_t s = h->stream;
+    const auto t_begin = std::chrono::steady_clock::now();
     if (h->prm.precision != GPUTO_PREC_F64) mixed_begin(h);
     if (h->ln) learn_begin_solve(h);
     // a trained learned stage makes the preconditioner vary, and a level-0
@@ -506,7 +508,8 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     if (h->prm.precision != GPUTO_PREC_F64) mixed_end(h, *nit, r.done);
     if (h->ln) {
         int acc = 0, rej = 0;
-        learn_end_solve(h, &acc, &rej);
+        learn_end_solve(h, &acc, &rej,
+                        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count());
         if (h->mx) {
             h->mx->last_learned_ok = acc;
             h->mx->last_learned_bad = rej;
@@ -701,6 +704,7 @@ int gputo_set_params(gputo_handle *h, const gputo_params *params)
     if (!h->C.AQ && params->preset != GPUTO_PRESET_MGCG9) return GPUTO_ERR_WORKSPACE;  // no dense coarsest
     if (params->preset != h->prm.preset) h->hier_ready = false;
     coarse_invalidate(h->C);  // the next setup factors exactly
+    if (h->ln) learn_policy_reset(h->ln);  // the fallback re-times the stage under the new parameters
     h->prm = *params;
     try {
         if (h->prm.precision != GPUTO_PREC_F64) mixed_create(h);

@@ -133,6 +133,7 @@ SIGNATURES = {
     "gputo_learn_info": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _pI, _pI, _pI, _pI]),
     "gputo_learn_params": (_I, [_P, _I, _P]),
     "gputo_learn_reset": (_I, [_P, ctypes.c_uint]),
+    "gputo_learn_logging": (_I, [_P, _I]),
     "gputo_learn_forward": (_I, [_P, _P, _P]),
     "gputo_learn_grad": (_I, [_P, _P, _P, _P]),
     "gputo_learn_train": (_I, [_P, _I, _D, _pD, _pD]),

@@ -37,7 +37,9 @@ class Solver:
         self.lib.gputo_default_params(ctypes.byref(self.params))
         self.params.preset = preset
         self.params.precision = precision
-        self.params.learned = int(bool(learned))
+        # False/0: no learned stage; True/1: applied in every V-cycle of a
+        # trained network; 2: with the solve-level fallback policy
+        self.params.learned = 2 if learned == 2 else int(bool(learned))
         self.params.print_level = print_level
         self.params.l0_kernel = l0_kernel
         self.params.l1_kernel = l1_kernel
@@ -193,6 +195,9 @@ class Solver:
         buf = f64(values).ravel()
         assert buf.size == info["nparam"]
         check(self.lib.gputo_learn_params(self.h, 1, ptr(buf)), "gputo_learn_params")
+
+    def learn_logging(self, on):
+        check(self.lib.gputo_learn_logging(self.h, int(bool(on))), "gputo_learn_logging")
 
     def learn_reset(self, seed=1234):
         check(self.lib.gputo_learn_reset(self.h, seed), "gputo_learn_reset")

@@ -69,7 +69,12 @@ enum gputo_precision { GPUTO_PREC_F64 = 0, GPUTO_PREC_MIXED_F32 = 1, GPUTO_PREC_
 typedef struct gputo_params {
     int preset;        /* enum gputo_preset */
     int precision;     /* enum gputo_precision */
-    int learned;       /* 1: apply the learned level-2 correction inside flexible CG */
+    int learned;       /* learned level-2 correction inside flexible CG: 0 none; 1 applied in
+                          every V-cycle once the network is trained; 2 the same with a
+                          solve-level fallback: successive solves are timed, and the stage
+                          is kept only while a solve with it is faster than the last one
+                          without it (failed trials back off 4 .. 64 solves; a kept stage
+                          is re-checked every 8 solves) */
     int print_level;   /* 0 silent; >=1 summary on stderr; >=2 per-iteration */
     int l0_kernel;     /* level-0 operator of the FP32 V-cycle: 0 by precision, 1 FP32
                           node-gather, 2 FP32 MFMA, 3 split-BF16 MFMA (hi+lo),
@@ -226,6 +231,10 @@ int gputo_learn_info(gputo_handle *h, long long *nparam, int *n0, int *n1, int *
 /* set != 0: load params from buf (marks the network trained); 0: read them. */
 int gputo_learn_params(gputo_handle *h, int set, double *buf);
 int gputo_learn_reset(gputo_handle *h, unsigned seed);
+/* on != 0 (the default): each solve logs the level-2 defects of its first 64
+ * V-cycles as training samples; 0: no logging (a trained network in use
+ * then costs nothing when the fallback has switched it off). */
+int gputo_learn_logging(gputo_handle *h, int on);
 /* y[3][n] = network(x[4][n]) */
 int gputo_learn_forward(gputo_handle *h, const double *x, double *y);
 /* g = d<gy, network(x)>/dparams */

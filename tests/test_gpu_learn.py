@@ -108,3 +108,87 @@ def test_learned_solve_matches_plain(gputo):
     assert abs(C1 - C64) <= 1e-9 * abs(C64)
     s.close()
     s64.close()
+
+
+def test_trained_network_gain(gputo):
+    """What the trained network gains: the same problem solved by the mixed
+    MGCG without and with the stage (learned = 1) after training on the
+    defects of the first solve; the stage must not cost iterations here
+    (the problem it was trained on)."""
+    nelx, nely, nelz, nl = 48, 24, 24, 4
+    Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=43, kind="smooth")
+    s = gputo.Solver(nelx, nely, nelz, nl, precision=2, l0_kernel=3, learned=True)
+    U0, r0, n0, rc0 = s.solve(Ke, F, CX, fixed, 1e-10, 300)  # untrained: the plain V-cycle, logs samples
+    s.learn_train(300, 1e-3)
+    U1, r1, n1, rc1 = s.solve(Ke, F, CX, fixed, 1e-10, 300)
+    print("CG iterations: mixed %d, mixed + trained stage %d (gain %d)" % (n0, n1, n0 - n1), flush=True)
+    assert rc0 == 0 and rc1 == 0 and r1 < 1e-10
+    assert n1 <= n0
+    s.close()
+
+
+def design_loop(gputo, s, n, x0, Ke, F, fixed):
+    run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
+                       cg_tol=1e-8, cg_maxit=300)
+    hist = [run.iterate() for _ in range(n)]
+    run.close()
+    return hist
+
+
+def pretrain_small(gputo, s, Ke, fixed, nelx, nely, nelz, problems=4, steps=50):
+    """The benchmark's pretraining protocol in small: plain solves of
+    cantilevers with random volume fractions and load heights, Adam steps on
+    each solve's logged level-2 defects."""
+    rng = np.random.default_rng(5)
+    nel, nn = nelx * nely * nelz, (nely + 1) * (nelz + 1)
+    for _ in range(problems):
+        x = np.clip(rng.uniform(0.08, 0.3) + rng.uniform(-0.01, 0.01, nel), 1e-3, 1.0)
+        CX = 1e-9 + x ** 3 * (1 - 1e-9)
+        F = np.zeros(3 * (nelx + 1) * nn)
+        nodes = nelx * nn + (nely + 1) * int(rng.integers(0, nelz + 1)) + np.arange(nely + 1)
+        F[3 * nodes + 2] = -1.0 / (nely + 1)
+        U, r, n, rc = s.solve(Ke, F, CX, fixed, 1e-8, 300)
+        assert rc == 0
+        s.learn_train(steps, 1e-3)
+    s.learn_logging(False)
+
+
+@pytest.mark.parametrize("net", ["pretrained", "random"])
+def test_fallback_over_a_design_loop(gputo, net):
+    """12 successive designs of the 48x24x24 cantilever with the learned
+    stage under the solve-level fallback (learned = 2) against the plain
+    mixed MGCG.  Every solve reaches the tolerance, so the design
+    trajectories agree; the fallback keeps the stage only while it shortens
+    the solves, so the CG iterations summed over the loop stay within the
+    plain method's (5% and two iterations of slack: the designs differ at
+    the solver tolerance, which moves single counts by one).  A random
+    network (every V-cycle still takes the energy-optimal step along its
+    output) costs time without saving iterations: the fallback must switch
+    it off after its trial."""
+    nelx, nely, nelz, nl = 48, 24, 24, 4
+    Ke, F, fixed = gputo.cantilever(nelx, nely, nelz)
+    x0 = gputo.problems.perturbed_density(nelx, nely, nelz, 0.12, seed=0)
+    kw = dict(precision=2, l0_kernel=3, mixed_npre0=1, mixed_npost0=1, mixed_npre1=1, mixed_npost1=1,
+              mixed_npre2=6, mixed_npost2=6)
+    plain = gputo.Solver(nelx, nely, nelz, nl, **kw)
+    hp = design_loop(gputo, plain, 12, x0, Ke, F, fixed)
+    plain.close()
+    s = gputo.Solver(nelx, nely, nelz, nl, learned=2, **kw)
+    if net == "pretrained":
+        pretrain_small(gputo, s, Ke, fixed, nelx, nely, nelz)
+    else:
+        s.learn_params(np.random.default_rng(3).normal(size=s.learn_info()["nparam"]) * 0.05)
+        s.learn_logging(False)
+    hl = design_loop(gputo, s, 12, x0, Ke, F, fixed)
+    s.close()
+    on = [k + 1 for k, h in enumerate(hl) if h["learned_accepted"] + h["learned_rejected"] > 0]
+    np_, nl_ = [h["cg_iters"] for h in hp], [h["cg_iters"] for h in hl]
+    print("%s network: stage on in designs %s; CG plain %s (sum %d), learned+fallback %s (sum %d)"
+          % (net, on, np_, sum(np_), nl_, sum(nl_)), flush=True)
+    for a, b in zip(hp, hl):
+        assert b["relres"] < 1e-8
+        assert abs(a["compliance"] - b["compliance"]) <= 1e-6 * abs(a["compliance"])
+    assert sum(nl_) <= 1.05 * sum(np_) + 2
+    assert 1 not in on  # the first solve after training is the plain reference
+    if net == "random":
+        assert on[:1] == [2] and len(on) <= 2, on  # the trial, and at most one more after the backoff

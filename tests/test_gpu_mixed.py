@@ -135,7 +135,12 @@ def test_lagged_coarse_factor(gputo, oracle, lag_mode):
 def test_lagged_factor_dropped_after_parameter_change(gputo):
     """A lagged coarsest factorisation still in flight when the parameters
     change is not adopted: the first solve after the change factors exactly,
-    so it matches a fresh handle with the new parameters bit for bit."""
+    so it matches a fresh handle with the new parameters (same iteration
+    count, U to 1e-12; the pending factor of the old preset's hierarchy
+    would change the preconditioner, and U at the 1e-10 tolerance level).
+    Not bit for bit: a handle created with the REF_C preset has no node
+    stencil for level 1, which the Galerkin preset stores, so the two apply
+    level 1 in different summation orders."""
     nelx, nely, nelz, nl = 32, 16, 16, 4
     Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=61, kind="perturbed")
     x2 = gputo.problems.perturbed_density(nelx, nely, nelz, 0.15, seed=62)
@@ -152,6 +157,7 @@ def test_lagged_factor_dropped_after_parameter_change(gputo):
     fresh = gputo.Solver(nelx, nely, nelz, nl, precision=1, coarse_lag=1, preset=gputo.PRESET_MATLAB_CFG0)
     U2, r2, n2, rc = fresh.solve(Ke, F, CX2, fixed, 1e-10, 300)
     assert rc == 0
-    assert n1 == n2 and np.array_equal(U1, U2)
+    print("after change: nit %d, fresh handle: nit %d, |dU|/|U| %.2e" % (n1, n2, rel(U1, U2)), flush=True)
+    assert n1 == n2 and rel(U1, U2) < 1e-12
     s.close()
     fresh.close()
