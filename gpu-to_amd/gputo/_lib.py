@@ -116,6 +116,7 @@ SIGNATURES = {
     "gputo_device_bytes": (ctypes.c_longlong, [_P]),
     "gputo_matvec_f64": (_I, [_P, _P, _P, _P, _P, _I, _P]),
     "gputo_matvec_mixed": (_I, [_P, _I, _P, _P, _P, _P, _I, _P]),
+    "gputo_matvec_f64_noalloc": (_I, [_I, _I, _I, _P, _P, _P, _P, _I, _P, _P]),
     "gputo_selftest_mfma": (_I, [_pD]),
     "gputo_defect_f64": (_I, [_P, _P, _P, _P, _P, _P, _I, _P]),
     "gputo_diag_f64": (_I, [_P, _P, _P, _P]),

@@ -163,6 +163,20 @@ int gputo_matvec_f64(gputo_handle *h, const double *Ke0, const double *CX, const
  * BF16 MFMA product bf16(Ke) bf16(u). */
 int gputo_matvec_mixed(gputo_handle *h, int kernel, const double *Ke0, const double *CX, const double *u,
                        const double *fixed, int nfd, double *y);
+/* y = K(CX) u with y[fixed] = 0 on device arrays, allocating nothing: the
+ * same result, bit for bit, as gputo_matvec_f64.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops).  No handle: nelx x nely x nelz mesh;
+ * d_CX (nel doubles), d_u and d_y (ndof doubles) and d_fixed (nfd int32,
+ * 0-based DOF indices; entries outside [0, ndof) are skipped) are device
+ * pointers; Ke0 is a host 24x24 array with the unit-H8 entry structure of
+ * the top3D125 Ke0 (GPUTO_ERR_ARGUMENT otherwise).  The node mask lives in
+ * a static device buffer of the library, 8 MiB per GPU (meshes up to
+ * 8,388,608 nodes; GPUTO_ERR_WORKSPACE beyond), shared by every call on that
+ * GPU: calls that overlap on one GPU race.  The work is queued on `stream`
+ * (a hipStream_t; NULL: the null stream) and not waited for. */
+int gputo_matvec_f64_noalloc(int nelx, int nely, int nelz, const double *Ke0, const double *d_CX, const double *d_u,
+                             const int *d_fixed, int nfd, double *d_y, void *stream);
 /* Lane-layout check of the BF16 and FP32 MFMA instructions the level-0
  * kernels use: *max_err = max |D - A B| on small exact integer products
  * (0 expected). */
