@@ -48,6 +48,19 @@ VARIANTS = {
     "mixed_mfma3s1114": (2, 3, 1, 1, 0, 1, 1, 4, 4),  # 1 + 1 on levels 0 and 1, 4 + 4 below
     "mixed_mfma3s1116": (2, 3, 1, 1, 0, 1, 1, 6, 6),  # 1 + 1 on levels 0 and 1, 6 + 6 below
     "mixed_mfma3s1118": (2, 3, 1, 1, 0, 1, 1, 8, 8),  # 1 + 1 on levels 0 and 1, 8 + 8 below
+    "mixed_mfma3s2214": (2, 3, 2, 2, 0, 1, 1, 4, 4),  # 2 + 2 on level 0, 1 + 1 on level 1, 4 + 4 below
+    "mixed_mfma3s2216": (2, 3, 2, 2, 0, 1, 1, 6, 6),  # 2 + 2 on level 0, 1 + 1 on level 1, 6 + 6 below
+    "mixed_mfma3s2224": (2, 3, 2, 2, 0, 2, 2, 4, 4),  # 2 + 2 on levels 0 and 1, 4 + 4 below
+    "mixed_mfma3s1314": (2, 3, 1, 3, 0, 1, 1, 4, 4),  # 1 + 3 on level 0, 1 + 1 on level 1, 4 + 4 below
+    "mixed_mfma3s1312": (2, 3, 1, 3, 0, 1, 1, 2, 2),  # 1 + 3 on level 0, 1 + 1 on level 1, 2 + 2 below
+    "mixed_mfma3s1316": (2, 3, 1, 3, 0, 1, 1, 6, 6),  # 1 + 3 on level 0, 1 + 1 on level 1, 6 + 6 below
+    "mixed_mfma3s1334": (2, 3, 1, 3, 0, 1, 3, 4, 4),  # 1 + 3 on levels 0 and 1, 4 + 4 below
+    "mixed_mfma3s1324": (2, 3, 1, 3, 0, 2, 2, 4, 4),  # 1 + 3 on level 0, 2 + 2 on level 1, 4 + 4 below
+    "mixed_mfma3s3314": (2, 3, 3, 3, 0, 1, 1, 4, 4),  # 3 + 3 on level 0, 1 + 1 on level 1, 4 + 4 below
+    "mixed_mfma3s1514": (2, 3, 1, 5, 0, 1, 1, 4, 4),  # 1 + 5 on level 0, 1 + 1 on level 1, 4 + 4 below
+    "mixed_mfma3s2414": (2, 3, 2, 4, 0, 1, 1, 4, 4),  # 2 + 4 on level 0, 1 + 1 on level 1, 4 + 4 below
+    "mixed_mfma3s1314l2": (2, 3, 1, 3, 2, 1, 1, 4, 4),  # 1314 with the previous coarsest factor in 2 V-cycles
+    "mixed_mfma3s1314l3": (2, 3, 1, 3, 3, 1, 1, 4, 4),  # ... in 3 V-cycles
 }
 
 
@@ -368,7 +381,7 @@ def main():
     ap.add_argument("--nl", type=int, default=5)
     ap.add_argument("--cg-tol", type=float, default=1e-8)
     ap.add_argument("--cg-maxit", type=int, default=300)
-    ap.add_argument("--variants", default="f64,mixed_mfma3s1116,mixed_mfma3s1116+learned")
+    ap.add_argument("--variants", default="f64,mixed_mfma3s1314,mixed_mfma3s1314+learned")
     ap.add_argument("--op-reps", type=int, default=50)
     ap.add_argument("--pretrain-problems", type=int, default=16)
     ap.add_argument("--pretrain-steps", type=int, default=50)

@@ -246,8 +246,12 @@ void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long
 void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit);
 void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit);
 template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc);
-// flexible beta (partials needs 2*kRedBlocks entries)
-void launch_cg_beta_flex(hipStream_t s, const double *r, const double *z, const double *q, long n, CgScalars *sc,
+// P = Z + beta P with the FP32 output of the mixed V-cycle (widened exactly)
+void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc);
+// flexible beta (partials needs 2*kRedBlocks entries); z FP64, or the
+// mixed V-cycle's FP32 output (products in FP64, the same values)
+template <class TZ>
+void launch_cg_beta_flex(hipStream_t s, const double *r, const TZ *z, const double *q, long n, CgScalars *sc,
                          double *partials, int maxit);
 
 void check_hip(hipError_t e, const char *what);  // throws on an error

@@ -841,6 +841,8 @@ void launch_dot_partial(hipStream_t s, const Ta *a, const Tb *b, long n, double 
 template void launch_dot_partial<double, double>(hipStream_t, const double *, const double *, long, double *,
                                                  const int *);
 template void launch_dot_partial<float, float>(hipStream_t, const float *, const float *, long, double *, const int *);
+template void launch_dot_partial<double, float>(hipStream_t, const double *, const float *, long, double *,
+                                                const int *);
 
 __device__ double sum_partials(const double *p, int n)
 {
@@ -951,15 +953,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(CgScalars *sc, const double 
 // Flexible (Polak-Ribiere) beta for a preconditioner that varies between
 // iterations: beta = <z, r - r_old>/rho_old with r - r_old = -alpha q, so
 // beta = -alpha <z,q>/rho_old; rho = <r,z> is kept for the next alpha.
-__global__ __launch_bounds__(kBlock) void k_dot2_partial(const double *__restrict__ r, const double *__restrict__ z,
+template <class TZ>
+__global__ __launch_bounds__(kBlock) void k_dot2_partial(const double *__restrict__ r, const TZ *__restrict__ z,
                                                        const double *__restrict__ q, long n, double *partials,
                                                        const int *ctl)
 {
     if (ctl && ctl[0]) return;
     double a = 0.0, b = 0.0;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) {
-        a += r[i] * z[i];
-        b += q[i] * z[i];
+        const double zi = (double)z[i];  // exact for an FP32 z
+        a += r[i] * zi;
+        b += q[i] * zi;
     }
     a = block_sum(a);
     b = block_sum(b);
@@ -986,20 +990,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta_flex(CgScalars *sc, const do
     }
 }
 
-void launch_cg_beta_flex(hipStream_t s, const double *r, const double *z, const double *q, long n, CgScalars *sc,
+template <class TZ>
+void launch_cg_beta_flex(hipStream_t s, const double *r, const TZ *z, const double *q, long n, CgScalars *sc,
                          double *partials, int maxit)
 {
-    k_dot2_partial<<<kRedBlocks, kBlock, 0, s>>>(r, z, q, n, partials, &sc->done);
+    k_dot2_partial<TZ><<<kRedBlocks, kBlock, 0, s>>>(r, z, q, n, partials, &sc->done);
     k_cg_beta_flex<<<1, kBlock, 0, s>>>(sc, partials, maxit);
 }
+template void launch_cg_beta_flex<double>(hipStream_t, const double *, const double *, const double *, long,
+                                          CgScalars *, double *, int);
+template void launch_cg_beta_flex<float>(hipStream_t, const double *, const float *, const double *, long,
+                                         CgScalars *, double *, int);
 
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ P, const T *__restrict__ Z, long n,
+template <class T, class TZ>
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ P, const TZ *__restrict__ Z, long n,
                                                        const CgScalars *sc)
 {
     if (sc->done) return;
     const T be = (T)sc->beta;
-    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) P[i] = Z[i] + be * P[i];
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
+        P[i] = (T)Z[i] + be * P[i];
 }
 
 void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts)
@@ -1035,9 +1045,13 @@ void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int np
 template <class T>
 void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc)
 {
-    k_cg_update_p<T><<<grid_for(n), kBlock, 0, s>>>(P, Z, n, sc);
+    k_cg_update_p<T, T><<<grid_for(n), kBlock, 0, s>>>(P, Z, n, sc);
 }
 template void launch_cg_update_p<double>(hipStream_t, double *, const double *, long, CgScalars *);
 template void launch_cg_update_p<float>(hipStream_t, float *, const float *, long, CgScalars *);
+void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc)
+{
+    k_cg_update_p<double, float><<<grid_for(n), kBlock, 0, s>>>(P, Z, n, sc);
+}
 
 }  // namespace gputo

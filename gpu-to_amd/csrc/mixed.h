@@ -31,9 +31,11 @@ void mixed_create(gputo_handle *h);
 void mixed_destroy(gputo_handle *h);
 void mixed_stats(gputo_handle *h, gputo_to_stats *st);
 void mixed_begin(gputo_handle *h);                                        // FP32 hierarchy copies
-// z = V32(r); r32_ready: the FP32 copy of r is already in mixed_r32(h)
+// z = V32(r); r32_ready: the FP32 copy of r is already in mixed_r32(h);
+// z null: the result stays in FP32 (mixed_z32(h)) for consumers that widen it
 void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, bool r32_ready = false);
-float *mixed_r32(gputo_handle *h);  // the FP32 right-hand side of the V-cycle
+float *mixed_r32(gputo_handle *h);        // the FP32 right-hand side of the V-cycle
+const float *mixed_z32(gputo_handle *h);  // the FP32 result of the last V-cycle
 void mixed_end(gputo_handle *h, int nit, int converged);
 int mixed_apply_l0(gputo_handle *h, int kind, const double *u, double *y);
 int mixed_bench_operator(gputo_handle *h, int kind, int reps, double *avg_ms);

@@ -137,10 +137,11 @@ void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, b
     const long n = h->L[0].g.ndof;
     if (!r32_ready) launch_copy<double, float>(s, r, m->R, n, ctl);
     vcycle_t<float>(h, m->H, m->R, m->Z, ctl);
-    launch_copy<float, double>(s, m->Z, z, n, ctl);
+    if (z) launch_copy<float, double>(s, m->Z, z, n, ctl);
 }
 
 float *mixed_r32(gputo_handle *h) { return h->mx->R; }
+const float *mixed_z32(gputo_handle *h) { return h->mx->Z; }
 
 // Start of a mixed-precision solve: FP32 copies of the current hierarchy.
 void mixed_begin(gputo_handle *h)

@@ -446,6 +446,9 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     };
     // the residual update also writes the FP32 right-hand side of the mixed V-cycle
     float *r32 = h->prm.precision != GPUTO_PREC_F64 ? mixed_r32(h) : nullptr;
+    // ... whose FP32 result the CG then reads directly (not for a periodic
+    // cell, whose preconditioned residual is folded in FP64)
+    const float *z32 = (r32 && !h->prm.periodic) ? mixed_z32(h) : nullptr;
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
@@ -476,9 +479,23 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
             launch_cg_alpha(s, h->sc, h->partials, nmv);
             launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials, r32);
             launch_cg_check(s, h->sc, h->partials, kRedBlocks, tol, maxit);
+            if (z32) {
+                // the FP32 V-cycle's result feeds the FP64 dots and the P
+                // update directly (widened exactly, so nothing changes but
+                // the traffic of an FP64 copy of z)
+                precond_n(h->R, nullptr, true);
+                if (flex) {
+                    launch_cg_beta_flex<float>(s, h->R, z32, h->Q, n, h->sc, h->partials, maxit);
+                } else {
+                    launch_dot_partial<double, float>(s, h->R, z32, n, h->partials, ctl);
+                    launch_cg_beta(s, h->sc, h->partials, kRedBlocks, maxit);
+                }
+                launch_cg_update_p(s, h->P, z32, n, h->sc);
+                continue;
+            }
             precond_n(h->R, h->Z, r32 != nullptr);
             if (flex) {
-                launch_cg_beta_flex(s, h->R, h->Z, h->Q, n, h->sc, h->partials, maxit);
+                launch_cg_beta_flex<double>(s, h->R, h->Z, h->Q, n, h->sc, h->partials, maxit);
             } else {
                 launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
                 launch_cg_beta(s, h->sc, h->partials, kRedBlocks, maxit);
