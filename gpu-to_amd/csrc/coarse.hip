@@ -4,6 +4,7 @@
 // factor and the two triangular products that apply it in every V-cycle.
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
 #include <vector>
 
 #include "gputo_internal.h"
@@ -392,6 +393,7 @@ void launch_corrcsa(hipStream_t s, const Grid &g, const int *fixed_d, int nfd, d
 // work on whole 32x32 tiles.
 // ---------------------------------------------------------------------------
 constexpr int NB = 32;
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 __global__ void k_assemble(Grid g, const double *__restrict__ A, double *__restrict__ AQ, int N)
 {
@@ -438,23 +440,18 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// One full wave factors the 32x32 diagonal tile at T (row stride ld, lower
-// triangle): lane r < NB holds row r in registers.  At step jj every lane
-// publishes its multiplier L[r][jj] in LDS and reads the column back with
-// broadcast loads (LDS operations of one wave complete in order, so no
-// barrier is involved); the pivot comes from lane jj by v_readlane.
-// Returns true on a non-positive pivot.
-__device__ __forceinline__ bool potrf_tile(double *T, long ld)
+// Columns J0..J1-1 of the tile factorisation in one wave, lane r < NB
+// holding row r in L: at step jj the pivot comes from lane jj and each
+// multiplier L[c][jj] from lane c by v_readlane (scalar operands of the
+// updates), so no memory round trip is on the chain of pivots.  Only
+// columns below J1 are updated.  Entries right of the diagonal (c > r) take
+// garbage that is never read: lj = 0 on rows above the column, and only
+// c <= r is stored.
+template <int J0, int J1>
+__device__ __forceinline__ void potrf_cols(double (&L)[NB], int r, bool &bad)
 {
-    __shared__ double col[64];
-    const int lane = threadIdx.x & 63;
-    const int r = lane < NB ? lane : NB - 1;  // lanes >= NB shadow row NB-1, results dropped
-    double L[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) L[c] = c <= r ? T[(long)r * ld + c] : 0.0;
-    bool bad = false;
-#pragma unroll
-    for (int jj = 0; jj < NB; ++jj) {
+    for (int jj = J0; jj < J1; ++jj) {
         double d = readlane_f64(L[jj], jj);  // updated A[jj][jj]
         if (!(d > 0.0)) {  // not positive definite (or NaN)
             bad = true;
@@ -467,34 +464,75 @@ __device__ __forceinline__ bool potrf_tile(double *T, long ld)
         y = y * fma(-0.5 * d * y, y, 1.5);
         const double lj = r == jj ? d * y : (r > jj ? L[jj] * y : 0.0);
         L[jj] = lj;
-        if (jj + 1 < NB) {
-            col[lane] = lj;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-            for (int c = jj + 1; c < NB; ++c) {
-                const double lc = col[c];  // L[c][jj], broadcast
-                if (c <= r) L[c] -= lj * lc;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
+        for (int c = jj + 1; c < J1; ++c) L[c] -= lj * readlane_f64(lj, c);  // L[c][jj]
     }
+}
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    // LDS operations of one wave complete in order; the fences keep the
+    // compiler from moving them across
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One full wave factors the 32x32 diagonal tile at T in LDS (row stride
+// ld, lower triangle) in two halves of 16 columns.  The first half's
+// contribution to the second, T11 -= L10 L10^T (16x16, K = 16), runs as
+// four v_mfma_f64_16x16x4f64 with L10 staged in LDS (operand layout at
+// dgemm_tile64); the product is passed back through the tile's unused
+// upper-right block.  Returns true on a non-positive pivot.
+__device__ __forceinline__ bool potrf_tile(double *T, int ld)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane < NB ? lane : NB - 1;  // lanes >= NB shadow row NB-1, results dropped
+    double L[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) L[c] = c <= r ? T[r * ld + c] : 0.0;
+    bool bad = false;
+    potrf_cols<0, 16>(L, r, bad);
+    if (lane < NB) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) T[r * ld + c] = L[c];
+    }
+    wave_lds_sync();
+    f64x4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+        const double a = T[(16 + (lane & 15)) * ld + kk + (lane >> 4)];  // A = B = L10
+        d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, a, d, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) T[(4 * v + (lane >> 4)) * ld + 16 + (lane & 15)] = d[v];
+    wave_lds_sync();
+    if (r >= 16) {
+#pragma unroll
+        for (int c = 16; c < NB; ++c) L[c] -= T[(r - 16) * ld + c];
+    }
+    potrf_cols<16, NB>(L, r, bad);
+    wave_lds_sync();  // every lane has read the product before the tile is overwritten
     if (lane < NB) {
 #pragma unroll
         for (int c = 0; c < NB; ++c)
-            if (c <= r) T[(long)r * ld + c] = L[c];
+            if (c <= r) T[r * ld + c] = L[c];
     }
     return bad;
 }
 
 __global__ __launch_bounds__(64) void k_potrf_diag(double *A, int N, int k0, int *flag)
 {
-    const bool bad = potrf_tile(A + (long)k0 * N + k0, N);
+    __shared__ double T[NB][NB + 1];
+    for (int q = threadIdx.x; q < NB * NB; q += 64) T[q / NB][q % NB] = A[(long)(k0 + q / NB) * N + k0 + q % NB];
+    __syncthreads();
+    const bool bad = potrf_tile(&T[0][0], NB + 1);
+    __syncthreads();
+    for (int q = threadIdx.x; q < NB * NB; q += 64)
+        if (q % NB <= q / NB) A[(long)(k0 + q / NB) * N + k0 + q % NB] = T[q / NB][q % NB];
     if (bad && threadIdx.x == 0) flag[0] = 1;
 }
+
 
 // Panel rows solve x L^T = p: x[jj] = (p[jj] - sum_{m<jj} x[m] L[jj][m]) rd[jj]
 // with rd[jj] = 1/L[jj][jj] computed once per tile.
@@ -531,6 +569,16 @@ __device__ __forceinline__ void tri_index(int b, int &ti, int &tj)
     int t = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
     while ((t + 1) * (t + 2) / 2 <= b) ++t;
     while (t * (t + 1) / 2 > b) --t;
+    ti = t;
+    tj = b - t * (t + 1) / 2;
+}
+
+// the same for the few tiles of a band step, by integer steps (the index is
+// wave-uniform, so this runs on the scalar unit)
+__device__ __forceinline__ void tri_index_small(int b, int &ti, int &tj)
+{
+    int t = 0;
+    while ((t + 1) * (t + 2) / 2 <= b) ++t;
     ti = t;
     tj = b - t * (t + 1) / 2;
 }
@@ -593,252 +641,418 @@ __global__ void k_transpose(const double *__restrict__ In, double *__restrict__ 
 // ---------------------------------------------------------------------------
 // The same factorisation in one workgroup when the band spans at most
 // kBandRowsMax rows below the diagonal tile (the coarsest grids of the
-// usual hierarchies): every step stages the diagonal tile and the panel
-// below it in LDS, one wave factors the tile, one thread per panel row
-// solves the panel, and the trailing band is updated from LDS in 4x4
-// register blocks.  One launch instead of three per block column, with the
-// same operations in the same order as the tile-parallel kernels.
+// usual hierarchies), with a look-ahead.  At step k block column k (the
+// diagonal tile and the panel below it) sits in LDS, its tile already
+// updated by the earlier steps:
+//   A. wave 0 factors the tile while waves off its SIMD apply step k-1's
+//      update to the trailing band (in global memory) and to the panel rows
+//      of column k (in LDS);
+//   B. one thread per panel row solves the panel in two halves of 16
+//      columns while the other waves load block column k+1 and store the
+//      factored tile;
+//   C. panel k is stored, and the diagonal tile of column k+1 receives step
+//      k's update, so it is ready for step k+1's factorisation.
+// Every entry receives the updates of the steps in increasing order, as in
+// the tile-parallel kernels.  The trailing products run on
+// v_mfma_f64_16x16x4f64 in 16x16 blocks (operand layout at dgemm_tile64).
+// Blocks above the diagonal of a diagonal tile are never read and are not
+// updated, and blocks whose panel rows lie past the band are exact zeros
+// and are skipped: panel row x of a step (counted from the row below its
+// tile) is zero for x >= bw.
 // ---------------------------------------------------------------------------
 constexpr int kBandRowsMax = 256;
-constexpr int kCholThreads = 512;  // 2 waves per SIMD: 256 VGPRs for the panel row and 4x4 blocks
+constexpr int kCholThreads = 512;  // 2 waves per SIMD: 256 VGPRs for a panel row
+constexpr int kCholWaves = kCholThreads / 64;
+constexpr int kCholRows = NB + kBandRowsMax;  // rows of a block column in LDS
 
-__global__ __launch_bounds__(kCholThreads) void k_chol_band(double *A, int N, int band_rows, int *flag)
+// sum_m P[ra + i][m] P[rb + j][m] for a 16x16 block, D layout of the MFMA;
+// every operand is read from LDS before the chain starts, so the matrix
+// core does not wait on each pair of reads
+__device__ __forceinline__ f64x4 panel_product(const double (*P)[NB + 1], int ra, int rb, int lane)
 {
-    __shared__ double T[NB][NB + 1];
-    __shared__ double P[kBandRowsMax][NB + 1];
-    __shared__ double rdg[NB];  // 1/T[jj][jj]
-    const int tid = threadIdx.x;
+    const int i = lane & 15, k = lane >> 4;
+    double fa[NB / 4], fb[NB / 4];
+#pragma unroll
+    for (int s = 0; s < NB / 4; ++s) {
+        fa[s] = P[ra + i][4 * s + k];
+        fb[s] = P[rb + i][4 * s + k];
+    }
+    f64x4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NB / 4; ++s) d = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s], fb[s], d, 0, 0, 0);
+    return d;
+}
+
+// two blocks at once: independent accumulation chains overlap on the
+// matrix core
+__device__ __forceinline__ void panel_product2(const double (*P)[NB + 1], int ra0, int rb0, int ra1, int rb1, int lane,
+                                               f64x4 &d0, f64x4 &d1)
+{
+    const int i = lane & 15, k = lane >> 4;
+    double fa0[NB / 4], fb0[NB / 4], fa1[NB / 4], fb1[NB / 4];
+#pragma unroll
+    for (int s = 0; s < NB / 4; ++s) {
+        fa0[s] = P[ra0 + i][4 * s + k];
+        fb0[s] = P[rb0 + i][4 * s + k];
+        fa1[s] = P[ra1 + i][4 * s + k];
+        fb1[s] = P[rb1 + i][4 * s + k];
+    }
+    d0 = f64x4{0.0, 0.0, 0.0, 0.0};
+    d1 = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NB / 4; ++s) {
+        d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0[s], fb0[s], d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1[s], fb1[s], d1, 0, 0, 0);
+    }
+}
+
+// Columns J0..J1-1 of one panel row's substitution x L^T = p (x[m] receives
+// its updates in the same order as in k_trsm_panel), updating columns below
+// J1 only; Lr: row c of the factored tile in lane c.
+template <int J0, int J1>
+__device__ __forceinline__ void trsm_cols(double (&x)[NB], const double (&Lr)[NB], const double *rdg)
+{
+#pragma unroll
+    for (int jj = J0; jj < J1; ++jj) {
+        x[jj] *= rdg[jj];
+        // column jj of the tile is read out once x[jj] is final: otherwise
+        // the compiler hoists every readlane and spills them
+        double lcol = Lr[jj];
+        asm volatile("" : "+v"(lcol) : "v"(x[jj]));
+#pragma unroll
+        for (int m = jj + 1; m < J1; ++m) x[m] -= x[jj] * readlane_f64(lcol, m);
+    }
+}
+
+// rows x NB entries of the matrix at G (row stride N) into LDS, kLoads
+// loads in flight per thread; thread t0 of nt
+constexpr int kLoads = 24;
+__device__ __forceinline__ void load_column(double (*D)[NB + 1], const double *G, long N, int rows, int t0, int nt)
+{
+    const int n = rows * NB;
+    for (int q0 = t0; q0 < n; q0 += kLoads * nt) {
+        double v[kLoads];
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u) {
+            const int q = q0 + u * nt;
+            v[u] = q < n ? G[(long)(q / NB) * N + q % NB] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u) {
+            const int q = q0 + u * nt;
+            if (q < n) D[q / NB][q % NB] = v[u];
+        }
+    }
+}
+
+// Step k-1's update of global 16x16 blocks (R, C) of the trailing band, R
+// and C counted in 16-row blocks from row k0 + NB: A -= P_R P_C^T with P
+// the panel of step k-1 in LDS (row NB + 16 R for block row R).  Items
+// i0, i0 + stride, ... < items, mapped to blocks by MAP: 0 the blocks of
+// tile column k+1 (C < 2), 1 those right of it (2 <= C <= R).  All loads
+// of a batch are issued before its first store (the memory counter also
+// waits for earlier stores).
+template <int MAP>
+__device__ __forceinline__ void rest_blocks(double *A, long N, long k0, const double (*P)[NB + 1], int lane, int items,
+                                            int i0, int stride)
+{
+    constexpr int kBatch = 6;
+    const long lo = (long)(lane >> 4) * N + (lane & 15);  // the lane's entry of a block
+    for (int w0 = i0; w0 < items; w0 += kBatch * stride) {
+        double old[kBatch][4];
+        long go[kBatch];  // uniform: the block's first entry
+        int ra[kBatch], rb[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            const int w = w0 + b * stride;
+            if (w < items) {
+                int R, C;
+                if (MAP == 0) {  // slot 2R + C, slot 1 (above the diagonal) skipped
+                    const int sl = w + (w >= 1);
+                    R = sl >> 1;
+                    C = sl & 1;
+                } else {
+                    tri_index_small(w, R, C);
+                    R += 2;
+                    C += 2;
+                }
+                ra[b] = NB + 16 * R;
+                rb[b] = NB + 16 * C;
+                go[b] = (k0 + ra[b]) * N + k0 + rb[b];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) old[b][v] = A[go[b] + lo + 4L * v * N];
+            }
+        }
+        // every product of the batch first, then the stores: a store before
+        // a product would make the next operand waits wait for it too
+        f64x4 d[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; b += 2) {
+            if (w0 + (b + 1) * stride < items)
+                panel_product2(P, NB + ra[b], NB + rb[b], NB + ra[b + 1], NB + rb[b + 1], lane, d[b], d[b + 1]);
+            else if (w0 + b * stride < items)
+                d[b] = panel_product(P, NB + ra[b], NB + rb[b], lane);
+        }
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            if (w0 + b * stride < items) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) A[go[b] + lo + 4L * v * N] = old[b][v] - d[b][v];
+            }
+        }
+    }
+}
+
+// Step k's update of the column buffer D (the next block column, row q of D
+// = panel row q of P's step): the 16x16 blocks in slots sl = 2R + C, C < 2,
+// sl = s0, s0 + stride, ... < s1, slot 1 (above the diagonal) excluded
+__device__ __forceinline__ void column_blocks(double (*D)[NB + 1], const double (*P)[NB + 1], int lane, int s0, int s1,
+                                              int stride)
+{
+    for (int sl = s0; sl < s1; sl += 2 * stride) {
+        const int t0 = sl, t1 = sl + stride;
+        const int ra0 = (t0 >> 1) * 16, rb0 = (t0 & 1) * 16, ra1 = (t1 >> 1) * 16, rb1 = (t1 & 1) * 16;
+        if (t1 < s1) {
+            f64x4 d0, d1;
+            panel_product2(P, NB + ra0, NB + rb0, NB + ra1, NB + rb1, lane, d0, d1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                D[ra0 + 4 * v + (lane >> 4)][rb0 + (lane & 15)] -= d0[v];
+                D[ra1 + 4 * v + (lane >> 4)][rb1 + (lane & 15)] -= d1[v];
+            }
+        } else {
+            const f64x4 d = panel_product(P, NB + ra0, NB + rb0, lane);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) D[ra0 + 4 * v + (lane >> 4)][rb0 + (lane & 15)] -= d[v];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kCholThreads) void k_chol_band(double *A, int N, int band_rows, int bw, int *flag)
+{
+    __shared__ double buf[2][kCholRows][NB + 1];
+    __shared__ double rdg[NB];  // 1/L[jj][jj] of the current tile
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nb = N / NB;
     bool bad = false;
-    for (int kb = 0; kb < nb; ++kb) {
-        const int k0 = kb * NB;
-        const int rows = min(N - k0 - NB, band_rows);  // multiple of NB
-        {
-            // all loads of the step's tile and panel in flight before any store
-            constexpr int kT = NB * NB / kCholThreads, kP = kBandRowsMax * NB / kCholThreads;
-            double tv[kT], pv[kP];
-#pragma unroll
-            for (int u = 0; u < kT; ++u) {
-                const int q = tid + u * kCholThreads;
-                tv[u] = A[(long)(k0 + q / NB) * N + k0 + q % NB];
-            }
-#pragma unroll
-            for (int u = 0; u < kP; ++u) {
-                const int q = tid + u * kCholThreads;
-                pv[u] = A[q < rows * NB ? (long)(k0 + NB + q / NB) * N + k0 + q % NB : 0];
-            }
-#pragma unroll
-            for (int u = 0; u < kT; ++u) {
-                const int q = tid + u * kCholThreads;
-                T[q / NB][q % NB] = tv[u];
-            }
-#pragma unroll
-            for (int u = 0; u < kP; ++u) {
-                const int q = tid + u * kCholThreads;
-                if (q < rows * NB) P[q / NB][q % NB] = pv[u];
-            }
+    load_column(buf[0], A, N, min(N, NB + band_rows), tid, kCholThreads);  // block column 0 as assembled
+    __syncthreads();
+    for (int k = 0; k < nb; ++k) {
+        double(*Cur)[NB + 1] = buf[k & 1];
+        double(*Oth)[NB + 1] = buf[(k + 1) & 1];  // panel k-1 in phase A, block column k+1 from phase B on
+        const long k0 = (long)k * NB;
+        const int rows = min(N - (int)k0 - NB, band_rows);   // panel rows of step k
+        const int prows = k > 0 ? min(N - (int)k0, band_rows) : 0;  // panel rows of step k-1 (from row k0)
+        const int rc = prows > NB ? min(2 * (prows / NB - 1), max(0, (bw - NB + 15) / 16)) : 0;  // rest block rows
+        // A. factor the tile | step k-1's update of the trailing band in
+        // global memory and of column k's panel rows in LDS (slots from 4:
+        // below the diagonal tile), by the waves off the factoring wave's
+        // SIMD (waves w and w + 4 share a SIMD)
+        if (wave == 0) {
+            bad |= potrf_tile(&Cur[0][0], NB + 1);
+            if (lane < NB) rdg[lane] = 1.0 / Cur[lane][lane];
+        } else if (wave != 4 && k > 0) {
+            const int aw = wave - 1 - (wave > 4);  // 0..5
+            if (rc > 0) rest_blocks<0>(A, N, k0, Oth, lane, 2 * rc - 1, aw, kCholWaves - 2);
+            if (rc > 2) rest_blocks<1>(A, N, k0, Oth, lane, (rc - 2) * (rc - 1) / 2, aw, kCholWaves - 2);
+            column_blocks(Cur, Oth, lane, 4 + aw, 2 * (min(prows, bw + 15) / 16), kCholWaves - 2);
         }
         __syncthreads();
-        if (tid < 64) {
-            bad |= potrf_tile(&T[0][0], NB + 1);
-            if (tid < NB) rdg[tid] = 1.0 / T[tid][tid];
-        }
-        __syncthreads();
-        // The panel solve (one row per thread) and the coalesced store of
-        // the factored tile (row-contiguous, by the threads past the panel
-        // rows) run side by side.  Column-oriented substitution: x[m]
-        // receives its updates in the same order (and roundings) as the
-        // row-oriented panel kernel.
-        if (tid < rows) {
-            double x[NB];
+        // B. panel solve, one row per thread, in two halves of 16 columns
+        // (column-oriented substitution; lane c of each wave holds row c of
+        // the factored tile, whose entries reach the other lanes by
+        // v_readlane) | the other waves load block column k+1 into x (it
+        // reaches LDS after the second half, so the loads stay in flight
+        // meanwhile) and store the tile (after the loads, which would
+        // otherwise wait for the stores).  Between the halves, P[:, 16:32]
+        // -= P[:, 0:16] L[16:32, 0:16]^T on the matrix cores.
+        const int tw = (rows + 63) & ~63;  // threads of the waves with panel rows
+        double Lr[NB], x[NB];
+        if (tid < tw) {
+            const int lr = lane < NB ? lane : NB - 1;
 #pragma unroll
-            for (int c = 0; c < NB; ++c) x[c] = P[tid][c];
+            for (int c = 0; c < NB; ++c) Lr[c] = Cur[lr][c];
 #pragma unroll
-            for (int jj = 0; jj < NB; ++jj) {
-                x[jj] *= rdg[jj];
+            for (int c = 0; c < 16; ++c) x[c] = Cur[NB + tid][c];  // rows past the panel: stale, not stored
+            trsm_cols<0, 16>(x, Lr, rdg);
+            if (tid < rows) {
 #pragma unroll
-                for (int m = jj + 1; m < NB; ++m) x[m] -= x[jj] * T[m][jj];
+                for (int c = 0; c < 16; ++c) Cur[NB + tid][c] = x[c];
             }
+        } else {
+            const int t0 = tid - tw, nt = kCholThreads - tw;
+            const int n = k + 1 < nb ? min(N - (int)k0 - NB, NB + band_rows) * NB : 0;
+            const double *G = A + (k0 + NB) * N + k0 + NB;
 #pragma unroll
-            for (int c = 0; c < NB; ++c) P[tid][c] = x[c];
-        } else if (tid >= kBandRowsMax) {
-            for (int q = tid - kBandRowsMax; q < NB * NB; q += kCholThreads - kBandRowsMax) {
+            for (int u = 0; u < NB; ++u) {
+                const int q = t0 + u * nt;
+                x[u] = q < n ? G[(long)(q / NB) * N + q % NB] : 0.0;
+            }
+            for (int q = t0; q < NB * NB; q += nt) {
                 const int r = q / NB, c = q % NB;
-                if (c <= r) A[(long)(k0 + r) * N + k0 + c] = T[r][c];
+                if (c <= r) A[(k0 + r) * N + k0 + c] = Cur[r][c];
             }
         }
         __syncthreads();
-        // coalesced store of the solved panel; the trailing update below
-        // reads it from LDS and writes only columns past it
-        for (int q = tid; q < rows * NB; q += kCholThreads) A[(long)(k0 + NB + q / NB) * N + k0 + q % NB] = P[q / NB][q % NB];
-        const int nt = rows / NB;
-        const int nsub = nt * (nt + 1) / 2 * 64;  // 64 4x4 blocks per tile
-        for (int w = tid; w < nsub; w += kCholThreads) {
-            int ti, tj;
-            tri_index(w >> 6, ti, tj);
-            const int r0 = ti * NB + ((w >> 3) & 7) * 4, c0 = tj * NB + (w & 7) * 4;
-            double av[4][4];  // the trailing entries, loaded while the product runs
+        for (int w = wave; w < min(rows, bw + 15) / 16; w += kCholWaves) {  // panel rows past the band are zero
+            f64x4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int kk = 0; kk < 16; kk += 4)
+                d = __builtin_amdgcn_mfma_f64_16x16x4f64(Cur[NB + 16 * w + (lane & 15)][kk + (lane >> 4)],
+                                                         Cur[16 + (lane & 15)][kk + (lane >> 4)], d, 0, 0, 0);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) av[i][j] = A[(long)(k0 + NB + r0 + i) * N + k0 + NB + c0 + j];
-            double acc[4][4];
+            for (int v = 0; v < 4; ++v) Cur[NB + 16 * w + 4 * v + (lane >> 4)][16 + (lane & 15)] -= d[v];
+        }
+        __syncthreads();
+        if (tid < tw) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int c = 16; c < NB; ++c) x[c] = Cur[NB + tid][c];
+            trsm_cols<16, NB>(x, Lr, rdg);
+            if (tid < rows) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-#pragma unroll 8
-            for (int m = 0; m < NB; ++m) {
-                double a[4], b[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    a[i] = P[r0 + i][m];
-                    b[i] = P[c0 + i][m];
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+                for (int c = 16; c < NB; ++c) Cur[NB + tid][c] = x[c];
             }
+        } else {
+            const int t0 = tid - tw, nt = kCholThreads - tw;
+            const int n = k + 1 < nb ? min(N - (int)k0 - NB, NB + band_rows) * NB : 0;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int u = 0; u < NB; ++u) {
+                const int q = t0 + u * nt;
+                if (q < n) Oth[q / NB][q % NB] = x[u];
+            }
+            load_column(Oth, A + (k0 + NB) * N + k0 + NB, N, n / NB, t0 + NB * nt, nt);  // the rest, if any
+        }
+        __syncthreads();
+        // C. store panel k; the diagonal tile of column k+1 -= panel k's
+        // first tile x its transpose (slots 0, 2, 3)
+        for (int q = tid; q < rows * NB; q += kCholThreads) A[(k0 + NB + q / NB) * N + k0 + q % NB] = Cur[NB + q / NB][q % NB];
+        if (k + 1 < nb && wave < 3) {
+            const int sl = wave == 0 ? 0 : wave + 1;
+            const int ra = (sl >> 1) * 16, rb = (sl & 1) * 16;
+            const f64x4 d = panel_product(Cur, NB + ra, NB + rb, lane);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) A[(long)(k0 + NB + r0 + i) * N + k0 + NB + c0 + j] = av[i][j] - acc[i][j];
+            for (int v = 0; v < 4; ++v) Oth[ra + 4 * v + (lane >> 4)][rb + (lane & 15)] -= d[v];
         }
         __syncthreads();
     }
     if (bad && tid == 0) flag[0] = 1;
 }
 
-// Column block jb of L^-1 in one workgroup, marching down the block rows:
-// X[ib][jb] = -inv(L[ib][ib]) sum_{kb=jb}^{ib-1} L[ib][kb] X[kb][jb] with
-// the sum over the factor's band (kb >= ib - nbt) and X[jb][jb] =
-// inv(L[jb][jb]) written by k_trtri_diag.  The column blocks are
-// independent: one launch of nb - 1 workgroups.  The last kLinvWin tiles of
-// the column stay in LDS (a circular window), the factor tile of the next
-// product is prefetched into registers while the current one is
-// multiplied, and each thread owns 2x2 entries of the 32x32 result.
-constexpr int kLinvWin = 8;
-constexpr int kXs = NB + 2;  // even row stride: 16-byte aligned pairs
+// ---------------------------------------------------------------------------
+// L^-1 by recursive halving.  With the block rows of a group of g tiles
+// split in halves, L = [L11 0; L21 L22] and inv(L) = [X11 0; X21 X22] with
+//   X21 = -X22 L21 X11,
+// where X11 and X22, the inverses of the halves, come from the level below
+// (g/2; level 1 is the 32x32 diagonal tile inverses of k_trtri_diag).  L21
+// has nonzero tiles only in its top-right nbt x nbt corner (the band), so
+//   W   = L21[0:r, n1-c:n1] X11[n1-c:n1, 0:n1]      (r x n1 tiles)
+//   X21 = -X22[0:n2, 0:r] W                          (n2 x n1 tiles)
+// with n1 = g/2, n2 <= n1 the tiles of the second half, r = min(nbt, n2)
+// and c = min(nbt, n1).  The groups of a level are independent: each level
+// is two batched products (blockIdx.z = group), so the inverse takes
+// 2 ceil(log2 nb) launches instead of a march down nb dependent block rows.
+// ---------------------------------------------------------------------------
+struct LinvLevel {
+    int g, nb, nbt;  // group size in tiles, tiles of the matrix, band in tiles
+    long wstride;    // doubles of W per group
+};
 
-__global__ __launch_bounds__(256) void k_linv_cols(const double *A, double *Linv, int N, int nbt)
+// One 64x64 tile of C = alpha A B, with A (M x K), B (K x Nc) and C
+// row-major (leading dimensions lda, ldb, ldc), M, Nc and K multiples of 32.
+// Four waves, each a 32x32 quadrant as 2 x 2 v_mfma_f64_16x16x4f64 tiles;
+// the operand layout of that instruction: A[i][k] in lane i + 16k, B[k][j]
+// in lane j + 16k, D[4v + q][j] in element v of lane j + 16q.
+__device__ __forceinline__ void dgemm_tile64(const double *__restrict__ A, long lda, const double *__restrict__ B,
+                                             long ldb, double *__restrict__ C, long ldc, int M, int Nc, int K,
+                                             double alpha)
 {
-    __shared__ __align__(16) double X[kLinvWin][NB][kXs];
-    __shared__ double La[NB][NB + 1];
-    const int jb = blockIdx.x, nb = N / NB, tid = threadIdx.x;
-    const int r0 = (tid >> 4) * 2, c0 = (tid & 15) * 2;
-    // 4 tile entries per thread for the copies: rows tid/32 + 8u, column tid%32
-    const int cr = tid >> 5, cc = tid & 31;
-    for (int u = 0; u < 4; ++u) X[jb % kLinvWin][cr + 8 * u][cc] = Linv[(long)(jb * NB + cr + 8 * u) * N + jb * NB + cc];
-    for (int ib = jb + 1; ib < nb; ++ib) {
-        const int kb0 = max(jb, ib - nbt);
-        double pf[4];
+    __shared__ double As[64][33];
+    __shared__ double Bs[32][65];
+    const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+    if (i0 >= M || j0 >= Nc) return;  // whole workgroup
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    f64x4 d[2][2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pf[u] = A[(long)(ib * NB + cr + 8 * u) * N + kb0 * NB + cc];
-        double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-        for (int kb = kb0; kb <= ib; ++kb) {
-            __syncthreads();  // previous readers of La are done
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) La[cr + 8 * u][cc] = pf[u];
-            if (kb < ib) {
-                // next factor tile, or inv(L[ib][ib]) for the final product
-                const double *src = kb + 1 < ib ? A + (long)(ib * NB) * N + (kb + 1) * NB
-                                                : Linv + (long)(ib * NB) * N + ib * NB;
+        for (int b = 0; b < 2; ++b) d[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        // rows past M (A) and columns past Nc (B) read as zeros
+        double av[8], bv[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) pf[u] = src[(long)(cr + 8 * u) * N + cc];
-            } else {
-                // final product: B = the accumulated sum
+        for (int u = 0; u < 8; ++u) {
+            const int r = (tid >> 5) + 8 * u, c = tid & 31;
+            av[u] = i0 + r < M ? A[(long)(i0 + r) * lda + k0 + c] : 0.0;
+        }
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+        for (int u = 0; u < 8; ++u) {
+            const int r = (tid >> 6) + 4 * u, c = tid & 63;
+            bv[u] = j0 + c < Nc ? B[(long)(k0 + r) * ldb + j0 + c] : 0.0;
+        }
+        __syncthreads();  // the previous step's readers are done
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) X[ib % kLinvWin][r0 + i][c0 + j] = acc[i][j];
-            }
-            __syncthreads();
-            const double(*B)[kXs] = X[(kb < ib ? kb : ib) % kLinvWin];
-            double sm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll 8
-            for (int m = 0; m < NB; ++m) {
-                const double a0 = La[r0][m], a1 = La[r0 + 1][m];
-                const double2 bv = *reinterpret_cast<const double2 *>(&B[m][c0]);
-                sm[0][0] += a0 * bv.x;
-                sm[0][1] += a0 * bv.y;
-                sm[1][0] += a1 * bv.x;
-                sm[1][1] += a1 * bv.y;
-            }
-            if (kb < ib) {
+        for (int u = 0; u < 8; ++u) As[(tid >> 5) + 8 * u][tid & 31] = av[u];
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+        for (int u = 0; u < 8; ++u) Bs[(tid >> 6) + 4 * u][tid & 63] = bv[u];
+        __syncthreads();
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) acc[i][j] += sm[i][j];
-            } else {
-                __syncthreads();  // every thread has read the accumulated sum
+        for (int kk = 0; kk < 32; kk += 4) {
+            double fa[2], fb[2];
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+            for (int a = 0; a < 2; ++a) fa[a] = As[wr * 32 + a * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        X[ib % kLinvWin][r0 + i][c0 + j] = -sm[i][j];
-                        Linv[(long)(ib * NB + r0 + i) * N + jb * NB + c0 + j] = -sm[i][j];
-                    }
-            }
+            for (int b = 0; b < 2; ++b) fb[b] = Bs[kk + (lane >> 4)][wc * 32 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) d[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], d[a][b], 0, 0, 0);
         }
     }
+    const int qi = i0 + wr * 32, qj = j0 + wc * 32;
+    if (qi >= M || qj >= Nc) return;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                C[(long)(qi + a * 16 + 4 * v + (lane >> 4)) * ldc + qj + b * 16 + (lane & 15)] = alpha * d[a][b][v];
 }
 
-// The same product for bands wider than the window: the previous tiles of
-// the column are read back from Linv (written by this workgroup, visible
-// after the barrier).
-__global__ __launch_bounds__(256) void k_linv_cols_wide(const double *A, double *Linv, int N, int nbt)
+template <int STEP>
+__global__ __launch_bounds__(256) void k_linv_level(const double *__restrict__ L, double *X, double *W, int N,
+                                                   LinvLevel lv)
 {
-    __shared__ double Ta[NB][NB + 1], Tb[NB][NB + 1];
-    const int jb = blockIdx.x, nb = N / NB, tid = threadIdx.x;
-    const int r0 = (tid >> 4) * 2, c0 = (tid & 15) * 2;
-    for (int ib = jb + 1; ib < nb; ++ib) {
-        double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-        for (int kb = max(jb, ib - nbt); kb < ib; ++kb) {
-            __syncthreads();
-            for (int q = tid; q < NB * NB; q += 256) {
-                const int r = q / NB, c = q % NB;
-                Ta[r][c] = A[(long)(ib * NB + r) * N + kb * NB + c];
-                Tb[r][c] = Linv[(long)(kb * NB + r) * N + jb * NB + c];
-            }
-            __syncthreads();
-            double sm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll 8
-            for (int m = 0; m < NB; ++m) {
-                const double a0 = Ta[r0][m], a1 = Ta[r0 + 1][m], b0 = Tb[m][c0], b1 = Tb[m][c0 + 1];
-                sm[0][0] += a0 * b0;
-                sm[0][1] += a0 * b1;
-                sm[1][0] += a1 * b0;
-                sm[1][1] += a1 * b1;
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] += sm[i][j];
-        }
-        __syncthreads();
-        for (int q = tid; q < NB * NB; q += 256) {
-            const int r = q / NB, c = q % NB;
-            Ta[r][c] = Linv[(long)(ib * NB + r) * N + ib * NB + c];  // inv(L[ib][ib])
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) Tb[r0 + i][c0 + j] = acc[i][j];
-        __syncthreads();
-        double sm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll 8
-        for (int m = 0; m < NB; ++m) {
-            const double a0 = Ta[r0][m], a1 = Ta[r0 + 1][m], b0 = Tb[m][c0], b1 = Tb[m][c0 + 1];
-            sm[0][0] += a0 * b0;
-            sm[0][1] += a0 * b1;
-            sm[1][0] += a1 * b0;
-            sm[1][1] += a1 * b1;
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) Linv[(long)(ib * NB + r0 + i) * N + jb * NB + c0 + j] = -sm[i][j];
+    const int z = blockIdx.z, n1 = lv.g / 2, s = z * lv.g, h = s + n1;
+    if (h >= lv.nb) return;
+    const int n2 = min(lv.g, lv.nb - s) - n1;
+    const int r = min(lv.nbt, n2), c = min(lv.nbt, n1);
+    double *Wz = W + z * lv.wstride;
+    const long ldw = (long)n1 * NB;
+    if (STEP == 0)  // W = L21[0:r, n1-c:n1] X11[n1-c:n1, 0:n1]
+        dgemm_tile64(L + (long)h * NB * N + (long)(h - c) * NB, N, X + (long)(h - c) * NB * N + (long)s * NB, N, Wz,
+                     ldw, r * NB, n1 * NB, c * NB, 1.0);
+    else  // X21 = -X22[0:n2, 0:r] W
+        dgemm_tile64(X + (long)h * NB * N + (long)h * NB, N, Wz, ldw, X + (long)h * NB * N + (long)s * NB, N, n2 * NB,
+                     n1 * NB, r * NB, -1.0);
+}
+
+// Levels 2, 4, ... of the halving; X holds the diagonal tile inverses and
+// zeros above them, W has room for wcap doubles.
+static void launch_linv_levels(hipStream_t s, const double *L, double *X, double *W, long wcap, int N, int nbt)
+{
+    const int nb = N / NB;
+    for (int g = 2; g / 2 < nb; g *= 2) {
+        const int n1 = g / 2;
+        const int groups = (nb - n1 + g - 1) / g;
+        const int rmax = std::min(nbt, n1);
+        const LinvLevel lv{g, nb, nbt, (long)rmax * NB * n1 * NB};
+        if (lv.wstride * groups > wcap) throw std::runtime_error("coarsest inverse: work buffer too small");
+        const unsigned tn = (unsigned)((n1 * NB + 63) / 64);
+        k_linv_level<0><<<dim3(tn, (unsigned)((rmax * NB + 63) / 64), (unsigned)groups), 256, 0, s>>>(L, X, W, N, lv);
+        k_linv_level<1><<<dim3(tn, tn, (unsigned)groups), 256, 0, s>>>(L, X, W, N, lv);
     }
 }
 
@@ -848,7 +1062,6 @@ void coarse_factor(hipStream_t s0, Coarse &C, bool lag)
     C.solve_waits = C.target == C.cur;
     C.Linv = C.Lbuf[C.cur];
     C.LinvT = C.LTbuf[C.cur];
-    double *Li = C.Lbuf[C.target], *LiT = C.LTbuf[C.target];
     const int N = (C.n + NB - 1) / NB * NB;
     const int nb = N / NB;
     // the factor of a banded matrix keeps its band: panel rows and trailing
@@ -860,7 +1073,7 @@ void coarse_factor(hipStream_t s0, Coarse &C, bool lag)
     HIPCHK(hipStreamWaitEvent(s, C.ev_in, 0));
     HIPCHK(hipMemsetAsync(C.flag, 0, sizeof(int), s));
     if (band_rows <= kBandRowsMax) {
-        k_chol_band<<<1, kCholThreads, 0, s>>>(C.AQ, N, band_rows, C.flag);
+        k_chol_band<<<1, kCholThreads, 0, s>>>(C.AQ, N, band_rows, bw, C.flag);
     } else {
         for (int kb = 0; kb < nb; ++kb) {
             const int k0 = kb * NB;
@@ -873,24 +1086,36 @@ void coarse_factor(hipStream_t s0, Coarse &C, bool lag)
             }
         }
     }
+    HIPCHK(hipGetLastError());
+    // the inverse's launches are enqueued by coarse_finish, when the host
+    // has enqueued the start of the solve on the main stream
+    C.inv_pending = true;
+    C.pending = true;
+}
+
+void coarse_finish(const Coarse &C)
+{
+    if (!C.inv_pending) return;
+    C.inv_pending = false;
+    const int N = (C.n + NB - 1) / NB * NB;
+    const int nb = N / NB;
+    const int bw = std::max(1, std::min(C.bw, N));
+    double *Li = C.Lbuf[C.target], *LiT = C.LTbuf[C.target];
+    hipStream_t s = C.side;
     launch_fill(s, Li, (long)N * N, 0.0);
     k_trtri_diag<<<nb, 64, 0, s>>>(C.AQ, Li, N);
     const int nbt = std::min((bw + 2 * NB - 2) / NB, nb);
-    if (nb > 1) {
-        if (nbt < kLinvWin)
-            k_linv_cols<<<nb - 1, 256, 0, s>>>(C.AQ, Li, N, nbt);
-        else
-            k_linv_cols_wide<<<nb - 1, 256, 0, s>>>(C.AQ, Li, N, nbt);
-    }
+    // the transpose's buffer is free until the last step: it holds W
+    launch_linv_levels(s, C.AQ, Li, LiT, (long)N * N, N, nbt);
     k_transpose<<<dim3(nb, nb), 256, 0, s>>>(Li, LiT, N);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(C.ev_done, s));
-    C.pending = true;
 }
 
 int coarse_check(Coarse &C)
 {
     if (!C.pending || !C.solve_waits) return 0;  // a lagged factor is checked when adopted
+    coarse_finish(C);
     int flag = 0;
     HIPCHK(hipMemcpyAsync(&flag, C.flag, sizeof(int), hipMemcpyDeviceToHost, C.side));
     HIPCHK(hipStreamSynchronize(C.side));
@@ -913,6 +1138,7 @@ void coarse_adopt(Coarse &C)
         if (invalid) C.have = false;
         return;
     }
+    coarse_finish(C);
     int flag = 0;
     HIPCHK(hipMemcpyAsync(&flag, C.flag, sizeof(int), hipMemcpyDeviceToHost, C.side));
     HIPCHK(hipStreamSynchronize(C.side));  // launched a solve ago: normally finished
@@ -930,6 +1156,7 @@ void coarse_adopt(Coarse &C)
 void coarse_switch(Coarse &C, hipStream_t s)
 {
     if (!C.pending || C.solve_waits) return;  // no lagged factorisation in flight
+    coarse_finish(C);
     HIPCHK(hipStreamWaitEvent(s, C.ev_done, 0));
     C.cur = C.target;
     C.Linv = C.Lbuf[C.cur];
@@ -973,6 +1200,7 @@ template <class T>
 void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl)
 {
     const int N = (C.n + NB - 1) / NB * NB;
+    coarse_finish(C);  // also a lagged factorisation's, which runs meanwhile
     if (C.solve_waits) HIPCHK(hipStreamWaitEvent(s, C.ev_done, 0));  // the factorisation of this hierarchy
     k_trmv<false, T, double><<<C.n, 64, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);
     k_trmv<true, double, T><<<C.n, 64, 0, s>>>(C.LinvT, C.y, x, N, C.n, mask, ctl);

@@ -136,6 +136,7 @@ struct Coarse {
     hipStream_t side = nullptr;
     hipEvent_t ev_in = nullptr, ev_done = nullptr;
     bool pending = false;  // factorisation launched, SPD flag not read yet
+    mutable bool inv_pending = false;  // its inverse not enqueued yet (coarse_finish)
 };
 
 struct MixedState;
@@ -224,6 +225,9 @@ void launch_assemble(hipStream_t s, const Grid &g, const double *A, double *AQ);
 // exists, write the new one into the other buffer and let the solves keep
 // the earlier one (coarse_adopt switches at the next setup).
 void coarse_factor(hipStream_t s, Coarse &C, bool lag);
+// enqueues the inverse of a factorisation launched by coarse_factor and
+// records ev_done; every wait on ev_done comes after it
+void coarse_finish(const Coarse &C);
 int coarse_check(Coarse &C);  // waits for an exact factorisation; GPUTO_ERR_NOT_SPD on a bad pivot
 void coarse_adopt(Coarse &C);  // before a setup: take a finished lagged factor
 // after a parameter or operator change: no factor of an earlier setup

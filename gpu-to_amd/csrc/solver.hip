@@ -307,6 +307,7 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
     if (int rc = validate_fixed(fixed0, L0.g.ndof)) return rc;
     // a factorisation still running on the side stream reads the coarsest
     // matrix this setup overwrites; a finished lagged one becomes current
+    coarse_finish(h->C);
     if (h->C.ev_done) HIPCHK(hipStreamWaitEvent(s, h->C.ev_done, 0));
     coarse_adopt(h->C);
     ensure_tables(h);

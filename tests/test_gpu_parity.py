@@ -159,8 +159,15 @@ def test_transfers(solvers, oracle):
         assert rel(s.interp_add(l, uc, uf), oracle.interp_add(uc, uf, fx, fy, fz)) < 1e-14
 
 
-def test_coarse_solve(solvers, oracle):
-    nelx, nely, nelz, nl = 16, 8, 8, 3
+@pytest.mark.parametrize("mesh", [(16, 8, 8, 3), (48, 24, 24, 3), (32, 16, 16, 2)])
+def test_coarse_solve(solvers, oracle, mesh):
+    """The coarsest solve through the explicit inverse of the factor against
+    numpy's dense solve: a 5x3x3-node grid (5 tiles), the 13x7x7-node grid of
+    the 192x96x96 hierarchy (1911 DOFs, band 173, 60 tiles: one-workgroup
+    factorisation, six halving levels of the inverse) and a 17x9x9-node grid
+    (band 275: tile-parallel factorisation)."""
+    nelx, nely, nelz, nl = mesh
+    print("case", mesh, flush=True)
     Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=8)
     s = solvers(nelx, nely, nelz, nl)
     s.setup(Ke, CX, fixed)
