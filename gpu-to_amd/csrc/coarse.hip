@@ -11,6 +11,8 @@
 
 namespace gputo {
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 __host__ __device__ constexpr int lidx2(int x, int y, int z) { return 4 * z + (y ? (x ? 2 : 3) : (x ? 1 : 0)); }
 __host__ __device__ constexpr int pidx2(int m, int n) { return n <= m ? m * (m + 1) / 2 + n : n * (n + 1) / 2 + m; }
 __device__ constexpr int kIE[8] = {0, 1, 1, 0, 0, 1, 1, 0};
@@ -141,73 +143,83 @@ void launch_coarse_galerkin(hipStream_t s, const Grid &gc, const double *Af, con
 // work per level is one [nel_l x 8^l] x [8^l x 300] product.  Same operator
 // as the child-by-child product, summed in a different order.
 // ---------------------------------------------------------------------------
-constexpr int kGtE = 16;  // elements per block
-constexpr int kGtF = 16;  // table rows staged per step
+// The product on the matrix cores: D[p][e] = sum_f T[f][p] CX0[fine(e, f)]
+// as v_mfma_f64_16x16x4f64 tiles (operand layout at dgemm_tile64), p (the
+// packed entry) along the rows and e along the columns, so the stores of a
+// tile row are contiguous in A (p-major).  One wave per 32 x 32 tile of
+// (p, e), four waves per workgroup; the operands come straight from L2 (the
+// table is at most 1.2 MB, CX0 is read once per p tile), kGtAhead K steps
+// of loads in flight.
+constexpr int kGtAhead = 8;
 
-template <int KP>
 __global__ __launch_bounds__(256) void k_galerkin_table(Grid gc, int depth, const double *__restrict__ T,
                                                        const double *__restrict__ CX0, double *__restrict__ A)
 {
-    constexpr int PW = 16 * KP;  // packed entries per block
-    __shared__ double sT[kGtF][PW];
-    __shared__ double sC[kGtE][kGtF + 1];
-    const int nchunk = (300 + PW - 1) / PW;
-    const long eb = blockIdx.x / nchunk;
-    const int p0 = (blockIdx.x % nchunk) * PW;
-    const int el = threadIdx.x / 16, pl = threadIdx.x % 16;
+    const int lane = threadIdx.x & 63;
+    const long wt = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long etiles = (gc.nel + 31) / 32;
+    const int p0 = (int)(wt / etiles) * 32;
+    const long e0 = (wt % etiles) * 32;
+    if (p0 >= 300) return;  // whole wave
     const int S = 1 << depth, nf = S * S * S;
     const long ny0 = (long)gc.ny * S, nz0 = (long)gc.nz * S;
-    // the element staged by this thread's sC slot (el, ff = pl)
-    const long es = eb * kGtE + el;
-    long ebase = -1;
-    if (es < gc.nel) {
-        const long i = es / ((long)gc.ny * gc.nz), k = (es / gc.ny) % gc.nz, j = es % gc.ny;
-        ebase = (i * S) * nz0 * ny0 + (k * S) * ny0 + j * S;
-    }
-    double acc[KP];
+    const int i16 = lane & 15, kq = lane >> 4;
+    // B operand: element e0 + 16 b + i16 of this lane, its first fine element
+    long eb[2];
 #pragma unroll
-    for (int q = 0; q < KP; ++q) acc[q] = 0.0;
-    for (int f0 = 0; f0 < nf; f0 += kGtF) {
-        __syncthreads();
-        for (int q = threadIdx.x; q < kGtF * PW; q += 256) {
-            const int ff = q / PW, pp = q % PW, p = p0 + pp;
-            sT[ff][pp] = (f0 + ff < nf && p < 300) ? T[(long)(f0 + ff) * 300 + p] : 0.0;
+    for (int b = 0; b < 2; ++b) {
+        const long e = e0 + 16 * b + i16;
+        eb[b] = -1;
+        if (e < gc.nel) {
+            const long i = e / ((long)gc.ny * gc.nz), k = (e / gc.ny) % gc.nz, j = e % gc.ny;
+            eb[b] = (i * S) * nz0 * ny0 + (k * S) * ny0 + j * S;
         }
-        {
-            const int f = f0 + pl;
-            double c = 0.0;
-            if (ebase >= 0 && f < nf) {
-                const int fj = f % S, fk = (f / S) % S, fi = f / (S * S);
-                c = CX0[ebase + (long)fi * nz0 * ny0 + (long)fk * ny0 + fj];
+    }
+    f64x4 d[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) d[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int f0 = 0; f0 < nf; f0 += 4 * kGtAhead) {
+        double fa[kGtAhead][2], fb[kGtAhead][2];
+#pragma unroll
+        for (int u = 0; u < kGtAhead; ++u) {
+            const int f = f0 + 4 * u + kq;  // this lane's K index
+            const bool fin = f < nf;
+            const long fo = (long)(f / (S * S)) * nz0 * ny0 + (long)((f / S) % S) * ny0 + f % S;
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int p = p0 + 16 * a + i16;
+                fa[u][a] = (fin && p < 300) ? T[(long)f * 300 + p] : 0.0;
             }
-            sC[el][pl] = c;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int ff = 0; ff < kGtF; ++ff) {
-            const double c = sC[el][ff];
 #pragma unroll
-            for (int q = 0; q < KP; ++q) acc[q] = fma(c, sT[ff][pl + 16 * q], acc[q]);
+            for (int b = 0; b < 2; ++b) fb[u][b] = (fin && eb[b] >= 0) ? CX0[eb[b] + fo] : 0.0;
         }
-    }
-    if (es >= gc.nel) return;
 #pragma unroll
-    for (int q = 0; q < KP; ++q) {
-        const int p = p0 + pl + 16 * q;
-        if (p < 300 && p < p0 + PW) A[(long)p * gc.nel + es] = acc[q];
+        for (int u = 0; u < kGtAhead; ++u)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    d[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][a], fb[u][b], d[a][b], 0, 0, 0);
     }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const long e = e0 + 16 * b + i16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int p = p0 + 16 * a + 4 * v + kq;
+                if (p < 300 && e < gc.nel) A[(long)p * gc.nel + e] = d[a][b][v];
+            }
+        }
 }
 
 void launch_galerkin_table(hipStream_t s, const Grid &gc, int depth, const double *T, const double *CX0, double *A)
 {
-    const long eblocks = (gc.nel + kGtE - 1) / kGtE;
-    // split the 300 entries over more blocks when there are few elements
-    if (eblocks >= 512)
-        k_galerkin_table<19><<<(unsigned)eblocks, 256, 0, s>>>(gc, depth, T, CX0, A);
-    else if (eblocks >= 128)
-        k_galerkin_table<10><<<(unsigned)(eblocks * 2), 256, 0, s>>>(gc, depth, T, CX0, A);
-    else
-        k_galerkin_table<5><<<(unsigned)(eblocks * 4), 256, 0, s>>>(gc, depth, T, CX0, A);
+    const long waves = 10 * ((gc.nel + 31) / 32);  // ceil(300 / 32) p tiles
+    k_galerkin_table<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(gc, depth, T, CX0, A);
 }
 
 // Host: the tables T_l for l = 1..maxdepth, packed [8^l][300] with the
@@ -393,7 +405,6 @@ void launch_corrcsa(hipStream_t s, const Grid &g, const int *fixed_d, int nfd, d
 // work on whole 32x32 tiles.
 // ---------------------------------------------------------------------------
 constexpr int NB = 32;
-typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 __global__ void k_assemble(Grid g, const double *__restrict__ A, double *__restrict__ AQ, int N)
 {
