@@ -1185,7 +1185,11 @@ __global__ __launch_bounds__(64) void k_trmv(const double *__restrict__ M, const
                                             To *__restrict__ x, int N, int n, const uint8_t *__restrict__ mask,
                                             const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
+    // lower: b has n entries, x has N (the padded y); upper: the reverse
+    b = batch_ptr(b, UPPER ? (long)N : (long)n);
+    x = batch_ptr(x, UPPER ? (long)n : (long)N);
     const int row = blockIdx.x;
     const int lo = UPPER ? row : 0, hi = UPPER ? n - 1 : (row < n ? row : n - 1);
     // four independent partial sums keep four row loads in flight per lane
@@ -1208,17 +1212,18 @@ __global__ __launch_bounds__(64) void k_trmv(const double *__restrict__ M, const
 // The solve is FP64 for both vector types: the right-hand side is widened,
 // the result rounded once on output.
 template <class T>
-void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl)
+void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl, int nb)
 {
     const int N = (C.n + NB - 1) / NB * NB;
     coarse_finish(C);  // also a lagged factorisation's, which runs meanwhile
     if (C.solve_waits) HIPCHK(hipStreamWaitEvent(s, C.ev_done, 0));  // the factorisation of this hierarchy
-    k_trmv<false, T, double><<<C.n, 64, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);
-    k_trmv<true, double, T><<<C.n, 64, 0, s>>>(C.LinvT, C.y, x, N, C.n, mask, ctl);
+    // C.y has room for kMaxBatch right-hand sides
+    k_trmv<false, T, double><<<dim3(C.n, 1, batch_dim(nb)), 64, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);
+    k_trmv<true, double, T><<<dim3(C.n, 1, batch_dim(nb)), 64, 0, s>>>(C.LinvT, C.y, x, N, C.n, mask, ctl);
 }
 template void launch_coarse_solve<double>(hipStream_t, const Coarse &, const double *, double *, const uint8_t *,
-                                          const int *);
+                                          const int *, int);
 template void launch_coarse_solve<float>(hipStream_t, const Coarse &, const float *, float *, const uint8_t *,
-                                         const int *);
+                                         const int *, int);
 
 }  // namespace gputo

@@ -74,6 +74,7 @@ struct OpArgsT {
     T *out2;               // second result (prerelax+defect writes U and R)
     double *partials;      // per-block partial sums (fused dot), may be null
     const int *ctl;        // ctl[0] != 0: skip (converged)
+    int nb;                // right-hand sides of a batched launch (0 or 1: one)
     double omega, prerelax;
     double cx_floor;       // > 0: element scales below it count as 0 (mgcg9 smoother defect)
     int kc;                // Ke0 has the unit-H8 class structure: use kv
@@ -98,6 +99,36 @@ struct CgScalars {
     int noru;        // 0: replace the recursive residual by b - A x this iteration
     int nru;         // replacements done
 };
+
+// Batched launches: several right-hand sides on one hierarchy, one per
+// blockIdx.z.  The vectors of a batch follow each other with the level's DOF
+// count (or the vector length n) as stride, each right-hand side has its own
+// CgScalars, and a producer of per-block partial sums with G blocks writes
+// right-hand side z's at partials + z G.  The convergence flag of right-hand
+// side z is ctl[z * kCtlStride] (ctl points at CgScalars::done).  A launch
+// with one right-hand side has blockIdx.z = 0 and is unchanged.
+static_assert(sizeof(CgScalars) % sizeof(int) == 0, "CgScalars stride in ints");
+constexpr int kCtlStride = (int)(sizeof(CgScalars) / sizeof(int));
+constexpr int kMaxBatch = 6;  // the six load cases of a homogenisation
+
+template <class P>
+__device__ __forceinline__ P *batch_ptr(P *p, long stride)
+{
+    return p ? p + (long)blockIdx.z * stride : p;
+}
+__device__ __forceinline__ const int *batch_ctl(const int *ctl) { return batch_ptr(ctl, kCtlStride); }
+template <class T>
+__device__ __forceinline__ void batch_shift(OpArgsT<T> &a)
+{
+    const long st = a.g.ndof;
+    a.U = batch_ptr(a.U, st);
+    a.F = batch_ptr(a.F, st);
+    a.out = batch_ptr(a.out, st);
+    a.out2 = batch_ptr(a.out2, st);
+    a.partials = batch_ptr(a.partials, (long)gridDim.x);
+    a.ctl = batch_ctl(a.ctl);
+}
+inline unsigned batch_dim(int nb) { return nb > 1 ? (unsigned)nb : 1u; }
 
 struct Level {
     Grid g;
@@ -142,6 +173,21 @@ struct Coarse {
 struct MixedState;
 struct LearnState;
 
+// Buffers of a batched PCG (several right-hand sides on one hierarchy):
+// every vector holds nb vectors one after another, every level of the
+// V-cycle has its own batched work vectors (U, F, R, Tmp; FP64 for an FP64
+// solve, FP32 for the mixed one).
+struct BatchState {
+    int nb = 0;
+    int precision = -1;
+    double *R = nullptr, *P = nullptr, *Q = nullptr, *Z = nullptr;
+    double *partials = nullptr;  // nb x (the most per-block partials of a launch)
+    CgScalars *sc = nullptr, *sc_host = nullptr;
+    double *v64[kMaxLevels][4] = {};
+    float *v32[kMaxLevels][4] = {};
+    float *r32 = nullptr, *z32 = nullptr;  // the FP32 V-cycle's right-hand side and result
+};
+
 }  // namespace gputo
 
 struct gputo_handle {
@@ -175,6 +221,7 @@ struct gputo_handle {
     int gt_preset = -1;          // preset the tables were built for (-1: none yet)
     std::vector<double> gt_ke;  // Ke they were built from (Galerkin level 1)
     double *hom = nullptr;      // periodic homogenisation: 6 load vectors and 6 fluctuations
+    gputo::BatchState *bt = nullptr;  // batched PCG buffers (allocated on first use)
 };
 
 namespace gputo {
@@ -188,19 +235,23 @@ void launch_diagA(hipStream_t s, const Grid &g, const double *A, double *AD);
 void launch_stencil_build(hipStream_t s, const Grid &g, const double *A, double *S);
 constexpr int kStencilEntries = 243;  // 27 neighbours x 3 x 3 per node
 template <class T>
-void launch_restrict(hipStream_t s, const Grid &gc, const T *Rf, T *Fc, const uint8_t *maskc, const int *ctl);
+void launch_restrict(hipStream_t s, const Grid &gc, const T *Rf, T *Fc, const uint8_t *maskc, const int *ctl,
+                     int nb = 1);
 template <class T>
 void launch_interp(hipStream_t s, const Grid &gf, const T *Uc, T *Uf, const uint8_t *maskf, int zero_fixed,
-                   const int *ctl);
+                   const int *ctl, int nb = 1);
 template <class T>  // Uout = Uin + P Uc (Uout may equal Uin)
 void launch_interp_to(hipStream_t s, const Grid &gf, const T *Uc, const T *Uin, T *Uout, const uint8_t *maskf,
-                      int zero_fixed, const int *ctl);
+                      int zero_fixed, const int *ctl, int nb = 1);
 template <class T> void launch_fill(hipStream_t s, T *v, long n, double val);
 template <class Ta, class Tb>
-void launch_dot_partial(hipStream_t s, const Ta *a, const Tb *b, long n, double *partials, const int *ctl);
-template <class Ti, class To> void launch_copy(hipStream_t s, const Ti *src, To *dst, long n, const int *ctl);
+void launch_dot_partial(hipStream_t s, const Ta *a, const Tb *b, long n, double *partials, const int *ctl,
+                        int nb = 1);
+template <class Ti, class To>
+void launch_copy(hipStream_t s, const Ti *src, To *dst, long n, const int *ctl, int nb = 1);
 template <class T>  // U = prerelax F / AD
-void launch_prerelax(hipStream_t s, const T *F, const T *AD, T *U, long n, double prerelax, const int *ctl);
+void launch_prerelax(hipStream_t s, const T *F, const T *AD, T *U, long n, double prerelax, const int *ctl,
+                     int nb = 1);
 void launch_coarse_l1(hipStream_t s, const Grid &gc, const double *Aelm, const double *CXf, double *A);
 void launch_coarse_galerkin(hipStream_t s, const Grid &gc, const double *Af, const double *Ke, const double *CXf,
                             double *Ac);
@@ -237,26 +288,28 @@ void coarse_invalidate(Coarse &C);
 // use it from now on (no-op without a lagged factorisation)
 void coarse_switch(Coarse &C, hipStream_t s);
 template <class T>
-void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl);
+void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const uint8_t *mask, const int *ctl,
+                         int nb = 1);
 
 // PCG scalar steps (single-block kernels reading the reduction partials).
-void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
-void launch_cg_rho0(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
-void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int nparts);
+void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb = 1);
+void launch_cg_rho0(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb = 1);
+void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb = 1);
 // U += alpha P, R -= alpha Q, partials of R.R; R32 (if not null) gets R in FP32
 template <class T>
 void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials,
-                         float *R32 = nullptr);
-void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit);
-void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit);
-template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc);
+                         float *R32 = nullptr, int nb = 1);
+void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit,
+                     int nb = 1);
+void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit, int nb = 1);
+template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc, int nb = 1);
 // P = Z + beta P with the FP32 output of the mixed V-cycle (widened exactly)
-void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc);
+void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc, int nb = 1);
 // flexible beta (partials needs 2*kRedBlocks entries); z FP64, or the
 // mixed V-cycle's FP32 output (products in FP64, the same values)
 template <class TZ>
 void launch_cg_beta_flex(hipStream_t s, const double *r, const TZ *z, const double *q, long n, CgScalars *sc,
-                         double *partials, int maxit);
+                         double *partials, int maxit, int nb = 1);
 
 void check_hip(hipError_t e, const char *what);  // throws on an error
 #define HIPCHK(x) ::gputo::check_hip((x), #x)
@@ -293,14 +346,14 @@ void to_release(gputo_handle *h);  // frees the design-loop state (driver.hip)
 // and the rest on its images: summing the images into the master (fold)
 // completes the periodic operator.
 // fold: master += images; images = bcast ? master : 0
-template <class T> void launch_per_fold(hipStream_t s, const Grid &g, T *v, int bcast, const int *ctl);
+template <class T> void launch_per_fold(hipStream_t s, const Grid &g, T *v, int bcast, const int *ctl, int nb = 1);
 // images = master
-template <class T> void launch_per_bcast(hipStream_t s, const Grid &g, T *v, const int *ctl);
+template <class T> void launch_per_bcast(hipStream_t s, const Grid &g, T *v, const int *ctl, int nb = 1);
 // pre: U = prerelax F/AD (0 on fixed DOFs); else U += omega fold(R)/AD;
 // the new master value is written to the images too
 template <class T>
 void launch_per_jacobi(hipStream_t s, const Grid &g, T *U, const T *R, const T *F, const T *AD, const uint8_t *mask,
-                       double omega, double prerelax, int pre, const int *ctl);
+                       double omega, double prerelax, int pre, const int *ctl, int nb = 1);
 // dense coarsest matrix of the periodic operator in the full numbering:
 // master rows and columns hold the wrapped assembly, image rows are unit
 void launch_assemble_per(hipStream_t s, const Grid &g, const double *A, double *AQ);
@@ -313,5 +366,8 @@ void launch_homog_rhs(hipStream_t s, const Grid &g, const double *CX, const doub
 
 int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0);
 int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit);
+// nb <= kMaxBatch right-hand sides together (F, U: nb vectors one after another)
+int run_pcg_batch(gputo_handle *h, const double *F, double *U, int nb, double tol, int maxit, double *relres,
+                  int *nit);
 
 }  // namespace gputo

@@ -40,6 +40,7 @@ __host__ __device__ constexpr int pidx(int m, int n) { return n <= m ? m * (m + 
 template <class T, int MODE, bool KC>
 __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
 {
+    batch_shift(a);
     if (a.ctl && a.ctl[0]) return;
     __shared__ T su[3][kMaxHalo];
     __shared__ T sE[kMaxElemBox];
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
 template <class T>
 void launch_op0(hipStream_t s, int mode, const OpArgsT<T> &a)
 {
-    const dim3 grid((unsigned)xcd_launch(a.t.blocks())), block(kBlock);
+    const dim3 grid((unsigned)xcd_launch(a.t.blocks()), 1, batch_dim(a.nb)), block(kBlock);
     if (a.kc) {
         switch (mode) {
         case OP_MATVEC: k_gather<T, OP_MATVEC, true><<<grid, block, 0, s>>>(a); break;
@@ -326,6 +327,7 @@ __device__ __forceinline__ T stored_u(const OpArgsT<T> &a, long dof)
 template <class T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_stored(OpArgsT<T> a)
 {
+    batch_shift(a);
     if (a.ctl && a.ctl[0]) return;
     __shared__ T red[8][3][kStNodes];
     const Grid g = a.g;
@@ -473,6 +475,7 @@ void launch_stencil_build(hipStream_t s, const Grid &g, const double *A, double 
 template <class T, int MODE>
 __global__ __launch_bounds__(192) void k_stencil(OpArgsT<T> a)
 {
+    batch_shift(a);
     if (a.ctl && a.ctl[0]) return;
     __shared__ T red[3][3][64];
     const Grid g = a.g;
@@ -551,7 +554,7 @@ void launch_opA(hipStream_t s, int mode, const OpArgsT<T> &a)
 {
     if (a.S && !a.partials) {
         // the stencil form (its launches carry no fused dot)
-        const dim3 sg((unsigned)((a.g.nnod + 63) / 64)), sb(192);
+        const dim3 sg((unsigned)((a.g.nnod + 63) / 64), 1, batch_dim(a.nb)), sb(192);
         switch (mode) {
         case OP_MATVEC: k_stencil<T, OP_MATVEC><<<sg, sb, 0, s>>>(a); return;
         case OP_DEFECT: k_stencil<T, OP_DEFECT><<<sg, sb, 0, s>>>(a); return;
@@ -561,7 +564,7 @@ void launch_opA(hipStream_t s, int mode, const OpArgsT<T> &a)
         default: return;
         }
     }
-    const dim3 grid((unsigned)a.t.blocks()), block(kBlock);
+    const dim3 grid((unsigned)a.t.blocks(), 1, batch_dim(a.nb)), block(kBlock);
     switch (mode) {
     case OP_MATVEC: k_stored<T, OP_MATVEC><<<grid, block, 0, s>>>(a); break;
     case OP_DEFECT: k_stored<T, OP_DEFECT><<<grid, block, 0, s>>>(a); break;
@@ -630,6 +633,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const T *__restrict__ Rf, T *__restrict__ Fc,
                                                     const uint8_t *__restrict__ maskc, const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
     const long nod = (long)blockIdx.x * kBlock + threadIdx.x;
     if (nod >= gc.nnod) return;
@@ -637,6 +641,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const T *__restric
     const int K = (int)((nod / gc.nny) % gc.nnz);
     const int I = (int)(nod / ((long)gc.nny * gc.nnz));
     const int fny = 2 * gc.ny + 1, fnz = 2 * gc.nz + 1, fnx = 2 * gc.nx + 1;
+    Rf = batch_ptr(Rf, 3L * fnx * fny * fnz);
+    Fc = batch_ptr(Fc, gc.ndof);
     T s0 = T(0), s1 = T(0), s2 = T(0);
     // one fine plane (9 nodes) per pass, its loads issued together; nodes
     // outside the mesh read node 0 with weight 0 (no branches around loads)
@@ -671,12 +677,15 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const T *__restric
 }
 
 template <class T>
-void launch_restrict(hipStream_t s, const Grid &gc, const T *Rf, T *Fc, const uint8_t *maskc, const int *ctl)
+void launch_restrict(hipStream_t s, const Grid &gc, const T *Rf, T *Fc, const uint8_t *maskc, const int *ctl, int nb)
 {
-    k_restrict<T><<<(unsigned)((gc.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gc, Rf, Fc, maskc, ctl);
+    k_restrict<T><<<dim3((unsigned)((gc.nnod + kBlock - 1) / kBlock), 1, batch_dim(nb)), kBlock, 0, s>>>(gc, Rf, Fc,
+                                                                                                       maskc, ctl);
 }
-template void launch_restrict<double>(hipStream_t, const Grid &, const double *, double *, const uint8_t *, const int *);
-template void launch_restrict<float>(hipStream_t, const Grid &, const float *, float *, const uint8_t *, const int *);
+template void launch_restrict<double>(hipStream_t, const Grid &, const double *, double *, const uint8_t *, const int *,
+                                      int);
+template void launch_restrict<float>(hipStream_t, const Grid &, const float *, float *, const uint8_t *, const int *,
+                                     int);
 
 // ---------------------------------------------------------------------------
 // Prolongation U_f += P U_c (interpc, mgcg1.cu:1587-1641) followed by
@@ -689,6 +698,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict__ Uc, const T *Uin, T *Uout,
                                                   const uint8_t *__restrict__ maskf, int zero_fixed, const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
     const long nod = (long)blockIdx.x * kBlock + threadIdx.x;
     if (nod >= gf.nnod) return;
@@ -696,6 +706,9 @@ __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict_
     const int k = (int)((nod / gf.nny) % gf.nnz);
     const int i = (int)(nod / ((long)gf.nny * gf.nnz));
     const int cny = gf.ny / 2 + 1, cnz = gf.nz / 2 + 1;
+    Uc = batch_ptr(Uc, 3L * (gf.nx / 2 + 1) * cny * cnz);
+    Uin = batch_ptr(Uin, gf.ndof);
+    Uout = batch_ptr(Uout, gf.ndof);
     const int ia = i >> 1, ib = (i + 1) >> 1, ja = j >> 1, jb = (j + 1) >> 1, ka = k >> 1, kb = (k + 1) >> 1;
     const T w = T(1) / (T)(1 << ((i & 1) + (j & 1) + (k & 1)));
     // corners (ia|ib, ja|jb, ka|kb), all 8 loaded together; for an even
@@ -734,26 +747,26 @@ __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict_
 
 template <class T>
 void launch_interp_to(hipStream_t s, const Grid &gf, const T *Uc, const T *Uin, T *Uout, const uint8_t *maskf,
-                      int zero_fixed, const int *ctl)
+                      int zero_fixed, const int *ctl, int nb)
 {
-    k_interp<T><<<(unsigned)((gf.nnod + kBlock - 1) / kBlock), kBlock, 0, s>>>(gf, Uc, Uin, Uout, maskf, zero_fixed,
-                                                                               ctl);
+    k_interp<T><<<dim3((unsigned)((gf.nnod + kBlock - 1) / kBlock), 1, batch_dim(nb)), kBlock, 0, s>>>(
+        gf, Uc, Uin, Uout, maskf, zero_fixed, ctl);
 }
 template void launch_interp_to<double>(hipStream_t, const Grid &, const double *, const double *, double *,
-                                       const uint8_t *, int, const int *);
+                                       const uint8_t *, int, const int *, int);
 template void launch_interp_to<float>(hipStream_t, const Grid &, const float *, const float *, float *,
-                                      const uint8_t *, int, const int *);
+                                      const uint8_t *, int, const int *, int);
 
 template <class T>
 void launch_interp(hipStream_t s, const Grid &gf, const T *Uc, T *Uf, const uint8_t *maskf, int zero_fixed,
-                   const int *ctl)
+                   const int *ctl, int nb)
 {
-    launch_interp_to<T>(s, gf, Uc, Uf, Uf, maskf, zero_fixed, ctl);
+    launch_interp_to<T>(s, gf, Uc, Uf, Uf, maskf, zero_fixed, ctl, nb);
 }
 template void launch_interp<double>(hipStream_t, const Grid &, const double *, double *, const uint8_t *, int,
-                                    const int *);
+                                    const int *, int);
 template void launch_interp<float>(hipStream_t, const Grid &, const float *, float *, const uint8_t *, int,
-                                   const int *);
+                                   const int *, int);
 
 // ---------------------------------------------------------------------------
 // BLAS-1 and reductions.  Dot products are two-stage with a fixed grid of
@@ -786,15 +799,18 @@ template void launch_fill<float>(hipStream_t, float *, long, double);
 template <class Ti, class To>
 __global__ void k_copy(const Ti *__restrict__ src, To *__restrict__ dst, long n, const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
+    src = batch_ptr(src, n);
+    dst = batch_ptr(dst, n);
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         dst[i] = (To)src[i];
 }
 
 template <class Ti, class To>
-void launch_copy(hipStream_t s, const Ti *src, To *dst, long n, const int *ctl)
+void launch_copy(hipStream_t s, const Ti *src, To *dst, long n, const int *ctl, int nb)
 {
-    k_copy<Ti, To><<<grid_for(n), kBlock, 0, s>>>(src, dst, n, ctl);
+    k_copy<Ti, To><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(src, dst, n, ctl);
 }
 // U = prerelax F / AD (preRelc, mgcg1.cu:1190-1197), the same expression the
 // fused PRE modes form on the fly
@@ -802,30 +818,38 @@ template <class T>
 __global__ void k_prerelax(const T *__restrict__ F, const T *__restrict__ AD, T *__restrict__ U, long n, T pr,
                            const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
+    F = batch_ptr(F, n);
+    U = batch_ptr(U, n);
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         U[i] = pr * F[i] / AD[i];
 }
 
 template <class T>
-void launch_prerelax(hipStream_t s, const T *F, const T *AD, T *U, long n, double prerelax, const int *ctl)
+void launch_prerelax(hipStream_t s, const T *F, const T *AD, T *U, long n, double prerelax, const int *ctl, int nb)
 {
-    k_prerelax<T><<<grid_for(n), kBlock, 0, s>>>(F, AD, U, n, T(prerelax), ctl);
+    k_prerelax<T><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(F, AD, U, n, T(prerelax), ctl);
 }
 template void launch_prerelax<double>(hipStream_t, const double *, const double *, double *, long, double,
-                                      const int *);
-template void launch_prerelax<float>(hipStream_t, const float *, const float *, float *, long, double, const int *);
+                                      const int *, int);
+template void launch_prerelax<float>(hipStream_t, const float *, const float *, float *, long, double, const int *,
+                                     int);
 
-template void launch_copy<double, double>(hipStream_t, const double *, double *, long, const int *);
-template void launch_copy<float, float>(hipStream_t, const float *, float *, long, const int *);
-template void launch_copy<double, float>(hipStream_t, const double *, float *, long, const int *);
-template void launch_copy<float, double>(hipStream_t, const float *, double *, long, const int *);
+template void launch_copy<double, double>(hipStream_t, const double *, double *, long, const int *, int);
+template void launch_copy<float, float>(hipStream_t, const float *, float *, long, const int *, int);
+template void launch_copy<double, float>(hipStream_t, const double *, float *, long, const int *, int);
+template void launch_copy<float, double>(hipStream_t, const float *, double *, long, const int *, int);
 
 template <class Ta, class Tb>
 __global__ __launch_bounds__(kBlock) void k_dot_partial(const Ta *__restrict__ a, const Tb *__restrict__ b, long n,
                                                        double *partials, const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
+    a = batch_ptr(a, n);
+    b = batch_ptr(b, n);
+    partials = batch_ptr(partials, (long)gridDim.x);
     double s = 0.0;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock)
         s += (double)a[i] * (double)b[i];
@@ -834,15 +858,16 @@ __global__ __launch_bounds__(kBlock) void k_dot_partial(const Ta *__restrict__ a
 }
 
 template <class Ta, class Tb>
-void launch_dot_partial(hipStream_t s, const Ta *a, const Tb *b, long n, double *partials, const int *ctl)
+void launch_dot_partial(hipStream_t s, const Ta *a, const Tb *b, long n, double *partials, const int *ctl, int nb)
 {
-    k_dot_partial<Ta, Tb><<<kRedBlocks, kBlock, 0, s>>>(a, b, n, partials, ctl);
+    k_dot_partial<Ta, Tb><<<dim3(kRedBlocks, 1, batch_dim(nb)), kBlock, 0, s>>>(a, b, n, partials, ctl);
 }
 template void launch_dot_partial<double, double>(hipStream_t, const double *, const double *, long, double *,
-                                                 const int *);
-template void launch_dot_partial<float, float>(hipStream_t, const float *, const float *, long, double *, const int *);
+                                                 const int *, int);
+template void launch_dot_partial<float, float>(hipStream_t, const float *, const float *, long, double *, const int *,
+                                               int);
 template void launch_dot_partial<double, float>(hipStream_t, const double *, const float *, long, double *,
-                                                const int *);
+                                                const int *, int);
 
 __device__ double sum_partials(const double *p, int n)
 {
@@ -855,6 +880,8 @@ __device__ double sum_partials(const double *p, int n)
 // device so the loop never waits on the host).
 __global__ __launch_bounds__(kBlock) void k_cg_init(CgScalars *sc, const double *p, int n)
 {
+    sc += blockIdx.z;
+    p += (long)blockIdx.z * n;
     const double ff = sum_partials(p, n);
     if (threadIdx.x == 0) {
         sc->res0 = sqrt(ff);
@@ -874,6 +901,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(CgScalars *sc, const double 
 
 __global__ __launch_bounds__(kBlock) void k_cg_rho0(CgScalars *sc, const double *p, int n)
 {
+    sc += blockIdx.z;
+    p += (long)blockIdx.z * n;
     if (sc->done) return;
     const double rz = sum_partials(p, n);
     if (threadIdx.x == 0) {
@@ -884,6 +913,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_rho0(CgScalars *sc, const double 
 
 __global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars *sc, const double *p, int n)
 {
+    sc += blockIdx.z;
+    p += (long)blockIdx.z * n;
     if (sc->done) return;
     const double pq = sum_partials(p, n);
     if (threadIdx.x == 0) {
@@ -902,7 +933,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *_
                                                         const T *__restrict__ Q, long n, const CgScalars *sc,
                                                         double *partials, float *__restrict__ R32)
 {
+    sc += blockIdx.z;
     if (sc->done) return;
+    U = batch_ptr(U, n);
+    R = batch_ptr(R, n);
+    P = batch_ptr(P, n);
+    Q = batch_ptr(Q, n);
+    R32 = batch_ptr(R32, n);
+    partials = batch_ptr(partials, (long)gridDim.x);
     const T al = (T)sc->alpha;
     double s = 0.0;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) {
@@ -918,6 +956,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *_
 
 __global__ __launch_bounds__(kBlock) void k_cg_check(CgScalars *sc, const double *p, int n, double tol, int maxit)
 {
+    sc += blockIdx.z;
+    p += (long)blockIdx.z * n;
     if (sc->done) return;
     const double rr = sum_partials(p, n);
     if (threadIdx.x == 0) {
@@ -936,6 +976,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_check(CgScalars *sc, const double
 
 __global__ __launch_bounds__(kBlock) void k_cg_beta(CgScalars *sc, const double *p, int n, int maxit)
 {
+    sc += blockIdx.z;
+    p += (long)blockIdx.z * n;
     if (sc->done) return;
     const double rz = sum_partials(p, n);
     if (threadIdx.x == 0) {
@@ -958,7 +1000,12 @@ __global__ __launch_bounds__(kBlock) void k_dot2_partial(const double *__restric
                                                        const double *__restrict__ q, long n, double *partials,
                                                        const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
+    r = batch_ptr(r, n);
+    z = batch_ptr(z, n);
+    q = batch_ptr(q, n);
+    partials = batch_ptr(partials, 2L * kRedBlocks);
     double a = 0.0, b = 0.0;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) {
         const double zi = (double)z[i];  // exact for an FP32 z
@@ -975,6 +1022,8 @@ __global__ __launch_bounds__(kBlock) void k_dot2_partial(const double *__restric
 
 __global__ __launch_bounds__(kBlock) void k_cg_beta_flex(CgScalars *sc, const double *p, int maxit)
 {
+    sc += blockIdx.z;
+    p += 2L * kRedBlocks * blockIdx.z;
     if (sc->done) return;
     const double rz = sum_partials(p, kRedBlocks);
     const double qz = sum_partials(p + kRedBlocks, kRedBlocks);
@@ -992,66 +1041,69 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta_flex(CgScalars *sc, const do
 
 template <class TZ>
 void launch_cg_beta_flex(hipStream_t s, const double *r, const TZ *z, const double *q, long n, CgScalars *sc,
-                         double *partials, int maxit)
+                         double *partials, int maxit, int nb)
 {
-    k_dot2_partial<TZ><<<kRedBlocks, kBlock, 0, s>>>(r, z, q, n, partials, &sc->done);
-    k_cg_beta_flex<<<1, kBlock, 0, s>>>(sc, partials, maxit);
+    k_dot2_partial<TZ><<<dim3(kRedBlocks, 1, batch_dim(nb)), kBlock, 0, s>>>(r, z, q, n, partials, &sc->done);
+    k_cg_beta_flex<<<dim3(1, 1, batch_dim(nb)), kBlock, 0, s>>>(sc, partials, maxit);
 }
 template void launch_cg_beta_flex<double>(hipStream_t, const double *, const double *, const double *, long,
-                                          CgScalars *, double *, int);
+                                          CgScalars *, double *, int, int);
 template void launch_cg_beta_flex<float>(hipStream_t, const double *, const float *, const double *, long,
-                                         CgScalars *, double *, int);
+                                         CgScalars *, double *, int, int);
 
 template <class T, class TZ>
 __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ P, const TZ *__restrict__ Z, long n,
                                                        const CgScalars *sc)
 {
+    sc += blockIdx.z;
     if (sc->done) return;
+    P = batch_ptr(P, n);
+    Z = batch_ptr(Z, n);
     const T be = (T)sc->beta;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
         P[i] = (T)Z[i] + be * P[i];
 }
 
-void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts)
+void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb)
 {
-    k_cg_init<<<1, kBlock, 0, s>>>(sc, partials, nparts);
+    k_cg_init<<<dim3(1, 1, batch_dim(nb)), kBlock, 0, s>>>(sc, partials, nparts);
 }
-void launch_cg_rho0(hipStream_t s, CgScalars *sc, const double *partials, int nparts)
+void launch_cg_rho0(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb)
 {
-    k_cg_rho0<<<1, kBlock, 0, s>>>(sc, partials, nparts);
+    k_cg_rho0<<<dim3(1, 1, batch_dim(nb)), kBlock, 0, s>>>(sc, partials, nparts);
 }
-void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int nparts)
+void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb)
 {
-    k_cg_alpha<<<1, kBlock, 0, s>>>(sc, partials, nparts);
+    k_cg_alpha<<<dim3(1, 1, batch_dim(nb)), kBlock, 0, s>>>(sc, partials, nparts);
 }
 template <class T>
 void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials,
-                         float *R32)
+                         float *R32, int nb)
 {
-    k_cg_update_ur<T><<<kRedBlocks, kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
+    k_cg_update_ur<T><<<dim3(kRedBlocks, 1, batch_dim(nb)), kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
 }
 template void launch_cg_update_ur<double>(hipStream_t, double *, double *, const double *, const double *, long,
-                                          CgScalars *, double *, float *);
+                                          CgScalars *, double *, float *, int);
 template void launch_cg_update_ur<float>(hipStream_t, float *, float *, const float *, const float *, long,
-                                         CgScalars *, double *, float *);
-void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit)
+                                         CgScalars *, double *, float *, int);
+void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit, int nb)
 {
-    k_cg_check<<<1, kBlock, 0, s>>>(sc, partials, nparts, tol, maxit);
+    k_cg_check<<<dim3(1, 1, batch_dim(nb)), kBlock, 0, s>>>(sc, partials, nparts, tol, maxit);
 }
-void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit)
+void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit, int nb)
 {
-    k_cg_beta<<<1, kBlock, 0, s>>>(sc, partials, nparts, maxit);
+    k_cg_beta<<<dim3(1, 1, batch_dim(nb)), kBlock, 0, s>>>(sc, partials, nparts, maxit);
 }
 template <class T>
-void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc)
+void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc, int nb)
 {
-    k_cg_update_p<T, T><<<grid_for(n), kBlock, 0, s>>>(P, Z, n, sc);
+    k_cg_update_p<T, T><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
 }
-template void launch_cg_update_p<double>(hipStream_t, double *, const double *, long, CgScalars *);
-template void launch_cg_update_p<float>(hipStream_t, float *, const float *, long, CgScalars *);
-void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc)
+template void launch_cg_update_p<double>(hipStream_t, double *, const double *, long, CgScalars *, int);
+template void launch_cg_update_p<float>(hipStream_t, float *, const float *, long, CgScalars *, int);
+void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc, int nb)
 {
-    k_cg_update_p<double, float><<<grid_for(n), kBlock, 0, s>>>(P, Z, n, sc);
+    k_cg_update_p<double, float><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
 }
 
 }  // namespace gputo

@@ -40,6 +40,7 @@ template <class T, int MODE>
 __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<T> a, Tile t, const T *__restrict__ CXf, T lam2mu,
                                                  T lam, T mu)
 {
+    batch_shift(a);
     if (a.ctl && a.ctl[0]) return;
     __shared__ T su[3 * L1Ew<T>::HALO];
     __shared__ T sy[L1Ew<T>::NE * kL1Ys];
@@ -269,7 +270,7 @@ void launch_op1_mf(hipStream_t s, int mode, const OpArgsT<T> &a, const T *CXf, c
     // Lame constants of D (E = 1): D11 = lam + 2mu, D12 = lam, D44 = mu
     const double c1 = 1.0 / ((1 + nu) * (1 - 2 * nu));
     const T l2m = T(c1 * (1 - nu)), lam = T(c1 * nu), mu = T(c1 * (1 - 2 * nu) / 2);
-    const dim3 grid((unsigned)t.blocks()), block(256);
+    const dim3 grid((unsigned)t.blocks(), 1, batch_dim(a.nb)), block(256);
     switch (mode) {
     case OP_MATVEC: k_l1_ew<T, OP_MATVEC><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;
     case OP_DEFECT: k_l1_ew<T, OP_DEFECT><<<grid, block, 0, s>>>(a, t, CXf, l2m, lam, mu); break;

@@ -548,6 +548,7 @@ static void launch_flavor(hipStream_t s, int mode, const OpArgsT<float> &a, cons
 void launch_op0_mfma(hipStream_t s, int mode, const OpArgsT<float> &a, const void *ctx, int flavor)
 {
     if (a.partials) throw std::runtime_error("MFMA level-0 operator: no fused dot product");
+    if (a.nb > 1) throw std::runtime_error("MFMA level-0 operator: one right-hand side per launch");
     const MixedState *m = static_cast<const MixedState *>(ctx);
     const MfWork &w = m->mf->w;
     if (flavor == MF_F32)

@@ -68,7 +68,9 @@ __device__ __forceinline__ int images_of(const Grid &g, int i, int j, int k, lon
 template <class T>
 __global__ __launch_bounds__(kPerBlock) void k_per_fold(Grid g, T *__restrict__ v, int bcast, const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
+    v = batch_ptr(v, g.ndof);
     const long t = (long)blockIdx.x * kPerBlock + threadIdx.x;
     if (t >= face_masters(g)) return;
     int i, j, k;
@@ -87,7 +89,9 @@ __global__ __launch_bounds__(kPerBlock) void k_per_fold(Grid g, T *__restrict__ 
 template <class T>
 __global__ __launch_bounds__(kPerBlock) void k_per_bcast(Grid g, T *__restrict__ v, const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
+    v = batch_ptr(v, g.ndof);
     const long t = (long)blockIdx.x * kPerBlock + threadIdx.x;
     if (t >= face_masters(g)) return;
     int i, j, k;
@@ -110,7 +114,11 @@ __global__ __launch_bounds__(kPerBlock) void k_per_jacobi(Grid g, T *__restrict_
                                                           const uint8_t *__restrict__ mask, T omega, T prerelax,
                                                           int pre, const int *ctl)
 {
+    ctl = batch_ctl(ctl);
     if (ctl && ctl[0]) return;
+    U = batch_ptr(U, g.ndof);
+    R = batch_ptr(R, g.ndof);
+    F = batch_ptr(F, g.ndof);
     const long t = (long)blockIdx.x * kPerBlock + threadIdx.x;
     if (t >= (long)g.nx * g.ny * g.nz) return;
     const int j = (int)(t % g.ny);
@@ -208,35 +216,36 @@ __global__ __launch_bounds__(kPerBlock) void k_homog_rhs(Grid g, const double *_
 }  // namespace
 
 template <class T>
-void launch_per_fold(hipStream_t s, const Grid &g, T *v, int bcast, const int *ctl)
+void launch_per_fold(hipStream_t s, const Grid &g, T *v, int bcast, const int *ctl, int nb)
 {
     const long n = face_masters(g);
-    k_per_fold<T><<<(unsigned)((n + kPerBlock - 1) / kPerBlock), kPerBlock, 0, s>>>(g, v, bcast, ctl);
+    k_per_fold<T><<<dim3((unsigned)((n + kPerBlock - 1) / kPerBlock), 1, batch_dim(nb)), kPerBlock, 0, s>>>(g, v, bcast,
+                                                                                                          ctl);
 }
-template void launch_per_fold<double>(hipStream_t, const Grid &, double *, int, const int *);
-template void launch_per_fold<float>(hipStream_t, const Grid &, float *, int, const int *);
+template void launch_per_fold<double>(hipStream_t, const Grid &, double *, int, const int *, int);
+template void launch_per_fold<float>(hipStream_t, const Grid &, float *, int, const int *, int);
 
 template <class T>
-void launch_per_bcast(hipStream_t s, const Grid &g, T *v, const int *ctl)
+void launch_per_bcast(hipStream_t s, const Grid &g, T *v, const int *ctl, int nb)
 {
     const long n = face_masters(g);
-    k_per_bcast<T><<<(unsigned)((n + kPerBlock - 1) / kPerBlock), kPerBlock, 0, s>>>(g, v, ctl);
+    k_per_bcast<T><<<dim3((unsigned)((n + kPerBlock - 1) / kPerBlock), 1, batch_dim(nb)), kPerBlock, 0, s>>>(g, v, ctl);
 }
-template void launch_per_bcast<double>(hipStream_t, const Grid &, double *, const int *);
-template void launch_per_bcast<float>(hipStream_t, const Grid &, float *, const int *);
+template void launch_per_bcast<double>(hipStream_t, const Grid &, double *, const int *, int);
+template void launch_per_bcast<float>(hipStream_t, const Grid &, float *, const int *, int);
 
 template <class T>
 void launch_per_jacobi(hipStream_t s, const Grid &g, T *U, const T *R, const T *F, const T *AD, const uint8_t *mask,
-                       double omega, double prerelax, int pre, const int *ctl)
+                       double omega, double prerelax, int pre, const int *ctl, int nb)
 {
     const long n = (long)g.nx * g.ny * g.nz;
-    k_per_jacobi<T><<<(unsigned)((n + kPerBlock - 1) / kPerBlock), kPerBlock, 0, s>>>(g, U, R, F, AD, mask, (T)omega,
-                                                                                    (T)prerelax, pre, ctl);
+    k_per_jacobi<T><<<dim3((unsigned)((n + kPerBlock - 1) / kPerBlock), 1, batch_dim(nb)), kPerBlock, 0, s>>>(
+        g, U, R, F, AD, mask, (T)omega, (T)prerelax, pre, ctl);
 }
 template void launch_per_jacobi<double>(hipStream_t, const Grid &, double *, const double *, const double *,
-                                        const double *, const uint8_t *, double, double, int, const int *);
+                                        const double *, const uint8_t *, double, double, int, const int *, int);
 template void launch_per_jacobi<float>(hipStream_t, const Grid &, float *, const float *, const float *,
-                                       const float *, const uint8_t *, double, double, int, const int *);
+                                       const float *, const uint8_t *, double, double, int, const int *, int);
 
 void launch_assemble_per(hipStream_t s, const Grid &g, const double *A, double *AQ)
 {

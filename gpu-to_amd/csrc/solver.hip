@@ -542,6 +542,169 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     return 0;
 }
 
+// Buffers of a batched PCG with nb right-hand sides for the handle's
+// precision (kept for later calls with the same nb and precision).
+static BatchState &batch_state(gputo_handle *h, int nb)
+{
+    BatchState *&b = h->bt;
+    const int prec = h->prm.precision;
+    if (b && b->nb == nb && b->precision == prec) return *b;
+    if (b) {
+        // a different shape: the device buffers stay with the handle (freed
+        // at destroy), the pinned scalars are replaced
+        if (b->sc_host) hipHostFree(b->sc_host);
+        delete b;
+        b = nullptr;
+    }
+    b = new BatchState();
+    b->nb = nb;
+    b->precision = prec;
+    const long n = h->L[0].g.ndof;
+    b->R = dalloc<double>(h, nb * n);
+    b->P = dalloc<double>(h, nb * n);
+    b->Q = dalloc<double>(h, nb * n);
+    b->Z = dalloc<double>(h, nb * n);
+    const long nmv = xcd_launch(h->L[0].t.blocks());
+    b->partials = dalloc<double>(h, nb * std::max<long>(nmv, 2 * kRedBlocks));
+    b->sc = dalloc<CgScalars>(h, nb);
+    HIPCHK(hipHostMalloc((void **)&b->sc_host, nb * sizeof(CgScalars), hipHostMallocDefault));
+    std::memset(b->sc_host, 0, nb * sizeof(CgScalars));
+    HIPCHK(hipMemcpy(b->sc, b->sc_host, nb * sizeof(CgScalars), hipMemcpyHostToDevice));
+    const bool mixed = prec != GPUTO_PREC_F64;
+    for (int l = 0; l < h->nl; ++l) {
+        const long nl_dof = h->L[l].g.ndof;
+        for (int k = 0; k < 4; ++k) {
+            if (l == 0 && k < 2) continue;  // level 0's U and F are the V-cycle's output and input
+            if (mixed)
+                b->v32[l][k] = dalloc<float>(h, nb * nl_dof);
+            else
+                b->v64[l][k] = dalloc<double>(h, nb * nl_dof);
+        }
+    }
+    if (mixed) {
+        b->r32 = dalloc<float>(h, nb * n);
+        b->z32 = dalloc<float>(h, nb * n);
+    }
+    return *b;
+}
+
+template <class T>
+static void batch_view(HierView<T> &H, T *const (&v)[kMaxLevels][4], int nl, int nb)
+{
+    for (int l = 0; l < nl; ++l) {
+        H.L[l].U = v[l][0];
+        H.L[l].F = v[l][1];
+        H.L[l].R = v[l][2];
+        H.L[l].Tmp = v[l][3];
+    }
+    H.nb = nb;
+}
+
+// The PCG of run_pcg for nb right-hand sides at once on one hierarchy
+// (F and U hold nb vectors one after another): every operator, transfer,
+// smoother, coarsest solve and vector launch applies to all nb vectors, each
+// with its own CG scalars, and a right-hand side that has converged skips
+// the rest (its control word).  The FP32 V-cycle of the mixed path applies
+// level 0 with the FP32 node-gather kernel, whose launches take a batch.
+int run_pcg_batch(gputo_handle *h, const double *F, double *U, int nb, double tol, int maxit, double *relres,
+                  int *nit)
+{
+    if (nb < 1 || nb > kMaxBatch || h->ln) return GPUTO_ERR_ARGUMENT;
+    hipStream_t s = h->stream;
+    const bool mixed = h->prm.precision != GPUTO_PREC_F64;
+    if (mixed) mixed_begin(h);
+    BatchState &b = batch_state(h, nb);
+    const bool flex = custom_schedule(h->prm);
+    Level &L0 = h->L[0];
+    const long n = L0.g.ndof;
+    const int *ctl = &b.sc->done;
+    const int nmv = (int)xcd_launch(L0.t.blocks());
+    const bool per = h->prm.periodic != 0;
+    HierView<double> H64 = f64_view(h);
+    HierView<float> H32;
+    if (mixed) {
+        H32 = h->mx->H;
+        H32.l0 = L0_GATHER;
+        H32.learn = false;
+        batch_view<float>(H32, b.v32, h->nl, nb);
+    } else {
+        batch_view<double>(H64, b.v64, h->nl, nb);
+    }
+    float *r32 = mixed ? b.r32 : nullptr;
+    auto precond = [&](const double *r, double *z, bool r32_ready) {
+        if (!mixed) {
+            vcycle_t<double>(h, H64, r, z, ctl);
+            return;
+        }
+        if (!r32_ready) launch_copy<double, float>(s, r, b.r32, n, ctl, nb);
+        vcycle_t<float>(h, H32, b.r32, b.z32, ctl);
+        launch_copy<float, double>(s, b.z32, z, n, ctl, nb);
+    };
+    launch_fill(s, U, nb * n, 0.0);
+    launch_copy(s, F, b.R, n, nullptr, nb);
+    if (per) launch_per_fold<double>(s, L0.g, b.R, 0, nullptr, nb);
+    launch_dot_partial(s, b.R, b.R, n, b.partials, nullptr, nb);
+    launch_cg_init(s, b.sc, b.partials, kRedBlocks, nb);
+    precond(b.R, b.P, false);
+    launch_copy(s, b.P, b.Z, n, ctl, nb);
+    launch_dot_partial(s, b.R, b.Z, n, b.partials, ctl, nb);
+    launch_cg_rho0(s, b.sc, b.partials, kRedBlocks, nb);
+    const int chunk = 4;
+    int enq = 0;
+    for (;;) {
+        for (int c = 0; c < chunk && enq < maxit; ++c, ++enq) {
+            OpArgs a = op_args(h, 0);
+            a.cx_floor = 0.0;
+            a.U = b.P;
+            a.out = b.Q;
+            a.partials = b.partials;
+            a.ctl = ctl;
+            a.nb = nb;
+            launch_op0(s, OP_MATVEC, a);
+            if (per) launch_per_fold<double>(s, L0.g, b.Q, 0, ctl, nb);
+            launch_cg_alpha(s, b.sc, b.partials, nmv, nb);
+            launch_cg_update_ur(s, U, b.R, b.P, b.Q, n, b.sc, b.partials, r32, nb);
+            launch_cg_check(s, b.sc, b.partials, kRedBlocks, tol, maxit, nb);
+            precond(b.R, b.Z, r32 != nullptr);
+            if (flex) {
+                launch_cg_beta_flex<double>(s, b.R, b.Z, b.Q, n, b.sc, b.partials, maxit, nb);
+            } else {
+                launch_dot_partial(s, b.R, b.Z, n, b.partials, ctl, nb);
+                launch_cg_beta(s, b.sc, b.partials, kRedBlocks, maxit, nb);
+            }
+            launch_cg_update_p(s, b.P, b.Z, n, b.sc, nb);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(b.sc_host, b.sc, nb * sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        bool all = true;
+        for (int q = 0; q < nb; ++q) all = all && b.sc_host[q].done;
+        if (all || enq >= maxit) break;
+    }
+    if (int frc = coarse_check(h->C)) {  // the solve used a failed factorisation
+        h->hier_ready = false;
+        coarse_invalidate(h->C);
+        for (int q = 0; q < nb; ++q) nit[q] = frc;
+        return frc;
+    }
+    int rc = 0;
+    for (int q = 0; q < nb; ++q) {
+        const CgScalars &r = b.sc_host[q];
+        relres[q] = r.relres;
+        nit[q] = r.done ? r.nit : maxit;
+        if (maxit <= 0) nit[q] = 0;
+        if (r.bad) {
+            nit[q] = GPUTO_ERR_NONFINITE;
+            rc = GPUTO_ERR_NONFINITE;
+        }
+    }
+    if (mixed) mixed_end(h, nit[0], b.sc_host[0].done);
+    if (h->prm.print_level >= 1)
+        for (int q = 0; q < nb; ++q)
+            std::fprintf(stderr, "gputo: MGCG rhs %d nit=%d relres=%.6g\n", q, nit[q], relres[q]);
+    return rc;
+}
+
 }  // namespace gputo
 
 // ============================================================================
@@ -674,7 +837,7 @@ int gputo_create(int nelx, int nely, int nelz, int nl, const gputo_params *param
             }
             C.Linv = C.Lbuf[0];
             C.LinvT = C.LTbuf[0];
-            C.y = dalloc<double>(h, N);
+            C.y = dalloc<double>(h, (long)kMaxBatch * N);  // one per right-hand side of a batch
         }
         C.direct = dense_ok;
         C.flag = dalloc<int>(h, 1);
@@ -703,6 +866,10 @@ int gputo_destroy(gputo_handle *h)
     if (h->C.side) hipStreamSynchronize(h->C.side);
     for (void *p : h->allocs) hipFree(p);
     if (h->sc_host) hipHostFree(h->sc_host);
+    if (h->bt) {
+        if (h->bt->sc_host) hipHostFree(h->bt->sc_host);
+        delete h->bt;
+    }
     if (h->stream) hipStreamDestroy(h->stream);
     if (h->C.side) hipStreamDestroy(h->C.side);
     if (h->C.ev_in) hipEventDestroy(h->C.ev_in);
@@ -1216,12 +1383,23 @@ int gputo_homogenize(gputo_handle *h, const double *Ke0, const double *CX, doubl
         homog_loads(Ke0, fe, Q0);  // Ke0 read row-major, as the operators read it
         launch_homog_rhs(s, g0, h->CX, fe, h->L[0].mask, Fq);
         HIPCHK(hipGetLastError());
-        for (int q = 0; q < 6; ++q) {
-            double rr = 0.0;
-            int ni = 0;
-            int rc = run_pcg(h, Fq + q * n, Xq + q * n, tol, maxit, &rr, &ni);
-            if (relres) relres[q] = rr;
-            if (nit) nit[q] = rc ? rc : ni;
+        if (h->prm.homog_sequential) {
+            for (int q = 0; q < 6; ++q) {
+                double rr = 0.0;
+                int ni = 0;
+                int rc = run_pcg(h, Fq + q * n, Xq + q * n, tol, maxit, &rr, &ni);
+                if (relres) relres[q] = rr;
+                if (nit) nit[q] = rc ? rc : ni;
+                if (rc) return rc;
+            }
+        } else {
+            double rr[6];
+            int ni[6];
+            const int rc = run_pcg_batch(h, Fq, Xq, 6, tol, maxit, rr, ni);
+            for (int q = 0; q < 6; ++q) {
+                if (relres) relres[q] = rr[q];
+                if (nit) nit[q] = ni[q];
+            }
             if (rc) return rc;
         }
         // CH_ab |Y| = sum_e CX_e (u0_a - chi_a)^T Ke0 (u0_b - chi_b)

@@ -53,6 +53,7 @@ struct HierView {
     bool learn = false;            // learned level-2 stage (FP32 views only)
     bool mg9 = false;              // every level matrix-free (mgcg9 engine)
     bool per = false;              // periodic cell: folded defects, images kept equal
+    int nb = 1;                    // right-hand sides: level vectors hold nb vectors each, one after another
 };
 
 constexpr int kLearnLevel = 2;
@@ -92,6 +93,7 @@ OpArgsT<T> view_args(const gputo_handle *h, const HierView<T> &H, int l, const i
     a.mask = h->L[l].mask;
     a.AD = H.L[l].AD;
     a.ctl = ctl;
+    a.nb = H.nb;
     const Preset p = preset_of(h->prm);
     a.omega = p.omega;
     a.prerelax = p.prerelax;
@@ -134,14 +136,14 @@ void sweeps_per(const gputo_handle *h, const HierView<T> &H, int l, const T *F, 
     const LevelView<T> &v = H.L[l];
     const Level &lv = h->L[l];
     const Preset p = preset_of(h->prm);
-    if (pre) launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 1, ctl);
+    if (pre) launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 1, ctl, H.nb);
     for (int s = 0; s < n; ++s) {
         OpArgsT<T> a = view_args(h, H, l, ctl);
         a.U = U;
         a.F = F;
         a.out = v.R;
         level_op(h, H, l, OP_DEFECT, a);
-        launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 0, ctl);
+        launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 0, ctl, H.nb);
     }
 }
 
@@ -161,14 +163,15 @@ void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T 
         // for every halo node; the FP64 node-gather keeps its fused form,
         // 87 us against 22 + 70 us).  It goes where the sweeps end in U.
         in_tmp = n % 2 == 1;
-        launch_prerelax<T>(h->stream, F, v.AD, in_tmp ? v.Tmp : U, h->L[l].g.ndof, preset_of(h->prm).prerelax, ctl);
+        launch_prerelax<T>(h->stream, F, v.AD, in_tmp ? v.Tmp : U, h->L[l].g.ndof, preset_of(h->prm).prerelax, ctl,
+                           H.nb);
         pre = false;
     }
     const T *src = in_tmp ? v.Tmp : U;
     for (int s = 0; s < n; ++s) {
         T *dst = ((n - 1 - s) % 2 == 0) ? U : v.Tmp;
         if (!(pre && s == 0) && dst == src) {
-            launch_copy<T, T>(h->stream, U, v.Tmp, h->L[l].g.ndof, ctl);
+            launch_copy<T, T>(h->stream, U, v.Tmp, h->L[l].g.ndof, ctl, H.nb);
             src = v.Tmp;
         }
         OpArgsT<T> a = view_args(h, H, l, ctl);
@@ -214,8 +217,8 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
         const T *F = l == 0 ? F0 : v.F;
         T *U = l == 0 ? U0 : v.U;
         if (l > 0) {
-            launch_restrict<T>(h->stream, lv.g, H.L[l - 1].R, v.F, lv.mask, ctl);
-            if (H.per) launch_per_fold<T>(h->stream, lv.g, v.F, 0, ctl);
+            launch_restrict<T>(h->stream, lv.g, H.L[l - 1].R, v.F, lv.mask, ctl, H.nb);
+            if (H.per) launch_per_fold<T>(h->stream, lv.g, v.F, 0, ctl, H.nb);
         }
         if (l < nl - 1) {
             const int npre = sweeps_of<T>(h, l, false);
@@ -227,7 +230,7 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
                 level_op(h, H, l, OP_PRE_DEFECT, a);
             } else {
                 if (npre == 0)
-                    launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 1, ctl);
+                    launch_per_jacobi<T>(h->stream, lv.g, U, v.R, F, v.AD, lv.mask, p.omega, p.prerelax, 1, ctl, H.nb);
                 else
                     sweeps_t(h, H, l, F, U, npre, true, ctl);
                 OpArgsT<T> a = view_args(h, H, l, ctl);
@@ -235,15 +238,15 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
                 a.F = F;
                 a.out = v.R;
                 level_op(h, H, l, OP_DEFECT, a);
-                if (H.per) launch_per_fold<T>(h->stream, lv.g, v.R, 0, ctl);
+                if (H.per) launch_per_fold<T>(h->stream, lv.g, v.R, 0, ctl, H.nb);
             }
         } else if (!h->C.direct) {
             // coarsest level too large to factor: 10*npre sweeps from the
             // level's previous iterate (mgcg9.cu:553-563)
             sweeps_t(h, H, l, F, U, 10 * p.npre, false, ctl);
         } else {
-            launch_coarse_solve<T>(h->stream, h->C, F, U, lv.mask, ctl);
-            if (H.per) launch_per_bcast<T>(h->stream, lv.g, U, ctl);
+            launch_coarse_solve<T>(h->stream, h->C, F, U, lv.mask, ctl, H.nb);
+            if (H.per) launch_per_bcast<T>(h->stream, lv.g, U, ctl, H.nb);
         }
     }
     for (int l = nl - 2; l >= 0; --l) {
@@ -255,7 +258,7 @@ void vcycle_t(const gputo_handle *h, const HierView<T> &H, const T *F0, T *U0, c
         const int npost = sweeps_of<T>(h, l, true);
         const bool to_tmp = !H.per && npost % 2 == 1;
         launch_interp_to<T>(h->stream, h->L[l].g, H.L[l + 1].U, U, to_tmp ? H.L[l].Tmp : U, h->L[l].mask,
-                            p.zero_after_interp, ctl);
+                            p.zero_after_interp, ctl, H.nb);
         sweeps_t(h, H, l, F, U, npost, false, ctl, to_tmp);
         if constexpr (std::is_same<T, float>::value) {
             if (H.learn && l == kLearnLevel) learn_stage(const_cast<gputo_handle *>(h), H, l, F, U, ctl);

@@ -57,6 +57,7 @@ class GputoParams(ctypes.Structure):
         ("mixed_npost1", ctypes.c_int),
         ("mixed_npre2", ctypes.c_int),
         ("mixed_npost2", ctypes.c_int),
+        ("homog_sequential", ctypes.c_int),
     ]
 
 

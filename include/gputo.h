@@ -114,6 +114,9 @@ typedef struct gputo_params {
     int mixed_npost1;
     int mixed_npre2;   /* the same for the levels from 2 down to the last above the coarsest */
     int mixed_npost2;
+    int homog_sequential; /* gputo_homogenize: 0 (default) solves the six load cases
+                             together, each operator launch applied to all six
+                             vectors; 1 solves them one after the other */
 } gputo_params;
 
 /* Fill *p with the defaults (REF_C, F64, no learned stage, operator
