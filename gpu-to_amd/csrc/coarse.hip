@@ -1177,33 +1177,48 @@ void coarse_switch(Coarse &C, hipStream_t s)
 
 // ---------------------------------------------------------------------------
 // Coarsest solve x = L^-T L^-1 b as two row-parallel triangular products
-// (one 64-lane wave per row, fixed reduction order), then x[fixed] = 0
-// (mgcg1.c:431-432).
+// (one 256-thread block per row, fixed reduction order), then x[fixed] = 0
+// (mgcg1.c:431-432).  Each thread issues kTrmvLoads row entries (and as
+// many of b) before its first product, so a row of up to 2048 entries is
+// one memory round trip; the skip flag is read while they are in flight.
 // ---------------------------------------------------------------------------
+constexpr int kTrmvThreads = 256, kTrmvLoads = 8;
+
 template <bool UPPER, class Ti, class To>
-__global__ __launch_bounds__(64) void k_trmv(const double *__restrict__ M, const Ti *__restrict__ b,
-                                            To *__restrict__ x, int N, int n, const uint8_t *__restrict__ mask,
-                                            const int *ctl)
+__global__ __launch_bounds__(kTrmvThreads) void k_trmv(const double *__restrict__ M, const Ti *__restrict__ b,
+                                                      To *__restrict__ x, int N, int n,
+                                                      const uint8_t *__restrict__ mask, const int *ctl)
 {
+    __shared__ double red[kTrmvThreads / 64];
     ctl = batch_ctl(ctl);
-    if (ctl && ctl[0]) return;
     // lower: b has n entries, x has N (the padded y); upper: the reverse
     b = batch_ptr(b, UPPER ? (long)N : (long)n);
     x = batch_ptr(x, UPPER ? (long)n : (long)N);
-    const int row = blockIdx.x;
-    const int lo = UPPER ? row : 0, hi = UPPER ? n - 1 : (row < n ? row : n - 1);
-    // four independent partial sums keep four row loads in flight per lane
-    double s4[4] = {0.0, 0.0, 0.0, 0.0};
-    int c = lo + threadIdx.x;
-    for (; c + 192 <= hi; c += 256) {
+    const int row = blockIdx.x;  // < n: the grid has n blocks
+    const int lo = UPPER ? row : 0, hi = UPPER ? n - 1 : row;
+    const double *__restrict__ Mr = M + (long)row * N;
+    double s = 0.0;
+    for (int c0 = lo + (int)threadIdx.x; c0 <= hi; c0 += kTrmvThreads * kTrmvLoads) {
+        double mv[kTrmvLoads], bv[kTrmvLoads];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s4[u] += M[(long)row * N + c + 64 * u] * (double)b[c + 64 * u];
+        for (int u = 0; u < kTrmvLoads; ++u) {
+            const int c = c0 + kTrmvThreads * u;
+            const int cc = c <= hi ? c : c0;  // in the row's band (c0 <= hi)
+            const double m = Mr[cc];
+            mv[u] = c <= hi ? m : 0.0;  // entries past the triangle are never used
+            bv[u] = (double)b[cc];
+        }
+        if (c0 == lo + (int)threadIdx.x && ctl && ctl[0]) return;
+#pragma unroll
+        for (int u = 0; u < kTrmvLoads; ++u) s += mv[u] * bv[u];
     }
-    for (int u = 0; c <= hi; c += 64, ++u) s4[u & 3] += M[(long)row * N + c] * (double)b[c];
-    double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+    if (ctl && ctl[0]) return;  // the threads with no entries
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (threadIdx.x == 0 && row < n) {
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s = (red[0] + red[1]) + (red[2] + red[3]);
         if (UPPER && mask && ((mask[row / 3] >> (row % 3)) & 1u)) s = 0.0;
         x[row] = (To)s;
     }
@@ -1218,8 +1233,9 @@ void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const
     coarse_finish(C);  // also a lagged factorisation's, which runs meanwhile
     if (C.solve_waits) HIPCHK(hipStreamWaitEvent(s, C.ev_done, 0));  // the factorisation of this hierarchy
     // C.y has room for kMaxBatch right-hand sides
-    k_trmv<false, T, double><<<dim3(C.n, 1, batch_dim(nb)), 64, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);
-    k_trmv<true, double, T><<<dim3(C.n, 1, batch_dim(nb)), 64, 0, s>>>(C.LinvT, C.y, x, N, C.n, mask, ctl);
+    k_trmv<false, T, double>
+        <<<dim3(C.n, 1, batch_dim(nb)), kTrmvThreads, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);
+    k_trmv<true, double, T><<<dim3(C.n, 1, batch_dim(nb)), kTrmvThreads, 0, s>>>(C.LinvT, C.y, x, N, C.n, mask, ctl);
 }
 template void launch_coarse_solve<double>(hipStream_t, const Coarse &, const double *, double *, const uint8_t *,
                                           const int *, int);

@@ -470,22 +470,41 @@ void launch_stencil_build(hipStream_t s, const Grid &g, const double *A, double 
 
 // Three waves per 64 consecutive nodes: wave w sums the 9 blocks of the
 // neighbour plane di = w - 1 (FP32: all 108 loads in flight together; FP64:
-// one dk row of 36 at a time), the three plane sums meet in LDS and wave 0
-// adds them in plane order and finishes as k_stored does.
+// one dk row of 36 at a time), the three plane sums meet in LDS and wave 1
+// (the node's own plane) adds them in plane order and finishes as k_stored
+// does.  On the small levels a launch is a few memory round trips long, so
+// every load is issued in the first one: wave 1 fetches its epilogue
+// operands (mask, F, AD, own u) with the stencil, and the skip flag is read
+// alongside and tested once they are in flight.
 template <class T, int MODE>
 __global__ __launch_bounds__(192) void k_stencil(OpArgsT<T> a)
 {
     batch_shift(a);
-    if (a.ctl && a.ctl[0]) return;
-    __shared__ T red[3][3][64];
+    __shared__ T red[2][3][64];
     const Grid g = a.g;
     const long nnod = g.nnod;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long p = (long)blockIdx.x * 64 + lane;
     const bool live = p < nnod;
     const long pc = live ? p : 0;
-    const int j = (int)(pc % g.nny), k = (int)((pc / g.nny) % g.nnz), i = (int)(pc / ((long)g.nny * g.nnz));
+    // 32-bit decomposition (a level has far fewer than 2^32 nodes)
+    const unsigned pu = (unsigned)pc, q1 = pu / (unsigned)g.nny;
+    const int j = (int)(pu - q1 * (unsigned)g.nny), k = (int)(q1 % (unsigned)g.nnz), i = (int)(q1 / (unsigned)g.nnz);
     const int ii = i + w - 1;
+    const long pw = pc + (long)(w - 1) * ((long)g.nny * g.nnz);  // the node's image in plane ii
+    // epilogue operands of wave 1 (in bounds for every lane: node pc)
+    const long dof = 3 * pc;
+    unsigned msk = 0;
+    T fe[3] = {T(0), T(0), T(0)}, ade[3] = {T(1), T(1), T(1)}, ue[3] = {T(0), T(0), T(0)};
+    if (w == 1) {
+        msk = a.mask[pc];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (MODE != OP_MATVEC) fe[c] = a.F[dof + c];
+            if (MODE != OP_MATVEC && MODE != OP_DEFECT) ade[c] = a.AD[dof + c];
+            if (MODE == OP_JACOBI) ue[c] = a.U[dof + c];
+        }
+    }
     T y0 = T(0), y1 = T(0), y2 = T(0);
     if (live && ii >= 0 && ii < g.nnx) {
         const T *__restrict__ Sp = a.S + (long)(81 * w) * nnod + p;  // blocks 9w .. 9w+8
@@ -502,7 +521,7 @@ __global__ __launch_bounds__(192) void k_stencil(OpArgsT<T> a)
                     const int kk = k + dk, jj = j + dj;
                     const bool ok = kk >= 0 && kk < g.nnz && jj >= 0 && jj < g.nny;
                     // outside the grid the stencil block is zero: read any valid node
-                    const long nb = ok ? (long)g.nny * g.nnz * ii + (long)g.nny * kk + jj : pc;
+                    const long nb = ok ? pw + (dk * g.nny + dj) : pc;
                     const int bl = 3 * (r0 + rr) + c3;
 #pragma unroll
                     for (int q = 0; q < 9; ++q) sv[rr][c3][q] = Sp[(long)(9 * bl + q) * nnod];
@@ -510,6 +529,7 @@ __global__ __launch_bounds__(192) void k_stencil(OpArgsT<T> a)
                     for (int c = 0; c < 3; ++c) uu[rr][c3][c] = stored_u<T, MODE>(a, 3 * nb + c);
                 }
             }
+            if (r0 == 0 && a.ctl && a.ctl[0]) return;  // converged: nothing is written
 #pragma unroll
             for (int rr = 0; rr < NR; ++rr)
 #pragma unroll
@@ -522,29 +542,29 @@ __global__ __launch_bounds__(192) void k_stencil(OpArgsT<T> a)
                 }
         }
     }
-    red[w][0][lane] = y0;
-    red[w][1][lane] = y1;
-    red[w][2][lane] = y2;
+    if (a.ctl && a.ctl[0]) return;
+    if (w != 1) {
+        red[w >> 1][0][lane] = y0;
+        red[w >> 1][1][lane] = y1;
+        red[w >> 1][2][lane] = y2;
+    }
     __syncthreads();
-    if (w != 0 || !live) return;
-    const long dof = 3 * p;
-    const unsigned msk = a.mask[p];
+    if (w != 1 || !live) return;
     const T om = T(a.omega), one = T(1);
+    const T yv[3] = {y0, y1, y2};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const T yc = red[0][c][lane] + red[1][c][lane] + red[2][c][lane];
+        const T yc = red[0][c][lane] + yv[c] + red[1][c][lane];
         const bool fixed = (msk >> c) & 1u;
-        const T u = stored_u<T, MODE>(a, dof + c);
+        const T u = (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT) ? T(a.prerelax) * fe[c] / ade[c] : ue[c];
         if (MODE == OP_MATVEC) {
             a.out[dof + c] = fixed ? T(0) : yc;
         } else if (MODE == OP_DEFECT || MODE == OP_PRE_DEFECT) {
-            a.out[dof + c] = fixed ? T(0) : a.F[dof + c] - yc;
+            a.out[dof + c] = fixed ? T(0) : fe[c] - yc;
             if (MODE == OP_PRE_DEFECT) a.out2[dof + c] = u;
         } else {
-            const T f = a.F[dof + c];
-            const T rv = fixed ? T(0) : f - yc;
-            const T ad = a.AD[dof + c];
-            a.out[dof + c] = (one - om) * u + om * (rv + ad * u) / ad;
+            const T rv = fixed ? T(0) : fe[c] - yc;
+            a.out[dof + c] = (one - om) * u + om * (rv + ade[c] * u) / ade[c];
         }
     }
 }
@@ -634,43 +654,43 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gc, const T *__restric
                                                     const uint8_t *__restrict__ maskc, const int *ctl)
 {
     ctl = batch_ctl(ctl);
-    if (ctl && ctl[0]) return;
     const long nod = (long)blockIdx.x * kBlock + threadIdx.x;
     if (nod >= gc.nnod) return;
-    const int J = (int)(nod % gc.nny);
-    const int K = (int)((nod / gc.nny) % gc.nnz);
-    const int I = (int)(nod / ((long)gc.nny * gc.nnz));
+    const unsigned nu = (unsigned)nod, q1 = nu / (unsigned)gc.nny;  // 32-bit: far fewer than 2^32 nodes
+    const int J = (int)(nu - q1 * (unsigned)gc.nny);
+    const int K = (int)(q1 % (unsigned)gc.nnz);
+    const int I = (int)(q1 / (unsigned)gc.nnz);
     const int fny = 2 * gc.ny + 1, fnz = 2 * gc.nz + 1, fnx = 2 * gc.nx + 1;
+    const long fc = (long)fny * fnz * (2 * I) + (long)fny * (2 * K) + 2 * J;  // the fine node at (2I,2K,2J)
     Rf = batch_ptr(Rf, 3L * fnx * fny * fnz);
     Fc = batch_ptr(Fc, gc.ndof);
-    T s0 = T(0), s1 = T(0), s2 = T(0);
-    // one fine plane (9 nodes) per pass, its loads issued together; nodes
+    // all 27 fine nodes' loads are issued before the first product; nodes
     // outside the mesh read node 0 with weight 0 (no branches around loads)
+    T v[27][3], w[27];
 #pragma unroll
-    for (int di = -1; di <= 1; ++di) {
-        const int iP = 2 * I + di;
-        T v[9][3], w[9];
+    for (int di = -1; di <= 1; ++di)
 #pragma unroll
         for (int dk = -1; dk <= 1; ++dk)
 #pragma unroll
             for (int dj = -1; dj <= 1; ++dj) {
-                const int q = 3 * (dk + 1) + (dj + 1);
-                const int kP = 2 * K + dk, jP = 2 * J + dj;
+                const int q = 9 * (di + 1) + 3 * (dk + 1) + (dj + 1);
+                const int iP = 2 * I + di, kP = 2 * K + dk, jP = 2 * J + dj;
                 const bool ok = iP >= 0 && iP < fnx && kP >= 0 && kP < fnz && jP >= 0 && jP < fny;
-                const long f = ok ? 3 * ((long)fny * fnz * iP + (long)fny * kP + jP) : 0;
+                const long f = ok ? 3 * (fc + ((long)di * fny * fnz + dk * fny + dj)) : 0;
                 w[q] = ok ? T(1) / (T)(1 << (abs(di) + abs(dj) + abs(dk))) : T(0);
                 v[q][0] = Rf[f];
                 v[q][1] = Rf[f + 1];
                 v[q][2] = Rf[f + 2];
             }
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            s0 += w[q] * v[q][0];
-            s1 += w[q] * v[q][1];
-            s2 += w[q] * v[q][2];
-        }
-    }
     const unsigned m = maskc[nod];
+    if (ctl && ctl[0]) return;  // read while the loads above are in flight
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        s0 += w[q] * v[q][0];
+        s1 += w[q] * v[q][1];
+        s2 += w[q] * v[q][2];
+    }
     Fc[3 * nod + 0] = (m & 1u) ? T(0) : s0;
     Fc[3 * nod + 1] = (m & 2u) ? T(0) : s1;
     Fc[3 * nod + 2] = (m & 4u) ? T(0) : s2;
@@ -699,12 +719,12 @@ __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict_
                                                   const uint8_t *__restrict__ maskf, int zero_fixed, const int *ctl)
 {
     ctl = batch_ctl(ctl);
-    if (ctl && ctl[0]) return;
     const long nod = (long)blockIdx.x * kBlock + threadIdx.x;
     if (nod >= gf.nnod) return;
-    const int j = (int)(nod % gf.nny);
-    const int k = (int)((nod / gf.nny) % gf.nnz);
-    const int i = (int)(nod / ((long)gf.nny * gf.nnz));
+    const unsigned nu = (unsigned)nod, q1 = nu / (unsigned)gf.nny;  // 32-bit: far fewer than 2^32 nodes
+    const int j = (int)(nu - q1 * (unsigned)gf.nny);
+    const int k = (int)(q1 % (unsigned)gf.nnz);
+    const int i = (int)(q1 / (unsigned)gf.nnz);
     const int cny = gf.ny / 2 + 1, cnz = gf.nz / 2 + 1;
     Uc = batch_ptr(Uc, 3L * (gf.nx / 2 + 1) * cny * cnz);
     Uin = batch_ptr(Uin, gf.ndof);
@@ -732,6 +752,8 @@ __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict_
             }
     const long d = 3 * nod;
     const T u0 = Uin[d + 0], u1 = Uin[d + 1], u2 = Uin[d + 2];
+    const unsigned m = zero_fixed ? maskf[nod] : 0u;
+    if (ctl && ctl[0]) return;  // read while the loads above are in flight
     T s0 = T(0), s1 = T(0), s2 = T(0);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -739,7 +761,6 @@ __global__ __launch_bounds__(kBlock) void k_interp(Grid gf, const T *__restrict_
         s1 += on[q] * v[q][1];
         s2 += on[q] * v[q][2];
     }
-    const unsigned m = zero_fixed ? maskf[nod] : 0u;
     Uout[d + 0] = (m & 1u) ? T(0) : u0 + w * s0;
     Uout[d + 1] = (m & 2u) ? T(0) : u1 + w * s1;
     Uout[d + 2] = (m & 4u) ? T(0) : u2 + w * s2;
@@ -851,6 +872,7 @@ __global__ __launch_bounds__(kBlock) void k_dot_partial(const Ta *__restrict__ a
     b = batch_ptr(b, n);
     partials = batch_ptr(partials, (long)gridDim.x);
     double s = 0.0;
+#pragma unroll 4
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock)
         s += (double)a[i] * (double)b[i];
     s = block_sum(s);
@@ -869,10 +891,21 @@ template void launch_dot_partial<float, float>(hipStream_t, const float *, const
 template void launch_dot_partial<double, float>(hipStream_t, const double *, const float *, long, double *,
                                                 const int *, int);
 
+// Each thread adds p[t], p[t + kBlock], p[t + 2 kBlock], ... in that order;
+// the loads of a chunk are all issued before the first add, so a few
+// thousand partials cost a round trip per chunk, not one per partial.
 __device__ double sum_partials(const double *p, int n)
 {
+    constexpr int kChunk = 16;
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlock) s += p[i];
+    for (int i0 = threadIdx.x; i0 < n; i0 += kChunk * kBlock) {
+        double v[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) v[k] = i0 + k * kBlock < n ? p[i0 + k * kBlock] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k)
+            if (i0 + k * kBlock < n) s += v[k];
+    }
     return block_sum(s);
 }
 
@@ -942,13 +975,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *_
     R32 = batch_ptr(R32, n);
     partials = batch_ptr(partials, (long)gridDim.x);
     const T al = (T)sc->alpha;
+    const long stride = (long)kRedBlocks * kBlock;
     double s = 0.0;
-    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) {
-        U[i] += al * P[i];
-        const T r = R[i] - al * Q[i];
-        R[i] = r;
-        if (R32) R32[i] = (float)r;  // the FP32 V-cycle's right-hand side
-        s += (double)r * (double)r;
+    // two elements per pass, both loaded before either is stored
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += 2 * stride) {
+        const bool two = i + stride < n;
+        const long j = two ? i + stride : i;  // in bounds; unused when !two
+        const T u0 = U[i], p0 = P[i], r0 = R[i], q0 = Q[i];
+        const T u1 = U[j], p1 = P[j], r1 = R[j], q1 = Q[j];
+        const T ra = r0 - al * q0, rb = r1 - al * q1;
+        U[i] = u0 + al * p0;
+        R[i] = ra;
+        if (R32) R32[i] = (float)ra;  // the FP32 V-cycle's right-hand side
+        s += (double)ra * (double)ra;
+        if (two) {
+            U[j] = u1 + al * p1;
+            R[j] = rb;
+            if (R32) R32[j] = (float)rb;
+            s += (double)rb * (double)rb;
+        }
     }
     s = block_sum(s);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -1007,6 +1052,7 @@ __global__ __launch_bounds__(kBlock) void k_dot2_partial(const double *__restric
     q = batch_ptr(q, n);
     partials = batch_ptr(partials, 2L * kRedBlocks);
     double a = 0.0, b = 0.0;
+#pragma unroll 4
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)kRedBlocks * kBlock) {
         const double zi = (double)z[i];  // exact for an FP32 z
         a += r[i] * zi;
@@ -1061,7 +1107,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ P, const
     Z = batch_ptr(Z, n);
     const T be = (T)sc->beta;
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
-        P[i] = (T)Z[i] + be * P[i];
+        P[i] = fma(be, P[i], (T)Z[i]);
 }
 
 void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb)

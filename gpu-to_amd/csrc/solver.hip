@@ -334,12 +334,8 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
         // mgcg9 (initVcycle, mgcg9.cu:342-414): coarse element scales and
         // diagonals from the unit Ke; the coarsest matrix, when factored,
         // is CX*Ke assembled with its fixed rows and columns decoupled
-        for (int l = 1; l < h->nl; ++l) {
-            Level &v = h->L[l];
-            launch_coarse_cx(s, v.g, l == 1 ? h->CX : h->L[l - 1].CXl, v.CXl);
-            launch_diag0(s, v.g, h->Ke, v.CXl, v.AD);
-            if (per) launch_per_fold<double>(s, v.g, v.AD, 1, nullptr);
-        }
+        for (int l = 1; l < h->nl; ++l)
+            launch_coarse_cx(s, h->L[l].g, l == 1 ? h->CX : h->L[l - 1].CXl, h->L[l].CXl);
         Level &vc = h->L[h->nl - 1];
         h->C.direct = vc.g.ndof < kMaxCoarseDofs;
         // a smoothed coarsest level starts each solve from zero and then
@@ -354,6 +350,13 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
             launch_zero_rowcol(s, (h->C.n + 31) / 32 * 32, vc.fixed_d, (int)vc.fixed.size(), h->C.AQ);
             HIPCHK(hipGetLastError());
             coarse_factor(s, h->C, coarse_lagged(h->prm));
+        }
+        // the diagonals are not needed by the factorisation: launched after
+        // it, they run while it does
+        for (int l = 1; l < h->nl; ++l) {
+            Level &v = h->L[l];
+            launch_diag0(s, v.g, h->Ke, v.CXl, v.AD);
+            if (per) launch_per_fold<double>(s, v.g, v.AD, 1, nullptr);
         }
         HIPCHK(hipGetLastError());
         h->hier_ready = true;
@@ -378,14 +381,19 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
                 launch_assemble(s, v.g, v.A, h->C.AQ);
             }
         }
-        launch_diagA(s, v.g, v.A, v.AD);
-        if (per) launch_per_fold<double>(s, v.g, v.AD, 1, nullptr);
-        if (v.S) launch_stencil_build(s, v.g, v.A, v.S);
     }
     HIPCHK(hipGetLastError());
     // overlaps the solve (lagged) or the first V-cycle's descent (exact);
     // errors surface in coarse_check / coarse_adopt
     coarse_factor(s, h->C, coarse_lagged(h->prm));
+    // diagonals and node stencils, which the factorisation does not read,
+    // run on the main stream while it does
+    for (int l = 1; l < h->nl; ++l) {
+        Level &v = h->L[l];
+        launch_diagA(s, v.g, v.A, v.AD);
+        if (per) launch_per_fold<double>(s, v.g, v.AD, 1, nullptr);
+        if (v.S) launch_stencil_build(s, v.g, v.A, v.S);
+    }
     HIPCHK(hipGetLastError());
     h->hier_ready = true;
     return 0;
