@@ -41,7 +41,6 @@ template <class T, int MODE, bool KC>
 __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
 {
     batch_shift(a);
-    if (a.ctl && a.ctl[0]) return;
     __shared__ T su[3][kMaxHalo];
     __shared__ T sE[kMaxElemBox];
     __shared__ T sK[KC ? 1 : 576];  // general Ke: wave-uniform (broadcast) LDS reads
@@ -95,6 +94,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
                 v[u][cc] = ok ? x : T(0);
             }
         }
+        // the skip flag is read while the first rows' loads are in flight
+        if (r0 == wave && a.ctl && a.ctl[0]) return;
 #pragma unroll
         for (int u = 0; u < kStageRows; ++u) {
             const int row = r0 + (kBlock / 64) * u;
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
             }
         }
     }
+    if (a.ctl && a.ctl[0]) return;  // every thread tests it before the barrier
     const int EJ = t.TJ + 1, EK = t.TK + 1;
     {
         const int EI = t.TI + 1;
@@ -803,6 +805,10 @@ static unsigned grid_for(long n)
     return (unsigned)nb;
 }
 
+// One element per thread: every load of a pass is issued at once (a
+// grid-stride loop would wait for its stores before the next loads).
+static unsigned grid_one(long n) { return (unsigned)std::max(1L, (n + kBlock - 1) / kBlock); }
+
 template <class T>
 __global__ void k_fill(T *v, long n, T val)
 {
@@ -840,17 +846,19 @@ __global__ void k_prerelax(const T *__restrict__ F, const T *__restrict__ AD, T 
                            const int *ctl)
 {
     ctl = batch_ctl(ctl);
-    if (ctl && ctl[0]) return;
     F = batch_ptr(F, n);
     U = batch_ptr(U, n);
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        U[i] = pr * F[i] / AD[i];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T f = F[i], ad = AD[i];
+    if (ctl && ctl[0]) return;  // read while the loads are in flight
+    U[i] = pr * f / ad;
 }
 
 template <class T>
 void launch_prerelax(hipStream_t s, const T *F, const T *AD, T *U, long n, double prerelax, const int *ctl, int nb)
 {
-    k_prerelax<T><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(F, AD, U, n, T(prerelax), ctl);
+    k_prerelax<T><<<dim3(grid_one(n), 1, batch_dim(nb)), kBlock, 0, s>>>(F, AD, U, n, T(prerelax), ctl);
 }
 template void launch_prerelax<double>(hipStream_t, const double *, const double *, double *, long, double,
                                       const int *, int);
@@ -932,12 +940,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(CgScalars *sc, const double 
     }
 }
 
+// The scalar steps read their CgScalars fields and the partials first and
+// test the converged flag after: one memory round trip, not three.
 __global__ __launch_bounds__(kBlock) void k_cg_rho0(CgScalars *sc, const double *p, int n)
 {
     sc += blockIdx.z;
     p += (long)blockIdx.z * n;
-    if (sc->done) return;
     const double rz = sum_partials(p, n);
+    if (sc->done) return;
     if (threadIdx.x == 0) {
         sc->rho = rz;
         sc->rho_p = rz;
@@ -948,11 +958,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars *sc, const double
 {
     sc += blockIdx.z;
     p += (long)blockIdx.z * n;
-    if (sc->done) return;
+    const double rho_p = sc->rho_p;
     const double pq = sum_partials(p, n);
+    if (sc->done) return;
     if (threadIdx.x == 0) {
         sc->pq = pq;
-        sc->alpha = sc->rho_p / pq;
+        sc->alpha = rho_p / pq;
         if (!(fabs(sc->alpha) < INFINITY)) {
             sc->bad = 1;
             sc->done = 1;
@@ -967,7 +978,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *_
                                                         double *partials, float *__restrict__ R32)
 {
     sc += blockIdx.z;
-    if (sc->done) return;
     U = batch_ptr(U, n);
     R = batch_ptr(R, n);
     P = batch_ptr(P, n);
@@ -976,13 +986,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *_
     partials = batch_ptr(partials, (long)gridDim.x);
     const T al = (T)sc->alpha;
     const long stride = (long)kRedBlocks * kBlock;
+    const long i0 = (long)blockIdx.x * kBlock + threadIdx.x;
     double s = 0.0;
     // two elements per pass, both loaded before either is stored
-    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += 2 * stride) {
+    for (long i = i0; i < n; i += 2 * stride) {
         const bool two = i + stride < n;
         const long j = two ? i + stride : i;  // in bounds; unused when !two
         const T u0 = U[i], p0 = P[i], r0 = R[i], q0 = Q[i];
         const T u1 = U[j], p1 = P[j], r1 = R[j], q1 = Q[j];
+        if (i == i0 && sc->done) return;  // read with alpha while the loads are in flight
         const T ra = r0 - al * q0, rb = r1 - al * q1;
         U[i] = u0 + al * p0;
         R[i] = ra;
@@ -995,6 +1007,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *_
             s += (double)rb * (double)rb;
         }
     }
+    if (sc->done) return;  // every thread tests it before the barrier
     s = block_sum(s);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
@@ -1003,11 +1016,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_check(CgScalars *sc, const double
 {
     sc += blockIdx.z;
     p += (long)blockIdx.z * n;
-    if (sc->done) return;
+    const double res0 = sc->res0;
     const double rr = sum_partials(p, n);
+    if (sc->done) return;
     if (threadIdx.x == 0) {
         sc->rr = rr;
-        sc->relres = sqrt(rr) / sc->res0;
+        sc->relres = sqrt(rr) / res0;
         if (!(sc->relres < INFINITY)) {
             sc->bad = 1;
             sc->done = 1;
@@ -1023,11 +1037,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(CgScalars *sc, const double 
 {
     sc += blockIdx.z;
     p += (long)blockIdx.z * n;
-    if (sc->done) return;
+    const double rho_p = sc->rho_p;
     const double rz = sum_partials(p, n);
+    if (sc->done) return;
     if (threadIdx.x == 0) {
         sc->rho = rz;
-        sc->beta = rz / sc->rho_p;
+        sc->beta = rz / rho_p;
         sc->rho_p = rz;
         sc->it += 1;
         if (sc->it >= maxit) {
@@ -1070,12 +1085,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta_flex(CgScalars *sc, const do
 {
     sc += blockIdx.z;
     p += 2L * kRedBlocks * blockIdx.z;
-    if (sc->done) return;
+    const double alpha = sc->alpha, rho_p = sc->rho_p;
     const double rz = sum_partials(p, kRedBlocks);
     const double qz = sum_partials(p + kRedBlocks, kRedBlocks);
+    if (sc->done) return;
     if (threadIdx.x == 0) {
         sc->rho = rz;
-        sc->beta = -sc->alpha * qz / sc->rho_p;
+        sc->beta = -alpha * qz / rho_p;
         sc->rho_p = rz;
         sc->it += 1;
         if (sc->it >= maxit) {
@@ -1102,12 +1118,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ P, const
                                                        const CgScalars *sc)
 {
     sc += blockIdx.z;
-    if (sc->done) return;
     P = batch_ptr(P, n);
     Z = batch_ptr(Z, n);
-    const T be = (T)sc->beta;
-    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
-        P[i] = fma(be, P[i], (T)Z[i]);
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const T pv = P[i], zv = (T)Z[i];
+    if (sc->done) return;  // read with beta while the loads are in flight
+    P[i] = fma((T)sc->beta, pv, zv);
 }
 
 void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb)
@@ -1143,13 +1160,13 @@ void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int np
 template <class T>
 void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc, int nb)
 {
-    k_cg_update_p<T, T><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
+    k_cg_update_p<T, T><<<dim3(grid_one(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
 }
 template void launch_cg_update_p<double>(hipStream_t, double *, const double *, long, CgScalars *, int);
 template void launch_cg_update_p<float>(hipStream_t, float *, const float *, long, CgScalars *, int);
 void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc, int nb)
 {
-    k_cg_update_p<double, float><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
+    k_cg_update_p<double, float><<<dim3(grid_one(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
 }
 
 }  // namespace gputo

@@ -41,7 +41,6 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<T> a, Tile t, const T 
                                                  T lam, T mu)
 {
     batch_shift(a);
-    if (a.ctl && a.ctl[0]) return;
     __shared__ T su[3 * L1Ew<T>::HALO];
     __shared__ T sy[L1Ew<T>::NE * kL1Ys];
     const Grid g = a.g;  // the coarse (level-1) grid
@@ -85,6 +84,8 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<T> a, Tile t, const T 
                 v[u][cc] = ok ? x : T(0);
             }
         }
+        // the skip flag is read while the first rows' loads are in flight
+        if (r0 == wv && a.ctl && a.ctl[0]) return;
 #pragma unroll
         for (int u = 0; u < kStageRows; ++u) {
             const int row = r0 + 4 * u;
@@ -96,6 +97,7 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<T> a, Tile t, const T 
             }
         }
     }
+    if (a.ctl && a.ctl[0]) return;  // every thread tests it before the barrier
     __syncthreads();
 
     // element phase

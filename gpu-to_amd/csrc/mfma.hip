@@ -346,7 +346,6 @@ __device__ __forceinline__ void mf_products(const bf16x8 (&Ka)[MfA<FL, KX>::n][2
 template <int FL, int MODE, int KR, bool KX>
 __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, MfWork w, float kscale)
 {
-    if (a.ctl && a.ctl[0]) return;
     constexpr int NA = MfA<FL, KX>::n;
     constexpr int KH = KR / 2;  // output rows per lane half
     const Grid &g = a.g;
@@ -373,6 +372,10 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, MfWork w, flo
         B.M = mf_buf(a.mask, g.nnod);
         B.O = mf_buf(a.out, vb);
         B.O2 = mf_buf(a.out2, a.out2 ? vb : 0);
+        // the first element layer's operands are in flight while Ke is read
+        // and split, and while the skip flag is read
+        MfLayer<KR> LA, LB;
+        mf_load_layer<MODE, KR>(a, B, i0 - 1, k0, je, jin, h, scale, LA);
 
         // A fragments: lane row r = c (output row 8j + 4hm + cm), K step st,
         // columns 16 st + 8 h + 4 yy + cc -> Ke column 3 lidx(h, yy, st) + cc
@@ -380,13 +383,22 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, MfWork w, flo
         {
             const int jm = c >> 3, hm = (c >> 2) & 1, cm = c & 3;
             const int lnr = mf_lidx(jm & 1, hm, jm >> 1);
+            float kev[2][8];
 #pragma unroll
             for (int st = 0; st < 2; ++st)
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const int yy = i >> 2, cc = i & 3;
                     const int lnc = mf_lidx(h, yy, st);
-                    const float v = (cm < 3 && cc < 3) ? a.Ke[24 * (3 * lnr + cm) + 3 * lnc + cc] : 0.0f;
+                    const float v = a.Ke[24 * (3 * lnr + min(cm, 2)) + 3 * lnc + min(cc, 2)];
+                    kev[st][i] = (cm < 3 && cc < 3) ? v : 0.0f;
+                }
+            if (a.ctl && a.ctl[0]) return;  // converged: read with the loads above
+#pragma unroll
+            for (int st = 0; st < 2; ++st)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float v = kev[st][i];
                     if constexpr (KX) {
                         Ka[0][st][i] = (__bf16)rintf(v / kscale);  // an integer of at most 8 bits: exact
                     } else {
@@ -445,8 +457,6 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, MfWork w, flo
                 for (int rr = 0; rr < KH; ++rr) mf_out_store<MODE>(a, B, O[rr], y[rr], om);
             }
         };
-        MfLayer<KR> LA, LB;
-        mf_load_layer<MODE, KR>(a, B, i0 - 1, k0, je, jin, h, scale, LA);
         for (int tt = 0; tt <= nlay; tt += 2) {
             step(tt, accA, accB, LA, LB);
             if (tt + 1 <= nlay) step(tt + 1, accB, accA, LB, LA);
