@@ -41,6 +41,7 @@ template <class T, int MODE, bool KC>
 __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
 {
     batch_shift(a);
+    if (a.ctl && a.ctl[0]) return;
     __shared__ T su[3][kMaxHalo];
     __shared__ T sE[kMaxElemBox];
     __shared__ T sK[KC ? 1 : 576];  // general Ke: wave-uniform (broadcast) LDS reads
@@ -94,8 +95,6 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
                 v[u][cc] = ok ? x : T(0);
             }
         }
-        // the skip flag is read while the first rows' loads are in flight
-        if (r0 == wave && a.ctl && a.ctl[0]) return;
 #pragma unroll
         for (int u = 0; u < kStageRows; ++u) {
             const int row = r0 + (kBlock / 64) * u;
@@ -110,7 +109,6 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
             }
         }
     }
-    if (a.ctl && a.ctl[0]) return;  // every thread tests it before the barrier
     const int EJ = t.TJ + 1, EK = t.TK + 1;
     {
         const int EI = t.TI + 1;
@@ -805,10 +803,6 @@ static unsigned grid_for(long n)
     return (unsigned)nb;
 }
 
-// One element per thread: every load of a pass is issued at once (a
-// grid-stride loop would wait for its stores before the next loads).
-static unsigned grid_one(long n) { return (unsigned)std::max(1L, (n + kBlock - 1) / kBlock); }
-
 template <class T>
 __global__ void k_fill(T *v, long n, T val)
 {
@@ -848,17 +842,24 @@ __global__ void k_prerelax(const T *__restrict__ F, const T *__restrict__ AD, T 
     ctl = batch_ctl(ctl);
     F = batch_ptr(F, n);
     U = batch_ptr(U, n);
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const T f = F[i], ad = AD[i];
-    if (ctl && ctl[0]) return;  // read while the loads are in flight
-    U[i] = pr * f / ad;
+    // two elements per pass, both loaded before either is stored (the
+    // memory counter retires in order: a load issued after a store waits
+    // for it); the skip flag is read while the first pass's loads fly
+    const long stride = (long)gridDim.x * blockDim.x, i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long i = i0; i < n; i += 2 * stride) {
+        const bool two = i + stride < n;
+        const long j = two ? i + stride : i;  // in bounds; unused when !two
+        const T f0 = F[i], a0 = AD[i], f1 = F[j], a1 = AD[j];
+        if (i == i0 && ctl && ctl[0]) return;
+        U[i] = pr * f0 / a0;
+        if (two) U[j] = pr * f1 / a1;
+    }
 }
 
 template <class T>
 void launch_prerelax(hipStream_t s, const T *F, const T *AD, T *U, long n, double prerelax, const int *ctl, int nb)
 {
-    k_prerelax<T><<<dim3(grid_one(n), 1, batch_dim(nb)), kBlock, 0, s>>>(F, AD, U, n, T(prerelax), ctl);
+    k_prerelax<T><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(F, AD, U, n, T(prerelax), ctl);
 }
 template void launch_prerelax<double>(hipStream_t, const double *, const double *, double *, long, double,
                                       const int *, int);
@@ -1120,11 +1121,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(T *__restrict__ P, const
     sc += blockIdx.z;
     P = batch_ptr(P, n);
     Z = batch_ptr(Z, n);
-    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const T pv = P[i], zv = (T)Z[i];
-    if (sc->done) return;  // read with beta while the loads are in flight
-    P[i] = fma((T)sc->beta, pv, zv);
+    const T be = (T)sc->beta;
+    // two elements per pass, as k_prerelax; done is read with beta
+    const long stride = (long)gridDim.x * kBlock, i0 = (long)blockIdx.x * kBlock + threadIdx.x;
+    for (long i = i0; i < n; i += 2 * stride) {
+        const bool two = i + stride < n;
+        const long j = two ? i + stride : i;  // in bounds; unused when !two
+        const T p0 = P[i], p1 = P[j], z0 = (T)Z[i], z1 = (T)Z[j];
+        if (i == i0 && sc->done) return;
+        P[i] = fma(be, p0, z0);
+        if (two) P[j] = fma(be, p1, z1);
+    }
 }
 
 void launch_cg_init(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb)
@@ -1160,13 +1167,13 @@ void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int np
 template <class T>
 void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc, int nb)
 {
-    k_cg_update_p<T, T><<<dim3(grid_one(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
+    k_cg_update_p<T, T><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
 }
 template void launch_cg_update_p<double>(hipStream_t, double *, const double *, long, CgScalars *, int);
 template void launch_cg_update_p<float>(hipStream_t, float *, const float *, long, CgScalars *, int);
 void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc, int nb)
 {
-    k_cg_update_p<double, float><<<dim3(grid_one(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
+    k_cg_update_p<double, float><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
 }
 
 }  // namespace gputo
