@@ -68,19 +68,26 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
     // kStageRows rows per wave are loaded into registers before any is
     // stored, so their loads are in flight together (addresses outside the
     // mesh are clamped to 0 and the values masked).
+    // A wave's rows are wave, wave + 4, ...: their (i, k) position is
+    // stepped from row to row (no division per row), node indices are
+    // 32-bit (a grid has far fewer than 2^32 nodes).
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int rowlen = 3 * HJ;  // <= 128: two values per lane
+    constexpr int kStep = kBlock / 64;  // waves, and the row stride of one wave
+    const int rowlen = 3 * HJ;          // <= 128: two values per lane
     const int nrows = HK * HI;
-    for (int r0 = wave; r0 < nrows; r0 += (kBlock / 64) * kStageRows) {
+    const unsigned nplane = (unsigned)g.nny * (unsigned)g.nnz;
+    int shi = wave / HK, shk = wave - shi * HK;  // position of the wave's next row
+    for (int r0 = wave; r0 < nrows; r0 += kStep * kStageRows) {
         T v[kStageRows][2];
 #pragma unroll
         for (int u = 0; u < kStageRows; ++u) {
-            const int row = r0 + (kBlock / 64) * u;
-            const int hi = row / HK, hk = row - hi * HK;
-            const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
+            const int row = r0 + kStep * u;
+            const int gi = i0 - 1 + shi, gk = k0 - 1 + shk;
             const bool rowok = row < nrows && gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
-            const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
+            // node (gi, gk, j0 - 1), which is -1 at the grid's first corner
+            const long base = 3 * ((long)(nplane * (unsigned)gi + (unsigned)(g.nny * gk)) + (j0 - 1));
+            for (shk += kStep; shk >= HK; shk -= HK) ++shi;
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
                 const int col = lane + 64 * cc;
@@ -97,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
         }
 #pragma unroll
         for (int u = 0; u < kStageRows; ++u) {
-            const int row = r0 + (kBlock / 64) * u;
+            const int row = r0 + kStep * u;
             if (row >= nrows) break;
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
@@ -113,23 +120,25 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
     {
         const int EI = t.TI + 1;
         const int erows = EK * EI;
-        for (int r0 = wave; r0 < erows; r0 += (kBlock / 64) * kStageRows) {
+        const unsigned eplane = (unsigned)g.ny * (unsigned)g.nz;
+        int sei = wave / EK, sek = wave - sei * EK;
+        for (int r0 = wave; r0 < erows; r0 += kStep * kStageRows) {
             T v[kStageRows];
 #pragma unroll
             for (int u = 0; u < kStageRows; ++u) {
-                const int row = r0 + (kBlock / 64) * u;
-                const int ei = row / EK, ek = row - ei * EK;
-                const int gi = i0 - 1 + ei, gk = k0 - 1 + ek, gj = j0 - 1 + lane;
+                const int row = r0 + kStep * u;
+                const int gi = i0 - 1 + sei, gk = k0 - 1 + sek, gj = j0 - 1 + lane;
+                for (sek += kStep; sek >= EK; sek -= EK) ++sei;
                 const bool ok = row < erows && lane < EJ && gi >= 0 && gi < g.nx && gk >= 0 && gk < g.nz && gj >= 0 &&
                                 gj < g.ny;
-                const T x = a.CX[ok ? (long)g.ny * g.nz * gi + (long)g.ny * gk + gj : 0];
+                const T x = a.CX[ok ? (long)(eplane * (unsigned)gi + (unsigned)(g.ny * gk + gj)) : 0];
                 T e = ok ? x : T(0);
                 if (a.cx_floor > 0.0 && e < T(a.cx_floor)) e = T(0);  // mgcg9 defect0c skip
                 v[u] = e;
             }
 #pragma unroll
             for (int u = 0; u < kStageRows; ++u) {
-                const int row = r0 + (kBlock / 64) * u;
+                const int row = r0 + kStep * u;
                 if (row >= erows) break;
                 if (lane < EJ) sE[row * EJ + lane] = v[u];
             }

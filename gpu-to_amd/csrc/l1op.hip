@@ -58,18 +58,23 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<T> a, Tile t, const T 
     // halo brick of u, one j-row per wave (coalesced rows, wave-uniform row
     // split); kStageRows rows per wave are loaded before any is stored, with
     // addresses outside the mesh clamped and their values masked
+    // a wave's rows are wv, wv + 4, ...: their (i, k) position is stepped
+    // from row to row (no division per row); node indices are 32-bit
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
     const int rowlen = 3 * HJ;
     const int nrows = HK * HI;
+    const unsigned nplane = (unsigned)g.nny * (unsigned)g.nnz;
+    int shi = wv / HK, shk = wv - shi * HK;
     for (int r0 = wv; r0 < nrows; r0 += 4 * kStageRows) {
         T v[kStageRows][2];
 #pragma unroll
         for (int u = 0; u < kStageRows; ++u) {
             const int row = r0 + 4 * u;
-            const int hi = row / HK, hk = row - hi * HK;
-            const int gi = i0 - 1 + hi, gk = k0 - 1 + hk;
+            const int gi = i0 - 1 + shi, gk = k0 - 1 + shk;
+            for (shk += 4; shk >= HK; shk -= HK) ++shi;
             const bool rowok = row < nrows && gi >= 0 && gi < g.nnx && gk >= 0 && gk < g.nnz;
-            const long base = 3 * ((long)g.nny * g.nnz * gi + (long)g.nny * gk + (j0 - 1));
+            // node (gi, gk, j0 - 1), which is -1 at the grid's first corner
+            const long base = 3 * ((long)(nplane * (unsigned)gi + (unsigned)(g.nny * gk)) + (j0 - 1));
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
                 const int col = ln + 64 * cc;
