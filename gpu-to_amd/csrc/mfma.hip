@@ -354,15 +354,17 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, MfWork w, flo
     const int lane = threadIdx.x & 63, c = lane & 31, h = lane >> 5;
     const long unit = tb * kMfWaves + (threadIdx.x >> 6);
     if (tb < nb && unit < w.units()) {  // wave-uniform
-        long un = unit;
-        const int s = (int)(un % w.ns);
-        un /= w.ns;
-        const int kb = (int)(un % w.nkb);
-        const int ib = (int)(un / w.nkb);
+        // 32-bit: far fewer than 2^32 wave units
+        unsigned un = (unsigned)unit;
+        const int s = (int)(un % (unsigned)w.ns);
+        un /= (unsigned)w.ns;
+        const int kb = (int)(un % (unsigned)w.nkb);
+        const int ib = (int)(un / (unsigned)w.nkb);
         const int j0 = kMfStrip * s, k0 = KR * kb, i0 = w.TI * ib;
         const int je = j0 - 1 + c;  // element column of this lane
         const bool jin = je >= 0 && je < g.ny;
         const float scale = KX ? kscale : 1.0f;
+        const float kinv = KX ? 1.0f / kscale : 1.0f;
         const long vb = 4 * g.ndof;
         MfBufs B;
         B.U = mf_buf(a.U, a.U ? vb : 0);
@@ -400,7 +402,9 @@ __global__ __launch_bounds__(256) void k_mfma_op(OpArgsT<float> a, MfWork w, flo
                 for (int i = 0; i < 8; ++i) {
                     const float v = kev[st][i];
                     if constexpr (KX) {
-                        Ka[0][st][i] = (__bf16)rintf(v / kscale);  // an integer of at most 8 bits: exact
+                        // v / kscale is within rounding of an integer of at most 8 bits, so
+                        // the product with the rounded reciprocal rounds to the same one
+                        Ka[0][st][i] = (__bf16)rintf(v * kinv);
                     } else {
                         float pv[NA];
                         mf_split<NA>(v, pv);
