@@ -12,6 +12,8 @@
 //
 // One thread per coarse element evaluates the 8 Gauss points (k_l1_ew
 // below); the nodal forces meet in LDS.
+#include <type_traits>
+
 #include "vcycle.h"
 
 namespace gputo {
@@ -23,6 +25,149 @@ __device__ __forceinline__ T lane_w(int x, T xi)
 {
     // linear shape factor in one direction: (1 -/+ xi)/2
     return x ? T(0.5) * (T(1) + xi) : T(0.5) * (T(1) - xi);
+}
+
+// A_e u of one element, summed over its 2x2x2 Gauss points by sum
+// factorisation (the FP32 form): the displacement gradient at the Gauss
+// points from the nodal differences interpolated one direction at a time
+// (a trilinear field's x-derivative depends on y and z only, and so on),
+// the forces B^T S from the stresses summed over one direction at a time.
+// Node order of y as the element matrices (n: x = n in {1,2,5,6}, y = n in
+// {2,3,6,7}, z = n >= 4); child / Gauss point g = 4 sx + 2 sz + sy.
+template <class T>
+__device__ __forceinline__ void l1_element_sf(const T (&u)[2][2][2][3], const T (&cx)[8], T (&y)[24], T lam2mu,
+                                              T lam, T mu)
+{
+    const T g1 = T(0.57735026918962576451);  // 1/sqrt(3)
+    const T ih = T(0.5);                     // 1/h, h = 2
+    const T wa = T(0.5) * (T(1) + g1), wb = T(0.5) * (T(1) - g1);
+    // shape function n of one direction at Gauss coordinate s (0: -g1, 1: +g1)
+    auto W = [&](int n, int s) { return n == s ? wa : wb; };
+    T gx[3][2][2], gy[3][2][2], gz[3][2][2];  // [c][.][.]: at (sy,sz), (sx,sz), (sx,sy)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T t[2][2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                t[p][s] = W(0, s) * (u[1][p][0][c] - u[0][p][0][c]) + W(1, s) * (u[1][p][1][c] - u[0][p][1][c]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) gx[c][a][b] = ih * (W(0, a) * t[0][b] + W(1, a) * t[1][b]);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                t[p][s] = W(0, s) * (u[p][1][0][c] - u[p][0][0][c]) + W(1, s) * (u[p][1][1][c] - u[p][0][1][c]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) gy[c][a][b] = ih * (W(0, a) * t[0][b] + W(1, a) * t[1][b]);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                t[p][s] = W(0, s) * (u[p][0][1][c] - u[p][0][0][c]) + W(1, s) * (u[p][1][1][c] - u[p][1][0][c]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) gz[c][a][b] = ih * (W(0, a) * t[0][b] + W(1, a) * t[1][b]);
+    }
+    // stresses S[c][d] (symmetric) at every Gauss point [sx][sy][sz]
+    T S[3][3][2][2][2];
+#pragma unroll
+    for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+        for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+            for (int sz = 0; sz < 2; ++sz) {
+                T G[3][3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    G[c][0] = gx[c][sy][sz];
+                    G[c][1] = gy[c][sx][sz];
+                    G[c][2] = gz[c][sx][sy];
+                }
+                const T c0 = cx[4 * sx + 2 * sz + sy];
+                const T tr = G[0][0] + G[1][1] + G[2][2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) S[c][c][sx][sy][sz] = c0 * (lam * tr + (lam2mu - lam) * G[c][c]);
+                S[0][1][sx][sy][sz] = S[1][0][sx][sy][sz] = c0 * mu * (G[0][1] + G[1][0]);
+                S[1][2][sx][sy][sz] = S[2][1][sx][sy][sz] = c0 * mu * (G[1][2] + G[2][1]);
+                S[0][2][sx][sy][sz] = S[2][0][sx][sy][sz] = c0 * mu * (G[2][0] + G[0][2]);
+            }
+    auto nidx = [](int x, int yy, int z) { return 4 * z + (yy ? (x ? 2 : 3) : (x ? 1 : 0)); };
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        // d = 0 (x-derivative of the test function): sum over sx, then z, then y
+        {
+            T P[2][2], Q[2][2], R[2][2];
+#pragma unroll
+            for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+                for (int sz = 0; sz < 2; ++sz) P[sy][sz] = S[c][0][0][sy][sz] + S[c][0][1][sy][sz];
+#pragma unroll
+            for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+                for (int z = 0; z < 2; ++z) Q[sy][z] = W(z, 0) * P[sy][0] + W(z, 1) * P[sy][1];
+#pragma unroll
+            for (int yy = 0; yy < 2; ++yy)
+#pragma unroll
+                for (int z = 0; z < 2; ++z) R[yy][z] = ih * (W(yy, 0) * Q[0][z] + W(yy, 1) * Q[1][z]);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int yy = 0; yy < 2; ++yy)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) y[3 * nidx(x, yy, z) + c] += x ? R[yy][z] : -R[yy][z];
+        }
+        // d = 1: sum over sy, then z, then x
+        {
+            T P[2][2], Q[2][2], R[2][2];
+#pragma unroll
+            for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+                for (int sz = 0; sz < 2; ++sz) P[sx][sz] = S[c][1][sx][0][sz] + S[c][1][sx][1][sz];
+#pragma unroll
+            for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+                for (int z = 0; z < 2; ++z) Q[sx][z] = W(z, 0) * P[sx][0] + W(z, 1) * P[sx][1];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int z = 0; z < 2; ++z) R[x][z] = ih * (W(x, 0) * Q[0][z] + W(x, 1) * Q[1][z]);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int yy = 0; yy < 2; ++yy)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) y[3 * nidx(x, yy, z) + c] += yy ? R[x][z] : -R[x][z];
+        }
+        // d = 2: sum over sz, then y, then x
+        {
+            T P[2][2], Q[2][2], R[2][2];
+#pragma unroll
+            for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+                for (int sy = 0; sy < 2; ++sy) P[sx][sy] = S[c][2][sx][sy][0] + S[c][2][sx][sy][1];
+#pragma unroll
+            for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+                for (int yy = 0; yy < 2; ++yy) Q[sx][yy] = W(yy, 0) * P[sx][0] + W(yy, 1) * P[sx][1];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int yy = 0; yy < 2; ++yy) R[x][yy] = ih * (W(x, 0) * Q[0][yy] + W(x, 1) * Q[1][yy]);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int yy = 0; yy < 2; ++yy)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) y[3 * nidx(x, yy, z) + c] += z ? R[x][yy] : -R[x][yy];
+        }
+    }
 }
 
 // Element-per-thread form: one thread per coarse element of the tile (and
@@ -117,7 +262,27 @@ __global__ __launch_bounds__(256, 2) void k_l1_ew(OpArgsT<T> a, Tile t, const T 
         T y[24];
 #pragma unroll
         for (int m = 0; m < 24; ++m) y[m] = T(0);
-        if (ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz) {
+        if (ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz && std::is_same<T, float>::value) {
+            T cx[8];
+#pragma unroll
+            for (int gp = 0; gp < 8; ++gp) {
+                const int is = gp >> 2, it = (gp >> 1) & 1, ie = gp & 1;
+                cx[gp] = CXf[(long)fny * fnz * (2 * ex + is) + (long)fny * (2 * ez + it) + 2 * ey + ie];
+            }
+            T u[2][2][2][3];  // [x][y][z][c]
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int yy = 0; yy < 2; ++yy)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) {
+                        const int h = 3 * (((ei + x) * HK + (ek + z)) * HJ + (ej + yy));
+                        u[x][yy][z][0] = su[h];
+                        u[x][yy][z][1] = su[h + 1];
+                        u[x][yy][z][2] = su[h + 2];
+                    }
+            l1_element_sf<T>(u, cx, y, lam2mu, lam, mu);
+        } else if (ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz) {
             T cx[8];
 #pragma unroll
             for (int gp = 0; gp < 8; ++gp) {
