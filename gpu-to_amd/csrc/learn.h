@@ -3,6 +3,7 @@
 // library's own forward and backward kernels.
 #pragma once
 
+#include <utility>
 #include <vector>
 
 #include "gputo_internal.h"
@@ -59,6 +60,10 @@ struct LearnState {
     bool use = true;
     int pol_state = 0, pol_off_left = 0, pol_backoff = 0, pol_on_count = 0;
     double pol_t_plain = 0.0;
+    // (seconds, relres) at the host's polls: of the last plain solve, and of
+    // the current solve; a trial that falls behind the first is cut short
+    std::vector<std::pair<double, double>> pol_ref, pol_cur;
+    bool pol_abort = false;
     long long bytes = 0;
     std::vector<void *> allocs;
 };
@@ -87,6 +92,9 @@ void learn_stage(gputo_handle *h, const HierView<float> &H, int l, const float *
 void learn_begin_solve(gputo_handle *h);                   // reset counters and the sample log; policy
 // after a solve of `seconds` (wall): counters, logged samples, policy update
 void learn_end_solve(gputo_handle *h, int *accepted, int *rejected, double seconds = -1.0);
+// progress of the current solve at a host poll (CG iteration `it`, wall
+// seconds since its start, relative residual)
+void learn_poll(gputo_handle *h, int it, double seconds, double relres);
 int learn_train(gputo_handle *h, int steps, double lr, double *loss_first, double *loss_last);
 
 }  // namespace gputo

@@ -983,8 +983,17 @@ void learn_stage(gputo_handle *h, const HierView<float> &H, int l, const float *
 // slightly different problems; a harder next design counts against the
 // stage (the bias is towards the plain V-cycle).  A new training (or new
 // parameters) restarts at REF.
+// A trial is also watched while it runs: at every host poll from CG
+// iteration kPolAbortIt on, its wall time is compared with the time the
+// reference solve took to reach the same relative residual (log-linear
+// between the reference's polls).  More than kPolAbortSlack times that, and
+// the stage is switched off for the rest of the solve (the flexible beta
+// absorbs the change) and the trial counts as failed: a losing trial costs
+// a few V-cycles instead of a whole solve.
 enum LearnPolicy { POL_REF = 0, POL_TRIAL = 1, POL_ON = 2, POL_OFF = 3 };
 constexpr int kPolRecheck = 8, kPolBackoff0 = 4, kPolBackoffMax = 64;
+constexpr int kPolAbortIt = 12;
+constexpr double kPolAbortSlack = 1.1;
 
 void learn_policy_reset(LearnState *L)
 {
@@ -993,6 +1002,7 @@ void learn_policy_reset(LearnState *L)
     L->pol_backoff = kPolBackoff0;
     L->pol_on_count = 0;
     L->pol_t_plain = 0.0;
+    L->pol_ref.clear();
 }
 
 void learn_begin_solve(gputo_handle *h)
@@ -1000,6 +1010,8 @@ void learn_begin_solve(gputo_handle *h)
     LearnState *L = h->ln;
     check_hip(hipMemsetAsync(L->count, 0, 4 * sizeof(int), h->stream), "memset");
     L->vc_count = 0;
+    L->pol_cur.clear();
+    L->pol_abort = false;
     if (h->prm.learned == 2 && L->trained)
         L->use = L->pol_state == POL_TRIAL || L->pol_state == POL_ON;
     else
@@ -1019,10 +1031,11 @@ void learn_end_solve(gputo_handle *h, int *accepted, int *rejected, double secon
     switch (L->pol_state) {
     case POL_REF:
         L->pol_t_plain = seconds;
+        L->pol_ref.swap(L->pol_cur);
         L->pol_state = POL_TRIAL;
         break;
     case POL_TRIAL:
-        if (seconds < L->pol_t_plain) {
+        if (!L->pol_abort && seconds < L->pol_t_plain) {
             L->pol_state = POL_ON;
             L->pol_on_count = 0;
             L->pol_backoff = kPolBackoff0;
@@ -1037,8 +1050,34 @@ void learn_end_solve(gputo_handle *h, int *accepted, int *rejected, double secon
         break;
     default:  // POL_OFF
         L->pol_t_plain = seconds;
+        L->pol_ref.swap(L->pol_cur);
         if (--L->pol_off_left <= 0) L->pol_state = POL_TRIAL;
         break;
+    }
+}
+
+void learn_poll(gputo_handle *h, int it, double seconds, double relres)
+{
+    LearnState *L = h->ln;
+    if (h->prm.learned != 2 || !L->trained) return;
+    L->pol_cur.emplace_back(seconds, relres);
+    if (L->pol_state != POL_TRIAL || !L->use || it < kPolAbortIt || !(relres > 0.0) || L->pol_ref.empty()) return;
+    // the reference's time to reach relres, from (0 s, 1) through its polls
+    double t0 = 0.0, r0 = 1.0, t_ref = -1.0;
+    for (const auto &pt : L->pol_ref) {
+        if (pt.second <= relres && pt.second > 0.0) {
+            const double span = std::log(r0) - std::log(pt.second);
+            const double f = span > 0.0 ? (std::log(r0) - std::log(relres)) / span : 1.0;
+            t_ref = t0 + std::min(1.0, std::max(0.0, f)) * (pt.first - t0);
+            break;
+        }
+        t0 = pt.first;
+        r0 = pt.second;
+    }
+    if (t_ref <= 0.0) return;  // the reference never got this far: the trial is ahead
+    if (seconds > kPolAbortSlack * t_ref) {
+        L->use = false;
+        L->pol_abort = true;
     }
 }
 

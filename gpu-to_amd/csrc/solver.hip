@@ -515,6 +515,11 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
         HIPCHK(hipMemcpyAsync(h->sc_host, h->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         if (h->sc_host->done || enq >= maxit) break;
+        // the learned stage's fallback watches a trial's progress
+        if (h->ln)
+            learn_poll(h, h->sc_host->it,
+                       std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count(),
+                       h->sc_host->relres);
     }
     const CgScalars &r = *h->sc_host;
     *relres = r.relres;
