@@ -233,7 +233,7 @@ void launch_diag0(hipStream_t s, const Grid &g, const double *Ke, const double *
 void launch_diagA(hipStream_t s, const Grid &g, const double *A, double *AD);
 // S = the operator of the packed element matrices A as node stencils
 void launch_stencil_build(hipStream_t s, const Grid &g, const double *A, double *S);
-constexpr int kStencilEntries = 243;  // 27 neighbours x 3 x 3 per node
+constexpr int kStencilEntries = 126;  // the own and 13 forward 3 x 3 blocks per node (the rest by symmetry)
 template <class T>
 void launch_restrict(hipStream_t s, const Grid &gc, const T *Rf, T *Fc, const uint8_t *maskc, const int *ctl,
                      int nb = 1);

@@ -432,20 +432,25 @@ __global__ __launch_bounds__(kBlock) void k_stored(OpArgsT<T> a)
 // ---------------------------------------------------------------------------
 // The same operator assembled per node: block b = 9 (di+1) + 3 (dk+1) +
 // (dj+1) of node (i, k, j) couples it to node (i+di, k+dk, j+dj) and holds
-// the sum of the 3x3 sub-blocks of every element containing both nodes,
-// S[(9 b + 3 r + c) nnod + node].  Zero blocks stand for neighbours outside
-// the grid.  Built once per setup from the element matrices (elements
-// summed in a fixed order), applied by k_stencil with 81 coalesced matrix
-// loads and 27 neighbour loads per node, where the element form reads 576
-// matrix entries and 192 neighbour values per node.
+// the sum of the 3x3 sub-blocks of every element containing both nodes.
+// Both nodes of a pair sum the same elements in the same order (increasing
+// element position), and the element matrices are symmetric, so block
+// 26 - b of node p + d is exactly the transpose of block b of node p: only
+// the node's own block and the 13 forward blocks (b >= 13) are stored,
+// S[(9 (b - 13) + 3 r + c) nnod + node], and a backward block is read
+// transposed at the neighbour.  Zero blocks stand for neighbours outside
+// the grid.  Built once per setup from the element matrices, applied by
+// k_stencil with 81 coalesced matrix loads and 27 neighbour loads per node,
+// where the element form reads 576 matrix entries and 192 neighbour values
+// per node.
 // ---------------------------------------------------------------------------
-// one thread per (node, block b = blockIdx.y)
+// one thread per (node, stored block b = 13 + blockIdx.y)
 __global__ void k_stencil_build(Grid g, const double *__restrict__ A, double *__restrict__ S)
 {
     const long nnod = g.nnod;
     const long nod = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (nod >= nnod) return;
-    const int b = blockIdx.y;
+    const int b = 13 + blockIdx.y;
     const int j = (int)(nod % g.nny), k = (int)((nod / g.nny) % g.nnz), i = (int)(nod / ((long)g.nny * g.nnz));
     const int di = b / 9 - 1, dk = (b / 3) % 3 - 1, dj = b % 3 - 1;
     double sb[9];
@@ -469,12 +474,12 @@ __global__ void k_stencil_build(Grid g, const double *__restrict__ A, double *__
                     for (int c = 0; c < 3; ++c) sb[3 * r + c] += A[(long)pidx(3 * la + r, 3 * lb + c) * g.nel + e];
             }
 #pragma unroll
-    for (int q = 0; q < 9; ++q) S[(long)(9 * b + q) * nnod + nod] = sb[q];
+    for (int q = 0; q < 9; ++q) S[(long)(9 * (b - 13) + q) * nnod + nod] = sb[q];
 }
 
 void launch_stencil_build(hipStream_t s, const Grid &g, const double *A, double *S)
 {
-    k_stencil_build<<<dim3((unsigned)((g.nnod + 255) / 256), 27), 256, 0, s>>>(g, A, S);
+    k_stencil_build<<<dim3((unsigned)((g.nnod + 255) / 256), 14), 256, 0, s>>>(g, A, S);
 }
 
 // Three waves per 64 consecutive nodes: wave w sums the 9 blocks of the
@@ -485,6 +490,64 @@ void launch_stencil_build(hipStream_t s, const Grid &g, const double *A, double 
 // every load is issued in the first one: wave 1 fetches its epilogue
 // operands (mask, F, AD, own u) with the stencil, and the skip flag is read
 // alongside and tested once they are in flight.
+// Stencil products of neighbour plane di = W - 1 for one node (lane): the
+// blocks b = 9W + 3 (dk+1) + (dj+1), read at the node when b >= 13 and
+// transposed at the neighbour (its block 26 - b) when b < 13.  Returns false
+// when the solve has converged (the flag is read once the first loads are
+// in flight).
+template <int W, class T, int MODE>
+__device__ __forceinline__ bool stencil_plane(const OpArgsT<T> &a, long p, long pc, int j, int k, long pw, T &y0,
+                                              T &y1, T &y2)
+{
+    const long nnod = a.g.nnod;
+    constexpr int NR = sizeof(T) == 8 ? 1 : 3;  // dk rows per batch
+#pragma unroll
+    for (int r0 = 0; r0 < 3; r0 += NR) {
+        T sv[NR][3][9], uu[NR][3][3];
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr) {
+            const int dk = r0 + rr - 1;
+#pragma unroll
+            for (int c3 = 0; c3 < 3; ++c3) {
+                const int dj = c3 - 1;
+                const int kk = k + dk, jj = j + dj;
+                const bool ok = kk >= 0 && kk < a.g.nnz && jj >= 0 && jj < a.g.nny;
+                // outside the grid the block is zero: read any valid node
+                const long nb = ok ? pw + (dk * a.g.nny + dj) : pc;
+                const int b = 9 * W + 3 * (r0 + rr) + c3;
+                if (b >= 13) {
+                    const T *__restrict__ Sb = a.S + (long)(9 * (b - 13)) * nnod + p;
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) sv[rr][c3][q] = Sb[(long)q * nnod];
+                } else {
+                    const T *__restrict__ Sb = a.S + (long)(9 * (13 - b)) * nnod + nb;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const T v = Sb[(long)(3 * c + r) * nnod];
+                            sv[rr][c3][3 * r + c] = ok ? v : T(0);
+                        }
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) uu[rr][c3][c] = stored_u<T, MODE>(a, 3 * nb + c);
+            }
+        }
+        if (r0 == 0 && a.ctl && a.ctl[0]) return false;  // converged: nothing is written
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr)
+#pragma unroll
+            for (int c3 = 0; c3 < 3; ++c3) {
+                const T *m = sv[rr][c3];
+                const T *u = uu[rr][c3];
+                y0 += m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
+                y1 += m[3] * u[0] + m[4] * u[1] + m[5] * u[2];
+                y2 += m[6] * u[0] + m[7] * u[1] + m[8] * u[2];
+            }
+    }
+    return true;
+}
+
 template <class T, int MODE>
 __global__ __launch_bounds__(192) void k_stencil(OpArgsT<T> a)
 {
@@ -516,40 +579,10 @@ __global__ __launch_bounds__(192) void k_stencil(OpArgsT<T> a)
     }
     T y0 = T(0), y1 = T(0), y2 = T(0);
     if (live && ii >= 0 && ii < g.nnx) {
-        const T *__restrict__ Sp = a.S + (long)(81 * w) * nnod + p;  // blocks 9w .. 9w+8
-        constexpr int NR = sizeof(T) == 8 ? 1 : 3;  // dk rows per batch
-#pragma unroll
-        for (int r0 = 0; r0 < 3; r0 += NR) {
-            T sv[NR][3][9], uu[NR][3][3];
-#pragma unroll
-            for (int rr = 0; rr < NR; ++rr) {
-                const int dk = r0 + rr - 1;
-#pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) {
-                    const int dj = c3 - 1;
-                    const int kk = k + dk, jj = j + dj;
-                    const bool ok = kk >= 0 && kk < g.nnz && jj >= 0 && jj < g.nny;
-                    // outside the grid the stencil block is zero: read any valid node
-                    const long nb = ok ? pw + (dk * g.nny + dj) : pc;
-                    const int bl = 3 * (r0 + rr) + c3;
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) sv[rr][c3][q] = Sp[(long)(9 * bl + q) * nnod];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) uu[rr][c3][c] = stored_u<T, MODE>(a, 3 * nb + c);
-                }
-            }
-            if (r0 == 0 && a.ctl && a.ctl[0]) return;  // converged: nothing is written
-#pragma unroll
-            for (int rr = 0; rr < NR; ++rr)
-#pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) {
-                    const T *m = sv[rr][c3];
-                    const T *u = uu[rr][c3];
-                    y0 += m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
-                    y1 += m[3] * u[0] + m[4] * u[1] + m[5] * u[2];
-                    y2 += m[6] * u[0] + m[7] * u[1] + m[8] * u[2];
-                }
-        }
+        const bool go = w == 0   ? stencil_plane<0, T, MODE>(a, p, pc, j, k, pw, y0, y1, y2)
+                        : w == 1 ? stencil_plane<1, T, MODE>(a, p, pc, j, k, pw, y0, y1, y2)
+                                 : stencil_plane<2, T, MODE>(a, p, pc, j, k, pw, y0, y1, y2);
+        if (!go) return;
     }
     if (a.ctl && a.ctl[0]) return;
     if (w != 1) {
