@@ -73,6 +73,7 @@ struct OpArgsT {
     T *out;                // result vector
     T *out2;               // second result (prerelax+defect writes U and R)
     double *partials;      // per-block partial sums (fused dot), may be null
+    const float *Uf;       // OP_MATVEC_UF: the input vector in FP32
     const int *ctl;        // ctl[0] != 0: skip (converged)
     int nb;                // right-hand sides of a batched launch (0 or 1: one)
     double omega, prerelax;
@@ -87,7 +88,8 @@ enum OpMode {
     OP_DEFECT = 1,      // out = F - K U, masked
     OP_JACOBI = 2,      // out = (1-w) U + w (r + AD U)/AD with r = F - K U masked; partials = sum F.out
     OP_PRE_JACOBI = 3,  // as OP_JACOBI with U = prerelax*F/AD formed on the fly
-    OP_PRE_DEFECT = 4   // U = prerelax*F/AD written to out2, out = F - K U masked
+    OP_PRE_DEFECT = 4,  // U = prerelax*F/AD written to out2, out = F - K U masked
+    OP_MATVEC_UF = 5    // as OP_MATVEC with the input Uf in FP32 (level-0 node-gather, FP64)
 };
 
 // Scalars of one device-resident PCG (all on the device, one struct).
@@ -228,6 +230,7 @@ namespace gputo {
 
 // ---- launch helpers (kernels.hip) ------------------------------------
 template <class T> void launch_op0(hipStream_t s, int mode, const OpArgsT<T> &a);
+void launch_op0_uf(hipStream_t s, const OpArgs &a);  // OP_MATVEC_UF
 template <class T> void launch_opA(hipStream_t s, int mode, const OpArgsT<T> &a);
 void launch_diag0(hipStream_t s, const Grid &g, const double *Ke, const double *CX, double *AD);
 void launch_diagA(hipStream_t s, const Grid &g, const double *A, double *AD);
@@ -299,12 +302,17 @@ void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int n
 template <class T>
 void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials,
                          float *R32 = nullptr, int nb = 1);
+// the same with the search direction in FP32 (widened exactly)
+void launch_cg_update_ur(hipStream_t s, double *U, double *R, const float *P, const double *Q, long n, CgScalars *sc,
+                         double *partials, float *R32);
 void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit,
                      int nb = 1);
 void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit, int nb = 1);
 template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc, int nb = 1);
 // P = Z + beta P with the FP32 output of the mixed V-cycle (widened exactly)
 void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc, int nb = 1);
+// the same with P kept in FP32: formed in FP64, rounded once
+void launch_cg_update_p(hipStream_t s, float *P, const float *Z, long n, CgScalars *sc, int nb);
 // flexible beta (partials needs 2*kRedBlocks entries); z FP64, or the
 // mixed V-cycle's FP32 output (products in FP64, the same values)
 template <class TZ>

@@ -97,6 +97,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
                 T x;
                 if (MODE == OP_PRE_JACOBI || MODE == OP_PRE_DEFECT)
                     x = T(a.prerelax) * a.F[dof] / a.AD[dof];
+                else if (MODE == OP_MATVEC_UF)
+                    x = T(a.Uf[dof]);
                 else
                     x = a.U[dof];
                 v[u][cc] = ok ? x : T(0);
@@ -256,13 +258,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(OpArgsT<T> a)
             T fv[3] = {T(0), T(0), T(0)}, adv[3] = {T(1), T(1), T(1)};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                if (MODE != OP_MATVEC) fv[c] = a.F[dof + c];
+                if (MODE != OP_MATVEC && MODE != OP_MATVEC_UF) fv[c] = a.F[dof + c];
                 if (MODE == OP_JACOBI || MODE == OP_PRE_JACOBI) adv[c] = a.AD[dof + c];
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const bool fixed = (m >> c) & 1u;
-                if (MODE == OP_MATVEC) {
+                if (MODE == OP_MATVEC || MODE == OP_MATVEC_UF) {
                     const T v = fixed ? T(0) : y[c];
                     a.out[dof + c] = v;
                     part += (double)su[c][hc] * (double)v;
@@ -637,6 +639,15 @@ void launch_opA(hipStream_t s, int mode, const OpArgsT<T> &a)
     }
 }
 template void launch_op0<double>(hipStream_t, int, const OpArgsT<double> &);
+
+void launch_op0_uf(hipStream_t s, const OpArgs &a)
+{
+    const dim3 grid((unsigned)xcd_launch(a.t.blocks()), 1, batch_dim(a.nb)), block(kBlock);
+    if (a.kc)
+        k_gather<double, OP_MATVEC_UF, true><<<grid, block, 0, s>>>(a);
+    else
+        k_gather<double, OP_MATVEC_UF, false><<<grid, block, 0, s>>>(a);
+}
 template void launch_op0<float>(hipStream_t, int, const OpArgsT<float> &);
 template void launch_opA<double>(hipStream_t, int, const OpArgsT<double> &);
 template void launch_opA<float>(hipStream_t, int, const OpArgsT<float> &);
@@ -1015,8 +1026,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars *sc, const double
     }
 }
 
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ P,
+template <class T, class TP>
+__global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *__restrict__ R, const TP *__restrict__ P,
                                                         const T *__restrict__ Q, long n, const CgScalars *sc,
                                                         double *partials, float *__restrict__ R32)
 {
@@ -1035,8 +1046,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *_
     for (long i = i0; i < n; i += 2 * stride) {
         const bool two = i + stride < n;
         const long j = two ? i + stride : i;  // in bounds; unused when !two
-        const T u0 = U[i], p0 = P[i], r0 = R[i], q0 = Q[i];
-        const T u1 = U[j], p1 = P[j], r1 = R[j], q1 = Q[j];
+        const T u0 = U[i], p0 = T(P[i]), r0 = R[i], q0 = Q[i];
+        const T u1 = U[j], p1 = T(P[j]), r1 = R[j], q1 = Q[j];
         if (i == i0 && sc->done) return;  // read with alpha while the loads are in flight
         const T ra = r0 - al * q0, rb = r1 - al * q1;
         U[i] = u0 + al * p0;
@@ -1192,7 +1203,12 @@ template <class T>
 void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials,
                          float *R32, int nb)
 {
-    k_cg_update_ur<T><<<dim3(kRedBlocks, 1, batch_dim(nb)), kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
+    k_cg_update_ur<T, T><<<dim3(kRedBlocks, 1, batch_dim(nb)), kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
+}
+void launch_cg_update_ur(hipStream_t s, double *U, double *R, const float *P, const double *Q, long n, CgScalars *sc,
+                         double *partials, float *R32)
+{
+    k_cg_update_ur<double, float><<<kRedBlocks, kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
 }
 template void launch_cg_update_ur<double>(hipStream_t, double *, double *, const double *, const double *, long,
                                           CgScalars *, double *, float *, int);
@@ -1213,6 +1229,30 @@ void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc, 
 }
 template void launch_cg_update_p<double>(hipStream_t, double *, const double *, long, CgScalars *, int);
 template void launch_cg_update_p<float>(hipStream_t, float *, const float *, long, CgScalars *, int);
+// P = Z + beta P with P kept in FP32: formed in FP64 from the FP32 values,
+// rounded once (two elements per pass, as k_cg_update_p)
+__global__ __launch_bounds__(kBlock) void k_cg_update_p32(float *__restrict__ P, const float *__restrict__ Z, long n,
+                                                         const CgScalars *sc)
+{
+    sc += blockIdx.z;
+    P = batch_ptr(P, n);
+    Z = batch_ptr(Z, n);
+    const double be = sc->beta;
+    const long stride = (long)gridDim.x * kBlock, i0 = (long)blockIdx.x * kBlock + threadIdx.x;
+    for (long i = i0; i < n; i += 2 * stride) {
+        const bool two = i + stride < n;
+        const long j = two ? i + stride : i;  // in bounds; unused when !two
+        const float p0 = P[i], p1 = P[j], z0 = Z[i], z1 = Z[j];
+        if (i == i0 && sc->done) return;
+        P[i] = (float)fma(be, (double)p0, (double)z0);
+        if (two) P[j] = (float)fma(be, (double)p1, (double)z1);
+    }
+}
+void launch_cg_update_p(hipStream_t s, float *P, const float *Z, long n, CgScalars *sc, int nb)
+{
+    k_cg_update_p32<<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
+}
+
 void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc, int nb)
 {
     k_cg_update_p<double, float><<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);

@@ -458,6 +458,11 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     // ... whose FP32 result the CG then reads directly (not for a periodic
     // cell, whose preconditioned residual is folded in FP64)
     const float *z32 = (r32 && !h->prm.periodic) ? mixed_z32(h) : nullptr;
+    // ... and then keeps the search direction in FP32: p = z + beta p formed
+    // in FP64 and rounded once; u, r and the matvec take exactly the stored
+    // values, so the recurrences stay consistent (the FP64 residual decides
+    // convergence), at half the traffic of p
+    float *p32 = z32 ? mixed_p32(h) : nullptr;
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
@@ -470,6 +475,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     launch_cg_init(s, h->sc, h->partials, kRedBlocks);
     precond_n(h->R, h->P, false);
     launch_copy(s, h->P, h->Z, n, ctl);
+    if (p32) launch_copy<double, float>(s, h->P, p32, n, ctl);  // exact: z is the V-cycle's FP32 output
     launch_dot_partial(s, h->R, h->Z, n, h->partials, ctl);
     launch_cg_rho0(s, h->sc, h->partials, kRedBlocks);
     const int chunk = 4;
@@ -481,12 +487,20 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
             a.U = h->P;
             a.out = h->Q;
             a.partials = h->partials;
-            launch_op0(s, OP_MATVEC, a);
+            if (p32) {
+                a.Uf = p32;
+                launch_op0_uf(s, a);
+            } else {
+                launch_op0(s, OP_MATVEC, a);
+            }
             // p.q over the full grid is the periodic p.Kp before the fold
             // (p is equal on images); the fold completes q for the update
             if (per) launch_per_fold<double>(s, L0.g, h->Q, 0, ctl);
             launch_cg_alpha(s, h->sc, h->partials, nmv);
-            launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials, r32);
+            if (p32)
+                launch_cg_update_ur(s, U, h->R, p32, h->Q, n, h->sc, h->partials, r32);
+            else
+                launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials, r32);
             launch_cg_check(s, h->sc, h->partials, kRedBlocks, tol, maxit);
             if (z32) {
                 // the FP32 V-cycle's result feeds the FP64 dots and the P
@@ -499,7 +513,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
                     launch_dot_partial<double, float>(s, h->R, z32, n, h->partials, ctl);
                     launch_cg_beta(s, h->sc, h->partials, kRedBlocks, maxit);
                 }
-                launch_cg_update_p(s, h->P, z32, n, h->sc);
+                launch_cg_update_p(s, p32, z32, n, h->sc, 1);
                 continue;
             }
             precond_n(h->R, h->Z, r32 != nullptr);
