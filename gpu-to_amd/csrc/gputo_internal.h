@@ -302,17 +302,23 @@ void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int n
 template <class T>
 void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials,
                          float *R32 = nullptr, int nb = 1);
-// the same with the search direction in FP32 (widened exactly)
-void launch_cg_update_ur(hipStream_t s, double *U, double *R, const float *P, const double *Q, long n, CgScalars *sc,
-                         double *partials, float *R32);
 void launch_cg_check(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double tol, int maxit,
                      int nb = 1);
 void launch_cg_beta(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int maxit, int nb = 1);
 template <class T> void launch_cg_update_p(hipStream_t s, T *P, const T *Z, long n, CgScalars *sc, int nb = 1);
 // P = Z + beta P with the FP32 output of the mixed V-cycle (widened exactly)
 void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc, int nb = 1);
-// the same with P kept in FP32: formed in FP64, rounded once
-void launch_cg_update_p(hipStream_t s, float *P, const float *Z, long n, CgScalars *sc, int nb);
+// the same with P kept in FP32 (Pout = Z + beta Pin, formed in FP64 and
+// rounded once; Pout may be Pin)
+void launch_cg_update_p32(hipStream_t s, float *Pout, const float *Pin, const float *Z, long n, CgScalars *sc);
+// PCG with deferred u: the iterations' alphas are logged (0 once converged)
+// and u += sum alpha_t p_t is applied every kCgLog iterations from the
+// kept FP32 directions
+constexpr int kCgLog = 16;
+void launch_cg_alpha_log(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double *alog);
+void launch_cg_update_r(hipStream_t s, double *R, const double *Q, long n, CgScalars *sc, double *partials,
+                        float *R32);
+void launch_cg_flush_u(hipStream_t s, double *U, const float *P, long n, const double *alog, int m);
 // flexible beta (partials needs 2*kRedBlocks entries); z FP64, or the
 // mixed V-cycle's FP32 output (products in FP64, the same values)
 template <class TZ>

@@ -1008,16 +1008,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_rho0(CgScalars *sc, const double 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars *sc, const double *p, int n)
+// alog (optional): the iteration's slot of the alpha log, 0 once converged
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars *sc, const double *p, int n, double *alog)
 {
     sc += blockIdx.z;
     p += (long)blockIdx.z * n;
     const double rho_p = sc->rho_p;
     const double pq = sum_partials(p, n);
-    if (sc->done) return;
+    if (sc->done) {
+        if (alog && threadIdx.x == 0) *alog = 0.0;
+        return;
+    }
     if (threadIdx.x == 0) {
         sc->pq = pq;
         sc->alpha = rho_p / pq;
+        if (alog) *alog = sc->alpha;
         if (!(fabs(sc->alpha) < INFINITY)) {
             sc->bad = 1;
             sc->done = 1;
@@ -1197,18 +1202,76 @@ void launch_cg_rho0(hipStream_t s, CgScalars *sc, const double *partials, int np
 }
 void launch_cg_alpha(hipStream_t s, CgScalars *sc, const double *partials, int nparts, int nb)
 {
-    k_cg_alpha<<<dim3(1, 1, batch_dim(nb)), kBlock, 0, s>>>(sc, partials, nparts);
+    k_cg_alpha<<<dim3(1, 1, batch_dim(nb)), kBlock, 0, s>>>(sc, partials, nparts, nullptr);
+}
+void launch_cg_alpha_log(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double *alog)
+{
+    k_cg_alpha<<<1, kBlock, 0, s>>>(sc, partials, nparts, alog);
+}
+
+// Residual update of the PCG with deferred u (u += alpha p is applied later
+// from the logged alphas and directions): R -= alpha Q, R32 = R in FP32,
+// partials of R.R, two elements per pass as k_cg_update_ur.
+__global__ __launch_bounds__(kBlock) void k_cg_update_r(double *__restrict__ R, const double *__restrict__ Q, long n,
+                                                       const CgScalars *sc, double *partials,
+                                                       float *__restrict__ R32)
+{
+    const double al = sc->alpha;
+    const long stride = (long)kRedBlocks * kBlock;
+    const long i0 = (long)blockIdx.x * kBlock + threadIdx.x;
+    double s = 0.0;
+    for (long i = i0; i < n; i += 2 * stride) {
+        const bool two = i + stride < n;
+        const long j = two ? i + stride : i;  // in bounds; unused when !two
+        const double r0 = R[i], q0 = Q[i], r1 = R[j], q1 = Q[j];
+        if (i == i0 && sc->done) return;  // read with alpha while the loads are in flight
+        const double ra = r0 - al * q0, rb = r1 - al * q1;
+        R[i] = ra;
+        R32[i] = (float)ra;
+        s += ra * ra;
+        if (two) {
+            R[j] = rb;
+            R32[j] = (float)rb;
+            s += rb * rb;
+        }
+    }
+    if (sc->done) return;  // every thread tests it before the barrier
+    s = block_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+void launch_cg_update_r(hipStream_t s, double *R, const double *Q, long n, CgScalars *sc, double *partials,
+                        float *R32)
+{
+    k_cg_update_r<<<kRedBlocks, kBlock, 0, s>>>(R, Q, n, sc, partials, R32);
+}
+
+// U += alpha_0 P_0 + ... + alpha_{m-1} P_{m-1}, term by term in that order
+// (the same fused multiply-adds as one update per iteration).  Converged
+// iterations logged alpha 0; their directions are finite, so they add 0.
+// One element per thread, its m + 1 loads all in flight together.
+__global__ __launch_bounds__(kBlock) void k_cg_flush_u(double *__restrict__ U, const float *__restrict__ P, long n,
+                                                      const double *__restrict__ alog, int m)
+{
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float pv[kCgLog];
+#pragma unroll
+    for (int t = 0; t < kCgLog; ++t) pv[t] = t < m ? P[(long)t * n + i] : 0.0f;
+    double u = U[i];
+#pragma unroll
+    for (int t = 0; t < kCgLog; ++t)
+        if (t < m) u = fma(alog[t], (double)pv[t], u);
+    U[i] = u;
+}
+void launch_cg_flush_u(hipStream_t s, double *U, const float *P, long n, const double *alog, int m)
+{
+    if (m > 0) k_cg_flush_u<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(U, P, n, alog, m);
 }
 template <class T>
 void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials,
                          float *R32, int nb)
 {
     k_cg_update_ur<T, T><<<dim3(kRedBlocks, 1, batch_dim(nb)), kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
-}
-void launch_cg_update_ur(hipStream_t s, double *U, double *R, const float *P, const double *Q, long n, CgScalars *sc,
-                         double *partials, float *R32)
-{
-    k_cg_update_ur<double, float><<<kRedBlocks, kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
 }
 template void launch_cg_update_ur<double>(hipStream_t, double *, double *, const double *, const double *, long,
                                           CgScalars *, double *, float *, int);
@@ -1231,26 +1294,24 @@ template void launch_cg_update_p<double>(hipStream_t, double *, const double *, 
 template void launch_cg_update_p<float>(hipStream_t, float *, const float *, long, CgScalars *, int);
 // P = Z + beta P with P kept in FP32: formed in FP64 from the FP32 values,
 // rounded once (two elements per pass, as k_cg_update_p)
-__global__ __launch_bounds__(kBlock) void k_cg_update_p32(float *__restrict__ P, const float *__restrict__ Z, long n,
-                                                         const CgScalars *sc)
+// (Pout = Z + beta Pin; Pout may be Pin)
+__global__ __launch_bounds__(kBlock) void k_cg_update_p32(float *Pout, const float *Pin, const float *__restrict__ Z,
+                                                         long n, const CgScalars *sc)
 {
-    sc += blockIdx.z;
-    P = batch_ptr(P, n);
-    Z = batch_ptr(Z, n);
     const double be = sc->beta;
     const long stride = (long)gridDim.x * kBlock, i0 = (long)blockIdx.x * kBlock + threadIdx.x;
     for (long i = i0; i < n; i += 2 * stride) {
         const bool two = i + stride < n;
         const long j = two ? i + stride : i;  // in bounds; unused when !two
-        const float p0 = P[i], p1 = P[j], z0 = Z[i], z1 = Z[j];
+        const float p0 = Pin[i], p1 = Pin[j], z0 = Z[i], z1 = Z[j];
         if (i == i0 && sc->done) return;
-        P[i] = (float)fma(be, (double)p0, (double)z0);
-        if (two) P[j] = (float)fma(be, (double)p1, (double)z1);
+        Pout[i] = (float)fma(be, (double)p0, (double)z0);
+        if (two) Pout[j] = (float)fma(be, (double)p1, (double)z1);
     }
 }
-void launch_cg_update_p(hipStream_t s, float *P, const float *Z, long n, CgScalars *sc, int nb)
+void launch_cg_update_p32(hipStream_t s, float *Pout, const float *Pin, const float *Z, long n, CgScalars *sc)
 {
-    k_cg_update_p32<<<dim3(grid_for(n), 1, batch_dim(nb)), kBlock, 0, s>>>(P, Z, n, sc);
+    k_cg_update_p32<<<grid_for(n), kBlock, 0, s>>>(Pout, Pin, Z, n, sc);
 }
 
 void launch_cg_update_p(hipStream_t s, double *P, const float *Z, long n, CgScalars *sc, int nb)

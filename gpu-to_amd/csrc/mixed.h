@@ -18,7 +18,8 @@ struct MixedState {
     float *CXl32[kMaxLevels] = {};     // FP32 element scales of matrix-free coarse levels
     float *Ke32 = nullptr, *CX32 = nullptr;
     float *X = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr, *Z = nullptr;  // inner CG vectors
-    float *Pcg = nullptr;              // the FP64 PCG's search direction, kept in FP32
+    float *Pcg = nullptr;              // the FP64 PCG's last kCgLog search directions, kept in FP32
+    double *alog = nullptr;            // their alphas (0 for converged iterations)
     CgScalars *sc = nullptr, *sc_host = nullptr;
     double *norm = nullptr;            // ||r|| of the outer residual (device)
     double norm_host[2] = {0, 0};
@@ -37,7 +38,8 @@ void mixed_begin(gputo_handle *h);                                        // FP3
 void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, bool r32_ready = false);
 float *mixed_r32(gputo_handle *h);        // the FP32 right-hand side of the V-cycle
 const float *mixed_z32(gputo_handle *h);  // the FP32 result of the last V-cycle
-float *mixed_p32(gputo_handle *h);        // the PCG's FP32 search direction
+float *mixed_p32(gputo_handle *h);        // the PCG's FP32 search directions (kCgLog slots of ndof)
+double *mixed_alog(gputo_handle *h);      // and their alphas
 void mixed_end(gputo_handle *h, int nit, int converged);
 int mixed_apply_l0(gputo_handle *h, int kind, const double *u, double *y);
 int mixed_bench_operator(gputo_handle *h, int kind, int reps, double *avg_ms);

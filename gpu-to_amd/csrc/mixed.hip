@@ -59,7 +59,9 @@ void mixed_create(gputo_handle *h)
     m->P = malloc_dev<float>(m, g0.ndof);
     m->Q = malloc_dev<float>(m, g0.ndof);
     m->Z = malloc_dev<float>(m, g0.ndof);
-    m->Pcg = malloc_dev<float>(m, g0.ndof);
+    m->Pcg = malloc_dev<float>(m, (long)kCgLog * g0.ndof);
+    launch_fill<float>(h->stream, m->Pcg, (long)kCgLog * g0.ndof, 0.0);  // finite: unused slots add 0 * p
+    m->alog = malloc_dev<double>(m, kCgLog);
     m->sc = malloc_dev<CgScalars>(m, 1);
     m->norm = malloc_dev<double>(m, 2);
     check_hip(hipHostMalloc((void **)&m->sc_host, sizeof(CgScalars), hipHostMallocDefault), "hipHostMalloc");
@@ -144,6 +146,7 @@ void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, b
 float *mixed_r32(gputo_handle *h) { return h->mx->R; }
 const float *mixed_z32(gputo_handle *h) { return h->mx->Z; }
 float *mixed_p32(gputo_handle *h) { return h->mx->Pcg; }
+double *mixed_alog(gputo_handle *h) { return h->mx->alog; }
 
 // Start of a mixed-precision solve: FP32 copies of the current hierarchy.
 void mixed_begin(gputo_handle *h)

@@ -461,8 +461,14 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     // ... and then keeps the search direction in FP32: p = z + beta p formed
     // in FP64 and rounded once; u, r and the matvec take exactly the stored
     // values, so the recurrences stay consistent (the FP64 residual decides
-    // convergence), at half the traffic of p
+    // convergence), at half the traffic of p.  u += alpha p is deferred:
+    // the last kCgLog directions stay in a ring with their alphas, and u
+    // takes them every kCgLog iterations in one pass (the same fused
+    // multiply-adds in the same order), so u is not read and written by
+    // every iteration.
     float *p32 = z32 ? mixed_p32(h) : nullptr;
+    double *alog = z32 ? mixed_alog(h) : nullptr;
+    int folded = 0;  // iterations whose alpha p is in U
     Level &L0 = h->L[0];
     const long n = L0.g.ndof;
     const int *ctl = &h->sc->done;
@@ -482,13 +488,15 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     int enq = 0;
     for (;;) {
         for (int c = 0; c < chunk && enq < maxit; ++c, ++enq) {
+            const int sl = enq % kCgLog;  // ring slot of this iteration's direction
+            float *pk = p32 ? p32 + (long)sl * n : nullptr;
             OpArgs a = op_args(h, 0);
             a.cx_floor = 0.0;  // the CG matvec keeps every element (multAV0c, mgcg9.cu:791-848)
             a.U = h->P;
             a.out = h->Q;
             a.partials = h->partials;
             if (p32) {
-                a.Uf = p32;
+                a.Uf = pk;
                 launch_op0_uf(s, a);
             } else {
                 launch_op0(s, OP_MATVEC, a);
@@ -496,11 +504,17 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
             // p.q over the full grid is the periodic p.Kp before the fold
             // (p is equal on images); the fold completes q for the update
             if (per) launch_per_fold<double>(s, L0.g, h->Q, 0, ctl);
-            launch_cg_alpha(s, h->sc, h->partials, nmv);
-            if (p32)
-                launch_cg_update_ur(s, U, h->R, p32, h->Q, n, h->sc, h->partials, r32);
-            else
+            if (p32) {
+                launch_cg_alpha_log(s, h->sc, h->partials, nmv, alog + sl);
+                launch_cg_update_r(s, h->R, h->Q, n, h->sc, h->partials, r32);
+                if (sl == kCgLog - 1) {  // before slot 0 is overwritten by this iteration's p update
+                    launch_cg_flush_u(s, U, p32, n, alog, kCgLog);
+                    folded = enq + 1;
+                }
+            } else {
+                launch_cg_alpha(s, h->sc, h->partials, nmv);
                 launch_cg_update_ur(s, U, h->R, h->P, h->Q, n, h->sc, h->partials, r32);
+            }
             launch_cg_check(s, h->sc, h->partials, kRedBlocks, tol, maxit);
             if (z32) {
                 // the FP32 V-cycle's result feeds the FP64 dots and the P
@@ -513,7 +527,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
                     launch_dot_partial<double, float>(s, h->R, z32, n, h->partials, ctl);
                     launch_cg_beta(s, h->sc, h->partials, kRedBlocks, maxit);
                 }
-                launch_cg_update_p(s, p32, z32, n, h->sc, 1);
+                launch_cg_update_p32(s, p32 + (long)((sl + 1) % kCgLog) * n, pk, z32, n, h->sc);
                 continue;
             }
             precond_n(h->R, h->Z, r32 != nullptr);
@@ -528,7 +542,13 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(h->sc_host, h->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        if (h->sc_host->done || enq >= maxit) break;
+        if (h->sc_host->done || enq >= maxit) {
+            if (p32) {  // the directions since the last pass (converged ones logged alpha 0)
+                launch_cg_flush_u(s, U, p32, n, alog, enq - folded);
+                HIPCHK(hipStreamSynchronize(s));
+            }
+            break;
+        }
         // the learned stage's fallback watches a trial's progress
         if (h->ln)
             learn_poll(h, h->sc_host->it,
