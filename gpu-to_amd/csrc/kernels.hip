@@ -1031,8 +1031,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars *sc, const double
     }
 }
 
-template <class T, class TP>
-__global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *__restrict__ R, const TP *__restrict__ P,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ P,
                                                         const T *__restrict__ Q, long n, const CgScalars *sc,
                                                         double *partials, float *__restrict__ R32)
 {
@@ -1051,8 +1051,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_ur(T *__restrict__ U, T *_
     for (long i = i0; i < n; i += 2 * stride) {
         const bool two = i + stride < n;
         const long j = two ? i + stride : i;  // in bounds; unused when !two
-        const T u0 = U[i], p0 = T(P[i]), r0 = R[i], q0 = Q[i];
-        const T u1 = U[j], p1 = T(P[j]), r1 = R[j], q1 = Q[j];
+        const T u0 = U[i], p0 = P[i], r0 = R[i], q0 = Q[i];
+        const T u1 = U[j], p1 = P[j], r1 = R[j], q1 = Q[j];
         if (i == i0 && sc->done) return;  // read with alpha while the loads are in flight
         const T ra = r0 - al * q0, rb = r1 - al * q1;
         U[i] = u0 + al * p0;
@@ -1271,7 +1271,7 @@ template <class T>
 void launch_cg_update_ur(hipStream_t s, T *U, T *R, const T *P, const T *Q, long n, CgScalars *sc, double *partials,
                          float *R32, int nb)
 {
-    k_cg_update_ur<T, T><<<dim3(kRedBlocks, 1, batch_dim(nb)), kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
+    k_cg_update_ur<T><<<dim3(kRedBlocks, 1, batch_dim(nb)), kBlock, 0, s>>>(U, R, P, Q, n, sc, partials, R32);
 }
 template void launch_cg_update_ur<double>(hipStream_t, double *, double *, const double *, const double *, long,
                                           CgScalars *, double *, float *, int);
