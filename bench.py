@@ -64,6 +64,13 @@ VARIANTS = {
 }
 
 
+def variant_of(name):
+    """The sweep/kernel tuple of a variant name; its suffixes are "+learned"
+    (the learned level-2 stage, learned = 2) and "+graph" (the V-cycle
+    replayed from a recorded HIP graph, graph_vcycles = 1)."""
+    return VARIANTS[name.replace("+learned", "").replace("+graph", "")]
+
+
 def level2_sweeps(v):
     """(pre, post) sweeps of the levels from 2 on (0, 0: the preset's)."""
     return (v[7], v[8]) if len(v) > 7 else (0, 0)
@@ -94,7 +101,7 @@ def pmc_traffic(kernel):
 def level0_sweeps(name):
     """'pre+post' Jacobi sweeps on levels 0 and 1 of a mixed variant's FP32
     V-cycle (None for the FP64 variant, whose V-cycle is the preset's)."""
-    v = VARIANTS[name.replace("+learned", "")]
+    v = variant_of(name)
     if v[0] == 0:
         return None
     n1 = level1_sweeps(v)
@@ -157,14 +164,15 @@ def run_variant(gputo, name, args, Ke, F, fixed, x0):
     defects its solve logged; that training time is counted into the timed
     steps' cost (`s_per_iter` includes it, amortised over them).  The state
     (xPhys, U, compliance) is kept at design iterations 1, 50 and 100."""
-    v = VARIANTS[name.replace("+learned", "")]
+    v = variant_of(name)
     prec, kernel, npre0, npost0, lag = v[:5]
     npre1, npost1 = level1_sweeps(v)
     npre2, npost2 = level2_sweeps(v)
-    learned = name.endswith("+learned")
+    learned = "+learned" in name
     s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=2 if learned else 0,
                      l0_kernel=kernel, mixed_npre0=npre0, mixed_npost0=npost0, mixed_npre1=npre1,
-                     mixed_npost1=npost1, mixed_npre2=npre2, mixed_npost2=npost2, coarse_lag=lag)
+                     mixed_npost1=npost1, mixed_npre2=npre2, mixed_npost2=npost2, coarse_lag=lag,
+                     graph_vcycles="+graph" in name)
     pre = pretrain(gputo, s, args, Ke, fixed, name) if learned else None
     run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
                        cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
@@ -424,7 +432,7 @@ def main():
     # ridge of both precisions: priced against the compute peak of their
     # type.  The MFMA kernels (2-5) have the element products on the matrix
     # cores at a small fraction of their peak: priced against HBM.
-    kernel = VARIANTS[head.replace("+learned", "")][1]
+    kernel = variant_of(head)[1]
     s_bytes = 8 if kernel == 0 else 4
     op_bytes = (2 * ndof + nel) * s_bytes
     op_flops = 1152.0 * nel

@@ -1101,6 +1101,7 @@ void coarse_factor(hipStream_t s0, Coarse &C, bool lag)
     // the inverse's launches are enqueued by coarse_finish, when the host
     // has enqueued the start of the solve on the main stream
     C.inv_pending = true;
+    C.waited = false;
     C.pending = true;
 }
 
@@ -1169,6 +1170,7 @@ void coarse_switch(Coarse &C, hipStream_t s)
     if (!C.pending || C.solve_waits) return;  // no lagged factorisation in flight
     coarse_finish(C);
     HIPCHK(hipStreamWaitEvent(s, C.ev_done, 0));
+    C.waited = true;
     C.cur = C.target;
     C.Linv = C.Lbuf[C.cur];
     C.LinvT = C.LTbuf[C.cur];
@@ -1231,7 +1233,10 @@ void launch_coarse_solve(hipStream_t s, const Coarse &C, const T *b, T *x, const
 {
     const int N = (C.n + NB - 1) / NB * NB;
     coarse_finish(C);  // also a lagged factorisation's, which runs meanwhile
-    if (C.solve_waits) HIPCHK(hipStreamWaitEvent(s, C.ev_done, 0));  // the factorisation of this hierarchy
+    if (C.solve_waits && !C.waited) {  // the factorisation of this hierarchy, once per stream order
+        HIPCHK(hipStreamWaitEvent(s, C.ev_done, 0));
+        C.waited = true;
+    }
     // C.y has room for kMaxBatch right-hand sides
     k_trmv<false, T, double>
         <<<dim3(C.n, 1, batch_dim(nb)), kTrmvThreads, 0, s>>>(C.Linv, b, C.y, N, C.n, nullptr, ctl);

@@ -170,6 +170,7 @@ struct Coarse {
     hipEvent_t ev_in = nullptr, ev_done = nullptr;
     bool pending = false;  // factorisation launched, SPD flag not read yet
     mutable bool inv_pending = false;  // its inverse not enqueued yet (coarse_finish)
+    mutable bool waited = false;       // the solve stream already waits for it (stream order holds after)
 };
 
 struct MixedState;

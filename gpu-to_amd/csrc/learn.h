@@ -89,6 +89,11 @@ void learn_backward(gputo_handle *h);                  // G += d<gy, y>/dtheta
 // d = F - A U;  e = N(d);  gamma = <d,e>/<e,Ae>;  U += gamma e if gamma > 0.
 // Logs d as a training sample when logging is on.
 void learn_stage(gputo_handle *h, const HierView<float> &H, int l, const float *F, float *U, const int *ctl);
+// What learn_stage launches in the next V-cycle: -1 a logged sample (its
+// slot differs from V-cycle to V-cycle), 0 nothing, 1 the correction.
+int learn_stage_kind(const gputo_handle *h);
+// A V-cycle whose launches were replayed: counted as learn_stage counts it.
+void learn_stage_count(gputo_handle *h);
 void learn_begin_solve(gputo_handle *h);                   // reset counters and the sample log; policy
 // after a solve of `seconds` (wall): counters, logged samples, policy update
 void learn_end_solve(gputo_handle *h, int *accepted, int *rejected, double seconds = -1.0);

@@ -971,6 +971,15 @@ void learn_stage(gputo_handle *h, const HierView<float> &H, int l, const float *
                                            ctl);
 }
 
+int learn_stage_kind(const gputo_handle *h)
+{
+    const LearnState *L = h->ln;
+    if (L->logging && L->vc_count < L->nsamp_max) return -1;
+    return L->trained && L->use ? 1 : 0;
+}
+
+void learn_stage_count(gputo_handle *h) { ++h->ln->vc_count; }
+
 // Solve-level fallback of params.learned = 2.  The stage is worth its cost
 // only if it shortens the whole solve, which a per-V-cycle test cannot
 // see, so successive solves of a design loop are compared by wall time:

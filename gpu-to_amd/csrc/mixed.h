@@ -24,6 +24,9 @@ struct MixedState {
     double *norm = nullptr;            // ||r|| of the outer residual (device)
     double norm_host[2] = {0, 0};
     MfmaState *mf = nullptr;
+    hipGraphExec_t vc_exec = nullptr;  // the recorded V-cycle of the current solve (mixed_vcycle_graph)
+    int vc_kind = 0;                   // its learned stage (learn_stage_kind)
+    const int *vc_ctl = nullptr;
     int last_inner = 0, last_outer = 0, last_learned_ok = 0, last_learned_bad = 0;
     long long bytes = 0;
     std::vector<void *> allocs;
@@ -36,6 +39,15 @@ void mixed_begin(gputo_handle *h);                                        // FP3
 // z = V32(r); r32_ready: the FP32 copy of r is already in mixed_r32(h);
 // z null: the result stays in FP32 (mixed_z32(h)) for consumers that widen it
 void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, bool r32_ready = false);
+// The same V-cycle with r32_ready and z null (params.graph_vcycles), its
+// launches recorded once in a HIP graph and replayed with one launch while
+// they stay the same: the learned stage launches nothing, or the
+// correction, in every V-cycle (a logged sample is launched from the host).
+// Results equal mixed_vcycle's bit for bit.  The graph holds the
+// hierarchy's pointers and values: it is dropped by mixed_begin, mixed_end
+// and mixed_graph_drop.
+void mixed_vcycle_graph(gputo_handle *h, const int *ctl);
+void mixed_graph_drop(gputo_handle *h);
 float *mixed_r32(gputo_handle *h);        // the FP32 right-hand side of the V-cycle
 const float *mixed_z32(gputo_handle *h);  // the FP32 result of the last V-cycle
 float *mixed_p32(gputo_handle *h);        // the PCG's FP32 search directions (kCgLog slots of ndof)

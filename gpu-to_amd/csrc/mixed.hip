@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "learn.h"
 #include "mixed.h"
 #include "vcycle.h"
 
@@ -74,6 +75,7 @@ void mixed_destroy(gputo_handle *h)
 {
     MixedState *m = h->mx;
     if (!m) return;
+    mixed_graph_drop(h);
     mfma_destroy(h, m);
     for (void *p : m->allocs) (void)hipFree(p);
     if (m->sc_host) (void)hipHostFree(m->sc_host);
@@ -143,6 +145,49 @@ void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, b
     if (z) launch_copy<float, double>(s, m->Z, z, n, ctl);
 }
 
+void mixed_graph_drop(gputo_handle *h)
+{
+    MixedState *m = h->mx;
+    if (!m || !m->vc_exec) return;
+    (void)hipGraphExecDestroy(m->vc_exec);
+    m->vc_exec = nullptr;
+}
+
+void mixed_vcycle_graph(gputo_handle *h, const int *ctl)
+{
+    MixedState *m = h->mx;
+    hipStream_t s = h->stream;
+    const int kind = m->H.learn ? learn_stage_kind(h) : 0;
+    if (kind < 0) {
+        vcycle_t<float>(h, m->H, m->R, m->Z, ctl);
+        return;
+    }
+    if (!m->vc_exec || kind != m->vc_kind || ctl != m->vc_ctl) {
+        mixed_graph_drop(h);
+        // recorded, not run: the host side (learn_stage's count included)
+        // runs as for an eager V-cycle
+        hipGraph_t g = nullptr;
+        check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "graph capture");
+        try {
+            vcycle_t<float>(h, m->H, m->R, m->Z, ctl);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check_hip(hipStreamEndCapture(s, &g), "graph capture");
+        const hipError_t e = hipGraphInstantiate(&m->vc_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) m->vc_exec = nullptr;
+        check_hip(e, "graph instantiate");
+        m->vc_kind = kind;
+        m->vc_ctl = ctl;
+    } else if (m->H.learn) {
+        learn_stage_count(h);
+    }
+    check_hip(hipGraphLaunch(m->vc_exec, s), "graph launch");
+}
+
 float *mixed_r32(gputo_handle *h) { return h->mx->R; }
 const float *mixed_z32(gputo_handle *h) { return h->mx->Z; }
 float *mixed_p32(gputo_handle *h) { return h->mx->Pcg; }
@@ -152,12 +197,14 @@ double *mixed_alog(gputo_handle *h) { return h->mx->alog; }
 void mixed_begin(gputo_handle *h)
 {
     if (!h->mx) mixed_create(h);
+    mixed_graph_drop(h);
     mixed_prepare(h);
 }
 
 void mixed_end(gputo_handle *h, int nit, int converged)
 {
     if (!h->mx) return;
+    mixed_graph_drop(h);
     h->mx->last_inner = converged ? nit + 1 : nit;
     h->mx->last_outer = 1;
 }

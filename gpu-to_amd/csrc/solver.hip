@@ -448,16 +448,25 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     // coarse_lag = n >= 2: the first n V-cycles use the previous setup's
     // coarsest factor, later ones the factor of this setup
     int vcycles = 0;
-    auto precond_n = [&](const double *r, double *z, bool r32_ready) {
-        if (h->prm.coarse_lag >= 2 && vcycles == h->prm.coarse_lag) coarse_switch(h->C, s);
-        precond(h, r, z, r32_ready);
-        ++vcycles;
-    };
     // the residual update also writes the FP32 right-hand side of the mixed V-cycle
     float *r32 = h->prm.precision != GPUTO_PREC_F64 ? mixed_r32(h) : nullptr;
     // ... whose FP32 result the CG then reads directly (not for a periodic
     // cell, whose preconditioned residual is folded in FP64)
     const float *z32 = (r32 && !h->prm.periodic) ? mixed_z32(h) : nullptr;
+    // graph_vcycles: the loop's V-cycles of that form replay one recorded HIP
+    // graph (their launches are the same each time)
+    const bool graphs = z32 && h->prm.graph_vcycles;
+    auto precond_n = [&](const double *r, double *z, bool r32_ready) {
+        if (h->prm.coarse_lag >= 2 && vcycles == h->prm.coarse_lag) {
+            coarse_switch(h->C, s);
+            if (graphs) mixed_graph_drop(h);  // it holds the previous factor
+        }
+        if (graphs && r32_ready && !z)
+            mixed_vcycle_graph(h, &h->sc->done);
+        else
+            precond(h, r, z, r32_ready);
+        ++vcycles;
+    };
     // ... and then keeps the search direction in FP32: p = z + beta p formed
     // in FP64 and rounded once; u, r and the matvec take exactly the stored
     // values, so the recurrences stay consistent (the FP64 residual decides

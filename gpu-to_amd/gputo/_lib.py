@@ -58,6 +58,7 @@ class GputoParams(ctypes.Structure):
         ("mixed_npre2", ctypes.c_int),
         ("mixed_npost2", ctypes.c_int),
         ("homog_sequential", ctypes.c_int),
+        ("graph_vcycles", ctypes.c_int),
     ]
 
 

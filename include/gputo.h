@@ -117,6 +117,12 @@ typedef struct gputo_params {
     int homog_sequential; /* gputo_homogenize: 0 (default) solves the six load cases
                              together, each operator launch applied to all six
                              vectors; 1 solves them one after the other */
+    int graph_vcycles;    /* mixed precisions: 1 replays the FP32 V-cycle of the PCG
+                             loop from one recorded HIP graph where its launches
+                             do not change (one host launch instead of ~45);
+                             0 (default) launches every kernel from the host,
+                             measured faster on MI355X.  The results are the
+                             same bit for bit */
 } gputo_params;
 
 /* Fill *p with the defaults (REF_C, F64, no learned stage, operator
