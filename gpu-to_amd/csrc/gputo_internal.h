@@ -317,8 +317,10 @@ void launch_cg_update_p32(hipStream_t s, float *Pout, const float *Pin, const fl
 // kept FP32 directions
 constexpr int kCgLog = 16;
 void launch_cg_alpha_log(hipStream_t s, CgScalars *sc, const double *partials, int nparts, double *alog);
+// X0 (optional): also writes X0 = prerelax R32 / AD32, the prerelaxed
+// level-0 iterate of the next FP32 V-cycle (mixed_prerelax_target)
 void launch_cg_update_r(hipStream_t s, double *R, const double *Q, long n, CgScalars *sc, double *partials,
-                        float *R32);
+                        float *R32, const float *AD32 = nullptr, float *X0 = nullptr, double prerelax = 0.0);
 void launch_cg_flush_u(hipStream_t s, double *U, const float *P, long n, const double *alog, int m);
 // flexible beta (partials needs 2*kRedBlocks entries); z FP64, or the
 // mixed V-cycle's FP32 output (products in FP64, the same values)

@@ -1211,10 +1211,14 @@ void launch_cg_alpha_log(hipStream_t s, CgScalars *sc, const double *partials, i
 
 // Residual update of the PCG with deferred u (u += alpha p is applied later
 // from the logged alphas and directions): R -= alpha Q, R32 = R in FP32,
-// partials of R.R, two elements per pass as k_cg_update_ur.
+// partials of R.R, two elements per pass as k_cg_update_ur.  PRE: also the
+// FP32 V-cycle's prerelaxed level-0 iterate X0 = pr R32 / AD32, the
+// expression of k_prerelax on the same FP32 values.
+template <bool PRE>
 __global__ __launch_bounds__(kBlock) void k_cg_update_r(double *__restrict__ R, const double *__restrict__ Q, long n,
                                                        const CgScalars *sc, double *partials,
-                                                       float *__restrict__ R32)
+                                                       float *__restrict__ R32, const float *__restrict__ AD32,
+                                                       float *__restrict__ X0, float pr)
 {
     const double al = sc->alpha;
     const long stride = (long)kRedBlocks * kBlock;
@@ -1224,14 +1228,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_r(double *__restrict__ R, 
         const bool two = i + stride < n;
         const long j = two ? i + stride : i;  // in bounds; unused when !two
         const double r0 = R[i], q0 = Q[i], r1 = R[j], q1 = Q[j];
+        float d0 = 0.0f, d1 = 0.0f;
+        if (PRE) {
+            d0 = AD32[i];
+            d1 = AD32[j];
+        }
         if (i == i0 && sc->done) return;  // read with alpha while the loads are in flight
         const double ra = r0 - al * q0, rb = r1 - al * q1;
+        const float fa = (float)ra, fb = (float)rb;
         R[i] = ra;
-        R32[i] = (float)ra;
+        R32[i] = fa;
+        if (PRE) X0[i] = pr * fa / d0;
         s += ra * ra;
         if (two) {
             R[j] = rb;
-            R32[j] = (float)rb;
+            R32[j] = fb;
+            if (PRE) X0[j] = pr * fb / d1;
             s += rb * rb;
         }
     }
@@ -1240,9 +1252,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_r(double *__restrict__ R, 
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 void launch_cg_update_r(hipStream_t s, double *R, const double *Q, long n, CgScalars *sc, double *partials,
-                        float *R32)
+                        float *R32, const float *AD32, float *X0, double prerelax)
 {
-    k_cg_update_r<<<kRedBlocks, kBlock, 0, s>>>(R, Q, n, sc, partials, R32);
+    if (X0)
+        k_cg_update_r<true><<<kRedBlocks, kBlock, 0, s>>>(R, Q, n, sc, partials, R32, AD32, X0, (float)prerelax);
+    else
+        k_cg_update_r<false><<<kRedBlocks, kBlock, 0, s>>>(R, Q, n, sc, partials, R32, nullptr, nullptr, 0.0f);
 }
 
 // U += alpha_0 P_0 + ... + alpha_{m-1} P_{m-1}, term by term in that order

@@ -27,6 +27,7 @@ struct MixedState {
     hipGraphExec_t vc_exec = nullptr;  // the recorded V-cycle of the current solve (mixed_vcycle_graph)
     int vc_kind = 0;                   // its learned stage (learn_stage_kind)
     const int *vc_ctl = nullptr;
+    bool vc_pre0 = false;              // recorded without the level-0 prerelaxation pass
     int last_inner = 0, last_outer = 0, last_learned_ok = 0, last_learned_bad = 0;
     long long bytes = 0;
     std::vector<void *> allocs;
@@ -38,7 +39,14 @@ void mixed_stats(gputo_handle *h, gputo_to_stats *st);
 void mixed_begin(gputo_handle *h);                                        // FP32 hierarchy copies
 // z = V32(r); r32_ready: the FP32 copy of r is already in mixed_r32(h);
 // z null: the result stays in FP32 (mixed_z32(h)) for consumers that widen it
-void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, bool r32_ready = false);
+// pre0_ready: the prerelaxed level-0 iterate is already in
+// mixed_prerelax_target(h) (formed from mixed_r32(h) by the caller)
+void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, bool r32_ready = false,
+                  bool pre0_ready = false);
+// Where the FP32 V-cycle's first level-0 pass writes the prerelaxed iterate
+// prerelax*F/AD (its sweeps then start from it), with that AD and prerelax;
+// null when the V-cycle does not start with that pass.
+float *mixed_prerelax_target(gputo_handle *h, const float **ad, double *prerelax);
 // The same V-cycle with r32_ready and z null (params.graph_vcycles), its
 // launches recorded once in a HIP graph and replayed with one launch while
 // they stay the same: the learned stage launches nothing, or the
@@ -46,7 +54,7 @@ void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, b
 // Results equal mixed_vcycle's bit for bit.  The graph holds the
 // hierarchy's pointers and values: it is dropped by mixed_begin, mixed_end
 // and mixed_graph_drop.
-void mixed_vcycle_graph(gputo_handle *h, const int *ctl);
+void mixed_vcycle_graph(gputo_handle *h, const int *ctl, bool pre0_ready = false);
 void mixed_graph_drop(gputo_handle *h);
 float *mixed_r32(gputo_handle *h);        // the FP32 right-hand side of the V-cycle
 const float *mixed_z32(gputo_handle *h);  // the FP32 result of the last V-cycle

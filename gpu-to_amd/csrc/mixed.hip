@@ -135,14 +135,29 @@ static void mixed_prepare(gputo_handle *h)
 // z = V(r) with the FP32 V-cycle: r (FP64) is rounded to FP32, the whole
 // cycle runs on the FP32 copies of the hierarchy (FP64 coarsest Cholesky
 // inside), and z is widened back to FP64 for the FP64 Krylov recurrences.
-void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, bool r32_ready)
+void mixed_vcycle(gputo_handle *h, const double *r, double *z, const int *ctl, bool r32_ready, bool pre0_ready)
 {
     MixedState *m = h->mx;
     hipStream_t s = h->stream;
     const long n = h->L[0].g.ndof;
     if (!r32_ready) launch_copy<double, float>(s, r, m->R, n, ctl);
+    m->H.pre0_ready = pre0_ready;
     vcycle_t<float>(h, m->H, m->R, m->Z, ctl);
+    m->H.pre0_ready = false;
     if (z) launch_copy<float, double>(s, m->Z, z, n, ctl);
+}
+
+float *mixed_prerelax_target(gputo_handle *h, const float **ad, double *prerelax)
+{
+    MixedState *m = h->mx;
+    const HierView<float> &H = m->H;
+    // the condition under which the V-cycle's level-0 pre-smoothing starts
+    // with the elementwise prerelaxation (vcycle_t, sweeps_t)
+    const int npre = sweeps_of<float>(h, 0, false);
+    if (h->nl < 2 || H.per || H.mg9 || H.l0 < L0_MFMA_F32 || npre <= 0) return nullptr;
+    *ad = H.L[0].AD;
+    *prerelax = preset_of(h->prm).prerelax;
+    return npre % 2 == 1 ? H.L[0].Tmp : m->Z;
 }
 
 void mixed_graph_drop(gputo_handle *h)
@@ -153,24 +168,25 @@ void mixed_graph_drop(gputo_handle *h)
     m->vc_exec = nullptr;
 }
 
-void mixed_vcycle_graph(gputo_handle *h, const int *ctl)
+void mixed_vcycle_graph(gputo_handle *h, const int *ctl, bool pre0_ready)
 {
     MixedState *m = h->mx;
     hipStream_t s = h->stream;
     const int kind = m->H.learn ? learn_stage_kind(h) : 0;
     if (kind < 0) {
-        vcycle_t<float>(h, m->H, m->R, m->Z, ctl);
+        mixed_vcycle(h, nullptr, nullptr, ctl, true, pre0_ready);
         return;
     }
-    if (!m->vc_exec || kind != m->vc_kind || ctl != m->vc_ctl) {
+    if (!m->vc_exec || kind != m->vc_kind || ctl != m->vc_ctl || pre0_ready != m->vc_pre0) {
         mixed_graph_drop(h);
         // recorded, not run: the host side (learn_stage's count included)
         // runs as for an eager V-cycle
         hipGraph_t g = nullptr;
         check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "graph capture");
         try {
-            vcycle_t<float>(h, m->H, m->R, m->Z, ctl);
+            mixed_vcycle(h, nullptr, nullptr, ctl, true, pre0_ready);
         } catch (...) {
+            m->H.pre0_ready = false;
             (void)hipStreamEndCapture(s, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
@@ -182,6 +198,7 @@ void mixed_vcycle_graph(gputo_handle *h, const int *ctl)
         check_hip(e, "graph instantiate");
         m->vc_kind = kind;
         m->vc_ctl = ctl;
+        m->vc_pre0 = pre0_ready;
     } else if (m->H.learn) {
         learn_stage_count(h);
     }
@@ -199,6 +216,7 @@ void mixed_begin(gputo_handle *h)
     if (!h->mx) mixed_create(h);
     mixed_graph_drop(h);
     mixed_prepare(h);
+    h->mx->H.pre0_ready = false;
 }
 
 void mixed_end(gputo_handle *h, int nit, int converged)

@@ -426,12 +426,12 @@ static bool custom_schedule(const gputo_params &p)
 
 // Device-resident PCG (mgcg1.cu:235-280); U, F on the device.
 // z = M r: the FP64 V-cycle, or the FP32 V-cycle of the mixed precisions
-static void precond(gputo_handle *h, const double *r, double *z, bool r32_ready = false)
+static void precond(gputo_handle *h, const double *r, double *z, bool r32_ready = false, bool pre0_ready = false)
 {
     if (h->prm.precision == GPUTO_PREC_F64)
         vcycle(h, r, z);
     else
-        mixed_vcycle(h, r, z, &h->sc->done, r32_ready);
+        mixed_vcycle(h, r, z, &h->sc->done, r32_ready, pre0_ready);
 }
 
 int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, double *relres, int *nit)
@@ -456,15 +456,21 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
     // graph_vcycles: the loop's V-cycles of that form replay one recorded HIP
     // graph (their launches are the same each time)
     const bool graphs = z32 && h->prm.graph_vcycles;
+    // ... and then also forms the V-cycle's prerelaxed level-0 iterate
+    // (prerelax*r/AD), in place of the V-cycle's first elementwise pass
+    const float *ad32 = nullptr;
+    double prx = 0.0;
+    float *x0 = z32 ? mixed_prerelax_target(h, &ad32, &prx) : nullptr;
     auto precond_n = [&](const double *r, double *z, bool r32_ready) {
         if (h->prm.coarse_lag >= 2 && vcycles == h->prm.coarse_lag) {
             coarse_switch(h->C, s);
             if (graphs) mixed_graph_drop(h);  // it holds the previous factor
         }
+        const bool pre0 = x0 && r32_ready && !z;  // the loop's V-cycles after the residual update
         if (graphs && r32_ready && !z)
-            mixed_vcycle_graph(h, &h->sc->done);
+            mixed_vcycle_graph(h, &h->sc->done, pre0);
         else
-            precond(h, r, z, r32_ready);
+            precond(h, r, z, r32_ready, pre0);
         ++vcycles;
     };
     // ... and then keeps the search direction in FP32: p = z + beta p formed
@@ -515,7 +521,7 @@ int run_pcg(gputo_handle *h, const double *F, double *U, double tol, int maxit, 
             if (per) launch_per_fold<double>(s, L0.g, h->Q, 0, ctl);
             if (p32) {
                 launch_cg_alpha_log(s, h->sc, h->partials, nmv, alog + sl);
-                launch_cg_update_r(s, h->R, h->Q, n, h->sc, h->partials, r32);
+                launch_cg_update_r(s, h->R, h->Q, n, h->sc, h->partials, r32, ad32, x0, prx);
                 if (sl == kCgLog - 1) {  // before slot 0 is overwritten by this iteration's p update
                     launch_cg_flush_u(s, U, p32, n, alog, kCgLog);
                     folded = enq + 1;

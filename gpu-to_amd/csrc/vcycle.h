@@ -53,6 +53,7 @@ struct HierView {
     bool learn = false;            // learned level-2 stage (FP32 views only)
     bool mg9 = false;              // every level matrix-free (mgcg9 engine)
     bool per = false;              // periodic cell: folded defects, images kept equal
+    bool pre0_ready = false;       // level 0's prerelaxed iterate is already formed (by the caller)
     int nb = 1;                    // right-hand sides: level vectors hold nb vectors each, one after another
 };
 
@@ -163,8 +164,9 @@ void sweeps_t(const gputo_handle *h, const HierView<T> &H, int l, const T *F, T 
         // for every halo node; the FP64 node-gather keeps its fused form,
         // 87 us against 22 + 70 us).  It goes where the sweeps end in U.
         in_tmp = n % 2 == 1;
-        launch_prerelax<T>(h->stream, F, v.AD, in_tmp ? v.Tmp : U, h->L[l].g.ndof, preset_of(h->prm).prerelax, ctl,
-                           H.nb);
+        if (!H.pre0_ready)
+            launch_prerelax<T>(h->stream, F, v.AD, in_tmp ? v.Tmp : U, h->L[l].g.ndof, preset_of(h->prm).prerelax,
+                               ctl, H.nb);
         pre = false;
     }
     const T *src = in_tmp ? v.Tmp : U;
