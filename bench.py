@@ -28,7 +28,7 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "level0_operator_cantilever_192x96x
 # correction on levels 0 and 1 of the FP32 V-cycle (0 = the preset's 3 / 3),
 # coarse_lag: V-cycles that use the previous design's coarsest factor[,
 # level-1 sweeps before / after when they differ from level 0's[, sweeps
-# of the levels from 2 on]])
+# of the levels from 2 on[, sweeps of the levels from 3 on]]])
 VARIANTS = {
     "f64": (0, 0, 0, 0, 0),
     "mixed_gather": (1, 1, 0, 0, 0),  # FP32 inner solve, FP32 node-gather level-0 operator
@@ -54,6 +54,12 @@ VARIANTS = {
     "mixed_mfma3s1314": (2, 3, 1, 3, 0, 1, 1, 4, 4),  # 1 + 3 on level 0, 1 + 1 on level 1, 4 + 4 below
     "mixed_mfma3s1312": (2, 3, 1, 3, 0, 1, 1, 2, 2),  # 1 + 3 on level 0, 1 + 1 on level 1, 2 + 2 below
     "mixed_mfma3s1316": (2, 3, 1, 3, 0, 1, 1, 6, 6),  # 1 + 3 on level 0, 1 + 1 on level 1, 6 + 6 below
+    "mixed_mfma3s1311": (2, 3, 1, 3, 0, 1, 1, 1, 1),  # 1 + 3 on level 0, 1 + 1 on level 1, 1 + 1 below
+    "mixed_mfma3s1313": (2, 3, 1, 3, 0, 1, 1, 3, 3),  # 1 + 3 on level 0, 1 + 1 on level 1, 3 + 3 below
+    # 1 + 3 on level 0, 1 + 1 on level 1, 2 + 2 (1 + 1) on level 2, 8 + 8 (10 + 10) from level 3
+    "mixed_mfma3s1312c8": (2, 3, 1, 3, 0, 1, 1, 2, 2, 8, 8),
+    "mixed_mfma3s1312c10": (2, 3, 1, 3, 0, 1, 1, 2, 2, 10, 10),
+    "mixed_mfma3s1311c8": (2, 3, 1, 3, 0, 1, 1, 1, 1, 8, 8),
     "mixed_mfma3s1334": (2, 3, 1, 3, 0, 1, 3, 4, 4),  # 1 + 3 on levels 0 and 1, 4 + 4 below
     "mixed_mfma3s1324": (2, 3, 1, 3, 0, 2, 2, 4, 4),  # 1 + 3 on level 0, 2 + 2 on level 1, 4 + 4 below
     "mixed_mfma3s3314": (2, 3, 3, 3, 0, 1, 1, 4, 4),  # 3 + 3 on level 0, 1 + 1 on level 1, 4 + 4 below
@@ -74,6 +80,11 @@ def variant_of(name):
 def level2_sweeps(v):
     """(pre, post) sweeps of the levels from 2 on (0, 0: the preset's)."""
     return (v[7], v[8]) if len(v) > 7 else (0, 0)
+
+
+def level3_sweeps(v):
+    """(pre, post) sweeps of the levels from 3 on (0, 0: those from 2 on)."""
+    return (v[9], v[10]) if len(v) > 9 else (0, 0)
 
 
 def level1_sweeps(v):
@@ -109,7 +120,11 @@ def level0_sweeps(name):
     s1 = "%d+%d" % (n1[0] or 3, n1[1] or 3)
     out = s0 if s0 == s1 else "%s on level 0, %s on level 1" % (s0, s1)
     n2 = level2_sweeps(v)
-    if n2[0] or n2[1]:
+    n3 = level3_sweeps(v)
+    if n3[0] or n3[1]:
+        out += ", %d+%d on level 2, %d+%d from level 3" % (n2[0] or 3, n2[1] or 3, n3[0] or n2[0] or 3,
+                                                         n3[1] or n2[1] or 3)
+    elif n2[0] or n2[1]:
         out += ", %d+%d on the levels from 2" % (n2[0] or 3, n2[1] or 3)
     return out
 
@@ -168,11 +183,12 @@ def run_variant(gputo, name, args, Ke, F, fixed, x0):
     prec, kernel, npre0, npost0, lag = v[:5]
     npre1, npost1 = level1_sweeps(v)
     npre2, npost2 = level2_sweeps(v)
+    npre3, npost3 = level3_sweeps(v)
     learned = "+learned" in name
     s = gputo.Solver(args.nelx, args.nely, args.nelz, args.nl, precision=prec, learned=2 if learned else 0,
                      l0_kernel=kernel, mixed_npre0=npre0, mixed_npost0=npost0, mixed_npre1=npre1,
                      mixed_npost1=npost1, mixed_npre2=npre2, mixed_npost2=npost2, coarse_lag=lag,
-                     graph_vcycles="+graph" in name)
+                     graph_vcycles="+graph" in name, mixed_npre3=npre3, mixed_npost3=npost3)
     pre = pretrain(gputo, s, args, Ke, fixed, name) if learned else None
     run = gputo.TopOpt(s, Ke, x0, F, fixed, volfrac=0.12, penal=3.0, rmin=np.sqrt(3.0), ft=1, move=0.2,
                        cg_tol=args.cg_tol, cg_maxit=args.cg_maxit)
@@ -389,7 +405,7 @@ def main():
     ap.add_argument("--nl", type=int, default=5)
     ap.add_argument("--cg-tol", type=float, default=1e-8)
     ap.add_argument("--cg-maxit", type=int, default=300)
-    ap.add_argument("--variants", default="f64,mixed_mfma3s1314,mixed_mfma3s1314+learned")
+    ap.add_argument("--variants", default="f64,mixed_mfma3s1311c8,mixed_mfma3s1311c8+learned")
     ap.add_argument("--op-reps", type=int, default=50)
     ap.add_argument("--pretrain-problems", type=int, default=16)
     ap.add_argument("--pretrain-steps", type=int, default=50)

@@ -407,7 +407,7 @@ int setup_hierarchy(gputo_handle *h, const std::vector<int> &fixed0)
 static bool schedule_ok(const gputo_params &p)
 {
     const Preset ps = preset_of(p);
-    for (int l = 0; l < 3; ++l) {
+    for (int l = 0; l < 4; ++l) {
         const int a = mixed_sweeps(p, l, false), b = mixed_sweeps(p, l, true);
         if (a < 0 || b < 0) return false;
         if (a == 0 && b == 0) continue;
@@ -421,7 +421,7 @@ static bool custom_schedule(const gputo_params &p)
 {
     return p.precision != GPUTO_PREC_F64 &&
            (p.mixed_npre0 > 0 || p.mixed_npost0 > 0 || p.mixed_npre1 > 0 || p.mixed_npost1 > 0 ||
-            p.mixed_npre2 > 0 || p.mixed_npost2 > 0);
+            p.mixed_npre2 > 0 || p.mixed_npost2 > 0 || p.mixed_npre3 > 0 || p.mixed_npost3 > 0);
 }
 
 // Device-resident PCG (mgcg1.cu:235-280); U, F on the device.

@@ -190,12 +190,16 @@ inline int mixed_sweeps(const gputo_params &prm, int l, bool post)
 {
     if (l == 0) return post ? prm.mixed_npost0 : prm.mixed_npre0;
     if (l == 1) return post ? prm.mixed_npost1 : prm.mixed_npre1;
+    if (l >= 3) {
+        const int n = post ? prm.mixed_npost3 : prm.mixed_npre3;
+        if (n != 0) return n;
+    }
     return post ? prm.mixed_npost2 : prm.mixed_npre2;
 }
 
 // Sweeps before / after the coarse correction on level l: the preset's,
-// or in the FP32 V-cycle the mixed_npre/npost params of level 0, level 1
-// and the levels from 2 on.
+// or in the FP32 V-cycle the mixed_npre/npost params of level 0, level 1,
+// level 2 and the levels from 3 on.
 template <class T>
 int sweeps_of(const gputo_handle *h, int l, bool post)
 {

@@ -59,6 +59,8 @@ class GputoParams(ctypes.Structure):
         ("mixed_npost2", ctypes.c_int),
         ("homog_sequential", ctypes.c_int),
         ("graph_vcycles", ctypes.c_int),
+        ("mixed_npre3", ctypes.c_int),
+        ("mixed_npost3", ctypes.c_int),
     ]
 
 

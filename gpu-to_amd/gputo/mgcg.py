@@ -29,7 +29,7 @@ class Solver:
     def __init__(self, nelx, nely, nelz, nl, preset=_lib.PRESET_REF_C, precision=_lib.PREC_F64, learned=False,
                  print_level=0, l0_kernel=0, l1_kernel=0, nswp=0, coarse_lag=0, periodic=False, mixed_npre0=0,
                  mixed_npost0=0, mixed_npre1=0, mixed_npost1=0, mixed_npre2=0, mixed_npost2=0, homog_sequential=0,
-                 graph_vcycles=0):
+                 graph_vcycles=0, mixed_npre3=0, mixed_npost3=0):
         self.lib = _lib.load()
         self.nelx, self.nely, self.nelz, self.nl = int(nelx), int(nely), int(nelz), int(nl)
         self.nel = self.nelx * self.nely * self.nelz
@@ -55,6 +55,8 @@ class Solver:
         self.params.mixed_npost2 = mixed_npost2
         self.params.homog_sequential = int(bool(homog_sequential))
         self.params.graph_vcycles = int(bool(graph_vcycles))
+        self.params.mixed_npre3 = mixed_npre3
+        self.params.mixed_npost3 = mixed_npost3
         h = ctypes.c_void_p()
         check(self.lib.gputo_create(self.nelx, self.nely, self.nelz, self.nl, ctypes.byref(self.params),
                                     ctypes.byref(h)), "gputo_create")

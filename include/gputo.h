@@ -112,7 +112,8 @@ typedef struct gputo_params {
     int mixed_npost0;
     int mixed_npre1;   /* the same for level 1 */
     int mixed_npost1;
-    int mixed_npre2;   /* the same for the levels from 2 down to the last above the coarsest */
+    int mixed_npre2;   /* the same for the levels from 2 down to the last above the coarsest
+                          (from level 3 on: mixed_npre3 / mixed_npost3 when set) */
     int mixed_npost2;
     int homog_sequential; /* gputo_homogenize: 0 (default) solves the six load cases
                              together, each operator launch applied to all six
@@ -123,6 +124,11 @@ typedef struct gputo_params {
                              0 (default) launches every kernel from the host,
                              measured faster on MI355X.  The results are the
                              same bit for bit */
+    int mixed_npre3;      /* FP32 V-cycle: Jacobi sweeps before / after the coarse
+                             correction on the levels from 3 down to the last above the
+                             coarsest (0: those of mixed_npre2 / mixed_npost2; the same
+                             parity rule) */
+    int mixed_npost3;
 } gputo_params;
 
 /* Fill *p with the defaults (REF_C, F64, no learned stage, operator

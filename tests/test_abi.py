@@ -74,3 +74,5 @@ def test_odd_sweep_total_rejected(gputo):
         gputo.Solver(16, 8, 8, 3, precision=2, l0_kernel=3, mixed_npre1=1, mixed_npost1=2)
     with pytest.raises(gputo.GputoError):
         gputo.Solver(32, 16, 16, 4, precision=2, l0_kernel=3, mixed_npre2=2, mixed_npost2=1)
+    with pytest.raises(gputo.GputoError):
+        gputo.Solver(64, 32, 32, 5, precision=2, l0_kernel=3, mixed_npre3=2, mixed_npost3=1)
