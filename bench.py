@@ -60,6 +60,8 @@ VARIANTS = {
     "mixed_mfma3s1312c8": (2, 3, 1, 3, 0, 1, 1, 2, 2, 8, 8),
     "mixed_mfma3s1312c10": (2, 3, 1, 3, 0, 1, 1, 2, 2, 10, 10),
     "mixed_mfma3s1311c8": (2, 3, 1, 3, 0, 1, 1, 1, 1, 8, 8),
+    # as mixed_mfma3s1311c8 with level 1 pre-smoothed by the prerelaxed iterate alone (-1: no sweep)
+    "mixed_mfma3s1311c8p1": (2, 3, 1, 3, 0, -1, 1, 1, 1, 8, 8),
     "mixed_mfma3s1334": (2, 3, 1, 3, 0, 1, 3, 4, 4),  # 1 + 3 on levels 0 and 1, 4 + 4 below
     "mixed_mfma3s1324": (2, 3, 1, 3, 0, 2, 2, 4, 4),  # 1 + 3 on level 0, 2 + 2 on level 1, 4 + 4 below
     "mixed_mfma3s3314": (2, 3, 3, 3, 0, 1, 1, 4, 4),  # 3 + 3 on level 0, 1 + 1 on level 1, 4 + 4 below
@@ -115,17 +117,20 @@ def level0_sweeps(name):
     v = variant_of(name)
     if v[0] == 0:
         return None
+    def cnt(x, y=0):  # a sweep count (0: the next setting, else the preset's 3; -1: none)
+        return max(x, 0) if x else (max(y, 0) if y else 3)
+
     n1 = level1_sweeps(v)
-    s0 = "%d+%d" % (v[2] or 3, v[3] or 3)
-    s1 = "%d+%d" % (n1[0] or 3, n1[1] or 3)
+    s0 = "%d+%d" % (cnt(v[2]), cnt(v[3]))
+    s1 = "%d+%d" % (cnt(n1[0]), cnt(n1[1]))
     out = s0 if s0 == s1 else "%s on level 0, %s on level 1" % (s0, s1)
     n2 = level2_sweeps(v)
     n3 = level3_sweeps(v)
     if n3[0] or n3[1]:
-        out += ", %d+%d on level 2, %d+%d from level 3" % (n2[0] or 3, n2[1] or 3, n3[0] or n2[0] or 3,
-                                                         n3[1] or n2[1] or 3)
+        out += ", %d+%d on level 2, %d+%d from level 3" % (cnt(n2[0]), cnt(n2[1]), cnt(n3[0], n2[0]),
+                                                         cnt(n3[1], n2[1]))
     elif n2[0] or n2[1]:
-        out += ", %d+%d on the levels from 2" % (n2[0] or 3, n2[1] or 3)
+        out += ", %d+%d on the levels from 2" % (cnt(n2[0]), cnt(n2[1]))
     return out
 
 
@@ -405,7 +410,7 @@ def main():
     ap.add_argument("--nl", type=int, default=5)
     ap.add_argument("--cg-tol", type=float, default=1e-8)
     ap.add_argument("--cg-maxit", type=int, default=300)
-    ap.add_argument("--variants", default="f64,mixed_mfma3s1311c8,mixed_mfma3s1311c8+learned")
+    ap.add_argument("--variants", default="f64,mixed_mfma3s1311c8p1,mixed_mfma3s1311c8p1+learned")
     ap.add_argument("--op-reps", type=int, default=50)
     ap.add_argument("--pretrain-problems", type=int, default=16)
     ap.add_argument("--pretrain-steps", type=int, default=50)

@@ -409,9 +409,10 @@ static bool schedule_ok(const gputo_params &p)
     const Preset ps = preset_of(p);
     for (int l = 0; l < 4; ++l) {
         const int a = mixed_sweeps(p, l, false), b = mixed_sweeps(p, l, true);
-        if (a < 0 || b < 0) return false;
+        if (a < -1 || b < -1) return false;
         if (a == 0 && b == 0) continue;
-        const int pre = a > 0 ? a : ps.npre, post = b > 0 ? b : ps.npost;
+        // -1: no sweep (0: the preset's count)
+        const int pre = a > 0 ? a : (a < 0 ? 0 : ps.npre), post = b > 0 ? b : (b < 0 ? 0 : ps.npost);
         if ((std::max(pre, 1) + post) % 2) return false;
     }
     return true;
@@ -420,8 +421,8 @@ static bool schedule_ok(const gputo_params &p)
 static bool custom_schedule(const gputo_params &p)
 {
     return p.precision != GPUTO_PREC_F64 &&
-           (p.mixed_npre0 > 0 || p.mixed_npost0 > 0 || p.mixed_npre1 > 0 || p.mixed_npost1 > 0 ||
-            p.mixed_npre2 > 0 || p.mixed_npost2 > 0 || p.mixed_npre3 > 0 || p.mixed_npost3 > 0);
+           (p.mixed_npre0 != 0 || p.mixed_npost0 != 0 || p.mixed_npre1 != 0 || p.mixed_npost1 != 0 ||
+            p.mixed_npre2 != 0 || p.mixed_npost2 != 0 || p.mixed_npre3 != 0 || p.mixed_npost3 != 0);
 }
 
 // Device-resident PCG (mgcg1.cu:235-280); U, F on the device.

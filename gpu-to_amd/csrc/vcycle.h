@@ -199,7 +199,8 @@ inline int mixed_sweeps(const gputo_params &prm, int l, bool post)
 
 // Sweeps before / after the coarse correction on level l: the preset's,
 // or in the FP32 V-cycle the mixed_npre/npost params of level 0, level 1,
-// level 2 and the levels from 3 on.
+// level 2 and the levels from 3 on (-1: none; before the correction the
+// prerelaxed iterate alone then smooths the level).
 template <class T>
 int sweeps_of(const gputo_handle *h, int l, bool post)
 {
@@ -207,6 +208,7 @@ int sweeps_of(const gputo_handle *h, int l, bool post)
     if (std::is_same<T, float>::value) {
         const int n = mixed_sweeps(h->prm, l, post);
         if (n > 0) return n;
+        if (n == -1) return 0;
     }
     return post ? p.npost : p.npre;
 }

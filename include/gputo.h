@@ -105,10 +105,12 @@ typedef struct gputo_params {
                           level needs at least 2 elements per direction; the
                           learned stage is not available */
     int mixed_npre0;   /* FP32 V-cycle of the mixed precisions: Jacobi sweeps on level 0
-                          before / after the coarse correction (0: the preset's).  The
-                          preconditioner changes, not the solution: the FP64 CG still
-                          stops at the FP64 tolerance.  With the prerelaxation counted as
-                          the first pre-sweep the total must be even (-5 otherwise) */
+                          before / after the coarse correction (0: the preset's; -1: none,
+                          the level is pre-smoothed by the prerelaxed iterate
+                          prerelax*F/AD alone).  The preconditioner changes, not the
+                          solution: the FP64 CG still stops at the FP64 tolerance.  With
+                          the prerelaxation counted as the first pre-sweep the total must
+                          be even (-5 otherwise) */
     int mixed_npost0;
     int mixed_npre1;   /* the same for level 1 */
     int mixed_npost1;

@@ -76,3 +76,7 @@ def test_odd_sweep_total_rejected(gputo):
         gputo.Solver(32, 16, 16, 4, precision=2, l0_kernel=3, mixed_npre2=2, mixed_npost2=1)
     with pytest.raises(gputo.GputoError):
         gputo.Solver(64, 32, 32, 5, precision=2, l0_kernel=3, mixed_npre3=2, mixed_npost3=1)
+    with pytest.raises(gputo.GputoError):  # -1 (no sweep) is the only negative count
+        gputo.Solver(16, 8, 8, 3, precision=2, l0_kernel=3, mixed_npre1=-2, mixed_npost1=1)
+    with pytest.raises(gputo.GputoError):  # no pre-sweep: the prerelaxation and two post-sweeps are odd
+        gputo.Solver(16, 8, 8, 3, precision=2, l0_kernel=3, mixed_npre1=-1, mixed_npost1=2)

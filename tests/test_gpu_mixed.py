@@ -74,23 +74,30 @@ def test_mixed_solve_matches_f64(gputo, oracle, prec, kernel, mesh):
 
 
 @pytest.mark.parametrize("kernel", [3, 1])
-@pytest.mark.parametrize("sweeps", [(1, 3), (2, 2), (1, 1), (1, 1, 6, 6), (1, 1, 2, 2), (1, 3, 2, 2, 8, 8)])
+@pytest.mark.parametrize("sweeps", [(1, 3), (2, 2), (1, 1), (1, 1, 6, 6), (1, 1, 2, 2), (1, 3, 2, 2, 8, 8),
+                                    (1, 3, -1, 1, 1, 1, 8, 8)])
 def test_level0_sweep_schedule(gputo, oracle, sweeps, kernel):
     """Other sweep counts on levels 0 and 1 (and, with four entries, on the
-    levels from 2; with six, 2 and from 3 on) of the FP32 V-cycle change the
+    levels from 2; with six, 2 and from 3 on; with eight, level 0, level 1,
+    level 2 and from 3 on, -1 standing for the prerelaxed iterate alone
+    before the coarse correction) of the FP32 V-cycle change the
     preconditioner, not the solution: the FP64 true residual meets the
-    tolerance and the compliance matches the FP64 solve.  The six-entry
-    case has 5 levels, so level 3 is smoothed."""
+    tolerance and the compliance matches the FP64 solve.  The six- and
+    eight-entry cases have 5 levels, so level 3 is smoothed."""
     nelx, nely, nelz, nl = (64, 32, 32, 5) if len(sweeps) > 4 else (48, 24, 24, 4)
     print("case", sweeps, kernel, flush=True)
     Ke, F, CX, fixed = cantilever_case(nelx, nely, nelz, seed=23, kind="perturbed")
     s64 = gputo.Solver(nelx, nely, nelz, nl)
     U64, r64, n64, rc = s64.solve(Ke, F, CX, fixed, 1e-10, 300)
     assert rc == 0
-    below = sweeps[2:4] if len(sweeps) > 2 else (0, 0)
-    below3 = sweeps[4:6] if len(sweeps) > 4 else (0, 0)
-    sm = gputo.Solver(nelx, nely, nelz, nl, precision=2, l0_kernel=kernel, mixed_npre0=sweeps[0],
-                      mixed_npost0=sweeps[1], mixed_npre1=sweeps[0], mixed_npost1=sweeps[1],
+    if len(sweeps) == 8:
+        s0, s1, below, below3 = sweeps[0:2], sweeps[2:4], sweeps[4:6], sweeps[6:8]
+    else:
+        s0 = s1 = sweeps[0:2]
+        below = sweeps[2:4] if len(sweeps) > 2 else (0, 0)
+        below3 = sweeps[4:6] if len(sweeps) > 4 else (0, 0)
+    sm = gputo.Solver(nelx, nely, nelz, nl, precision=2, l0_kernel=kernel, mixed_npre0=s0[0],
+                      mixed_npost0=s0[1], mixed_npre1=s1[0], mixed_npost1=s1[1],
                       mixed_npre2=below[0], mixed_npost2=below[1], mixed_npre3=below3[0], mixed_npost3=below3[1])
     Um, rm, nm, rc = sm.solve(Ke, F, CX, fixed, 1e-10, 300)
     print("iterations", n64, nm, flush=True)
